@@ -1,0 +1,254 @@
+"""ctypes front end to the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Two libraries share one calling convention:
+
+* ``liboracle.so``           -- this repository's plain-C restatement
+                                (oracle/ctsim_oracle.c)
+* ``_ref/libctsimref.so``    -- the CTSim reference itself, compiled from its
+                                own sources by oracle/Makefile, driven through
+                                oracle/ref_harness.cpp
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py may import this module.  The product package ``ctsim_amd`` never
+does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libctsimref.so")
+REF_SRC = "/root/reference"
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+# element type ids, include/phantom.h:33-38
+ELEM_TYPES = {"rectangle": 1, "triangle": 2, "ellipse": 3, "sector": 4, "segment": 5}
+# geometry ids, libctsim/scanner.cpp:29-33
+GEOMETRY_IDS = {"parallel": 0, "equiangular": 1, "equilinear": 2}
+
+
+def build(ref: bool | None = None) -> None:
+    """Compile liboracle.so, and the reference library when its sources exist."""
+    subprocess.check_call(["make", "-s", "-C", HERE, "oracle"])
+    if ref is None:
+        ref = os.path.isdir(REF_SRC)
+    if ref:
+        subprocess.check_call(["make", "-s", "-j8", "-C", HERE, "ref", f"REF={REF_SRC}"])
+
+
+def _bind(lib: C.CDLL, prefix: str) -> C.CDLL:
+    c = C.c_char_p
+    i = C.c_int
+    d = C.c_double
+    vp = C.c_void_p
+    f = getattr(lib, f"{prefix}_pipeline")
+    f.restype = i
+    f.argtypes = [vp, vp, i, c, c, i, i, i, i, d, d, d, d, d, c, d, c, i, c, c, i, c, i, i,
+                  vp, vp, vp, vp, vp, vp, vp]
+    f = getattr(lib, f"{prefix}_filter")
+    f.restype = i
+    f.argtypes = [c, c, d, d, i, d, c, c, i, i, i, d, d, vp, i, vp, vp, i, vp, vp]
+    f = getattr(lib, f"{prefix}_backproject")
+    f.restype = i
+    f.argtypes = [vp, vp, i, c, c, i, i, i, i, d, d, d, d, d, c, c, i, i, i, vp, vp, vp, i, vp]
+    f = getattr(lib, f"{prefix}_stats")
+    f.restype = i
+    f.argtypes = [i, i, vp, vp, vp]
+    getattr(lib, f"{prefix}_last_error").restype = C.c_char_p
+    return lib
+
+
+_libs: dict[str, C.CDLL] = {}
+
+
+def lib(kind: str = "oracle") -> C.CDLL:
+    """kind = 'oracle' (C restatement) or 'ref' (reference built from source)."""
+    if kind not in _libs:
+        path = ORACLE_SO if kind == "oracle" else REF_SO
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (run oracle.build())")
+        _libs[kind] = _bind(C.CDLL(path), "orc" if kind == "oracle" else "ref")
+    return _libs[kind]
+
+
+def have_ref() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _b(s):
+    return None if s is None else s.encode()
+
+
+@dataclass
+class Phantom:
+    """A phantom: optional built-in name plus explicit elements in definition
+    order, (type, cx, cy, u, v, rot_deg, atten) as in the .phm file format
+    (libctsim/phantom.cpp:184-213)."""
+    builtin: str | None = None
+    elems: list = field(default_factory=list)
+
+    def arrays(self):
+        n = len(self.elems)
+        t = np.array([ELEM_TYPES[e[0]] for e in self.elems] or [0], dtype=np.int32)
+        p = np.array([e[1:] for e in self.elems] or [[0.0] * 6], dtype=np.float64).reshape(-1, 6)
+        return t, np.ascontiguousarray(p), n
+
+
+@dataclass
+class ScanSpec:
+    """Scanner arguments, libctsim/scanner.cpp:86-91 (rotLen in radians)."""
+    geometry: str = "parallel"
+    nDet: int = 65
+    nView: int = 60
+    offsetView: int = 0
+    nSample: int = 1
+    rotLen: float = np.pi
+    focalRatio: float = 2.0
+    centerRatio: float = 2.0
+    viewRatio: float = 1.0
+    scanRatio: float = 1.0
+
+
+@dataclass
+class ReconSpec:
+    """Reconstructor arguments, libctsim/reconstruct.cpp:50-54 (pjrec defaults
+    tools/pjrec.cpp:128-137)."""
+    filter: str = "abs_bandlimit"
+    filterParam: float = -1.0
+    method: str = "convolution"
+    zeropad: int = 1
+    generation: str = "direct"
+    interp: str = "linear"
+    interpFactor: int = 1
+    backproj: str = "idiff"
+    nx: int = 64
+    ny: int = 64
+
+
+def pipeline(kind: str, phm: Phantom, scan: ScanSpec, rec: ReconSpec | None = None,
+             want_recon: bool = False):
+    """Run phantom -> projections -> filter -> backprojection on the CPU.
+
+    Returns a dict with sino [nView, nDet] f32, angles [nView] f64,
+    filtered [nView, nDet] f64, image [nx, ny] f32 (column-major layout),
+    info (16 geometry numbers) and timings (raysum, filter, backproject s).
+    """
+    L = lib(kind)
+    pre = "orc" if kind == "oracle" else "ref"
+    t, p, n = phm.arrays()
+    sino = np.zeros((scan.nView, scan.nDet), np.float32)
+    ang = np.zeros(scan.nView, np.float64)
+    info = np.zeros(16, np.float64)
+    tim = np.zeros(3, np.float64)
+    out = {"sino": sino, "angles": ang, "info": info, "timings": tim}
+    if rec is None:
+        rc = getattr(L, f"{pre}_pipeline")(
+            _ptr(t), _ptr(p), n, _b(phm.builtin), _b(scan.geometry), scan.nDet, scan.nView,
+            scan.offsetView, scan.nSample, scan.rotLen, scan.focalRatio, scan.centerRatio,
+            scan.viewRatio, scan.scanRatio, None, 0.0, None, 0, None, None, 1, None, 0, 0,
+            _ptr(sino), _ptr(ang), None, None, None, _ptr(info), _ptr(tim))
+    else:
+        nf = scan.nDet * rec.interpFactor
+        filt = np.zeros((scan.nView, nf), np.float64)
+        img = np.zeros((rec.nx, rec.ny), np.float32)
+        img2 = np.zeros((rec.nx, rec.ny), np.float32) if want_recon else None
+        rc = getattr(L, f"{pre}_pipeline")(
+            _ptr(t), _ptr(p), n, _b(phm.builtin), _b(scan.geometry), scan.nDet, scan.nView,
+            scan.offsetView, scan.nSample, scan.rotLen, scan.focalRatio, scan.centerRatio,
+            scan.viewRatio, scan.scanRatio, _b(rec.filter), rec.filterParam, _b(rec.method),
+            rec.zeropad, _b(rec.generation), _b(rec.interp), rec.interpFactor, _b(rec.backproj),
+            rec.nx, rec.ny, _ptr(sino), _ptr(ang), _ptr(filt), _ptr(img), _ptr(img2),
+            _ptr(info), _ptr(tim))
+        out.update(filtered=filt, image=img, image_recon=img2)
+    if rc:
+        raise RuntimeError(f"{kind} pipeline failed ({rc}): "
+                           f"{getattr(L, pre + '_last_error')().decode()}")
+    return out
+
+
+def filter_views(kind: str, sino: np.ndarray, filter: str, method: str, bw: float, inc: float,
+                 param: float = -1.0, domain: str = "spatial", generation: str = "direct",
+                 zeropad: int = 1, preinterp: int = 1, geometry: int = 0,
+                 focal: float = 0.0, sdl: float = 0.0):
+    """ProcessSignal(...).filterSignal on every row of ``sino``; returns
+    (filtered [nView, n] f64, filter kernel f64, (min, max, inc))."""
+    L = lib(kind)
+    pre = "orc" if kind == "oracle" else "ref"
+    sino = np.ascontiguousarray(sino, dtype=np.float32)
+    nView, n = sino.shape
+    out = np.zeros((nView, n), np.float64)
+    cap = 1 << 16
+    kern = np.zeros(cap, np.float64)
+    nfp = C.c_int(0)
+    mmi = np.zeros(3, np.float64)
+    rc = getattr(L, f"{pre}_filter")(
+        _b(filter), _b(method), bw, inc, n, param, _b(domain), _b(generation), zeropad,
+        preinterp, geometry, focal, sdl, _ptr(sino), nView, _ptr(out), _ptr(kern), cap,
+        C.byref(nfp), _ptr(mmi))
+    if rc:
+        raise RuntimeError(f"{kind} filter failed: {getattr(L, pre + '_last_error')().decode()}")
+    return out, kern[: nfp.value].copy(), tuple(mmi)
+
+
+def backproject(kind: str, phm: Phantom, scan: ScanSpec, backproj: str, interp: str, nx: int,
+                ny: int, filtered: np.ndarray, angles: np.ndarray, post: bool = True,
+                roi=None, interpFactor: int = 1):
+    L = lib(kind)
+    pre = "orc" if kind == "oracle" else "ref"
+    t, p, n = phm.arrays()
+    img = np.zeros((nx, ny), np.float32)
+    filtered = np.ascontiguousarray(filtered, dtype=np.float64)
+    angles = np.ascontiguousarray(angles, dtype=np.float64)
+    roi_a = None if roi is None else np.ascontiguousarray(roi, dtype=np.float64)
+    rc = getattr(L, f"{pre}_backproject")(
+        _ptr(t), _ptr(p), n, _b(phm.builtin), _b(scan.geometry), scan.nDet, scan.nView,
+        scan.offsetView, scan.nSample, scan.rotLen, scan.focalRatio, scan.centerRatio,
+        scan.viewRatio, scan.scanRatio, _b(backproj), _b(interp), interpFactor, nx, ny,
+        _ptr(roi_a), _ptr(filtered), _ptr(angles), int(post), _ptr(img))
+    if rc:
+        raise RuntimeError(f"{kind} backproject failed: "
+                           f"{getattr(L, pre + '_last_error')().decode()}")
+    return img
+
+
+def stats(kind: str, ref_img: np.ndarray, img: np.ndarray):
+    """Herman d, r, e with ``ref_img`` as the reference (imagefile.cpp:185-240)."""
+    L = lib(kind)
+    pre = "orc" if kind == "oracle" else "ref"
+    a = np.ascontiguousarray(ref_img, dtype=np.float32)
+    b = np.ascontiguousarray(img, dtype=np.float32)
+    dre = np.zeros(3, np.float64)
+    getattr(L, f"{pre}_stats")(a.shape[0], a.shape[1], _ptr(a), _ptr(b), _ptr(dre))
+    return tuple(dre)
+
+
+def seeded_phantom(n_ellipses: int, seed: int, n_rects: int = 0, radius: float = 1.0) -> Phantom:
+    """Seeded random phantom with the distributions the benchmark configs name:
+    centres uniform in the unit disk, semi-axes U[0.02, 0.4], rotation
+    U[0, pi) (written in degrees, as the .phm format stores it), attenuation
+    +-U[0.01, 1]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    elems = []
+    for kind, count in (("ellipse", n_ellipses), ("rectangle", n_rects)):
+        for _ in range(count):
+            r = radius * np.sqrt(rng.uniform())
+            phi = 2 * np.pi * rng.uniform()
+            u, v = rng.uniform(0.02, 0.4, size=2)
+            rot = np.degrees(rng.uniform(0.0, np.pi))
+            att = rng.uniform(0.01, 1.0) * (1.0 if rng.uniform() < 0.5 else -1.0)
+            elems.append((kind, float(r * np.cos(phi)), float(r * np.sin(phi)), float(u),
+                          float(v), float(rot), float(att)))
+    return Phantom(None, elems)
