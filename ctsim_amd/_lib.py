@@ -1,0 +1,39 @@
+"""ctypes binding of libctsim_amd.so (the C ABI of include/ctsim_hip.h)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libctsim_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is not built; run `python -m ctsim_amd.build` (there is no CPU fallback)")
+
+lib = C.CDLL(LIB_PATH)
+
+_i, _d, _vp, _cp = C.c_int, C.c_double, C.c_void_p, C.c_char_p
+_pp = C.POINTER(C.c_void_p)
+
+# name: (restype, argtypes)
+SIGNATURES = {
+    "ctsim_ctx_create": (_i, [_pp, _i, _vp]),
+    "ctsim_ctx_destroy": (None, [_vp]),
+    "ctsim_ctx_set_stream": (_i, [_vp, _vp]),
+    "ctsim_ctx_stream": (_vp, [_vp]),
+    "ctsim_ctx_synchronize": (_i, [_vp]),
+    "ctsim_last_error": (_cp, []),
+    "ctsim_version": (_cp, []),
+    "ctsim_backprojector_create": (_i, [_vp, _pp, _i, _i, _d, _i, _d, _d, _d, _cp, _cp, _i, _i,
+                                        _i, _vp]),
+    "ctsim_backprojector_destroy": (None, [_vp]),
+    "ctsim_backproject_views": (_i, [_vp, _vp, _i, _vp, _vp, _i, _vp]),
+    "ctsim_backproject_postprocess": (_i, [_vp, _vp, _vp]),
+    "ctsim_backprojector_info": (_i, [_vp, _vp]),
+}
+
+for _name, (_res, _args) in SIGNATURES.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args

@@ -1,0 +1,181 @@
+// Host side of the backprojector plan.  Compiled with g++ -O2 -ffp-contract=off
+// so every FP64 table comes out bit-identical to the reference's own.
+#include <strings.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "bp_plan.h"
+
+namespace ctsim {
+
+namespace {
+const char* const kBackprojNames[] = {"trig", "table", "diff", "idiff"};   // backprojectors.cpp:34-40
+const char* const kInterpNames[] = {"nearest", "linear", "cubic"};        // backprojectors.cpp:64-76
+
+int lookup(const char* name, const char* const* table, int n) {
+  if (!name) return -1;
+  for (int i = 0; i < n; i++)
+    if (strcasecmp(name, table[i]) == 0) return i;
+  return -1;
+}
+
+// nearest<long>, include/ctsupport.h:192-194
+long nearest_long(double x) { return x > 0 ? static_cast<long>(x + 0.5) : static_cast<long>(x - 0.5); }
+}  // namespace
+
+// Backprojector::initBackprojector + Backproject ctor + BackprojectTable and
+// BackprojectDiff ctors, backprojectors.cpp:135-180, 264-319, 413-429, 491-502.
+int bp_plan_init(BpPlan& p, int geometry, int nDet, double detInc, int nView,
+                 double viewDiameter, double F, double sdl, const char* backprojName,
+                 const char* interpName, int interpFactor, int nx, int ny, const double* roi) {
+  p.family = lookup(backprojName, kBackprojNames, 4);
+  if (p.family < 0)
+    return fail(CTSIM_ERR_INVALID,
+                std::string("Invalid backprojection name ") + (backprojName ? backprojName : "(null)"));
+  p.interp = lookup(interpName, kInterpNames, 3);
+  if (p.interp < 0)
+    return fail(CTSIM_ERR_INVALID,
+                std::string("Invalid interpolation name ") + (interpName ? interpName : "(null)"));
+  if (geometry == GEOM_EQUILINEAR) p.kind = KIND_EQUILINEAR;
+  else if (geometry == GEOM_EQUIANGULAR) p.kind = KIND_EQUIANGULAR;
+  else if (geometry == GEOM_PARALLEL) p.kind = p.family;
+  else
+    return fail(CTSIM_ERR_INVALID,
+                "Unable to select a backprojection method [Backprojector::initBackprojector]");
+  if (nDet < 2 || nView < 1 || nx < 1 || ny < 1)
+    return fail(CTSIM_ERR_INVALID, "backprojector: nDet >= 2, nView >= 1 and nx, ny >= 1 required");
+  if (interpFactor != 1)
+    return fail(CTSIM_ERR_UNSUPPORTED,
+                "backprojector: interpolation factor other than 1 needs frequency preinterpolation, "
+                "which the reference compiles out (backprojectors.cpp:64-76)");
+  p.geom = geometry;
+  p.interpFactor = interpFactor;
+  p.detInc = detInc;
+  p.nDet = nDet;
+  p.nView = nView;
+  p.iDetCenter = (nDet - 1) / 2;
+  p.rotScale = geometry == GEOM_PARALLEL ? kPi / nView : (2 * kPi) / nView;
+  p.nx = nx;
+  p.ny = ny;
+  p.viewDiameter = viewDiameter;
+  double phmLen = viewDiameter / kSqrt2;  // include/projections.h:165
+  p.xMin = -phmLen / 2;
+  p.xMax = p.xMin + phmLen;
+  p.yMin = -phmLen / 2;
+  p.yMax = p.yMin + phmLen;
+  if (roi) {  // ReconstructionROI member order: xMin, yMin, xMax, yMax
+    if (roi[0] > p.xMin) p.xMin = roi[0];
+    if (roi[2] < p.xMax) p.xMax = roi[2];
+    if (roi[1] > p.yMin) p.yMin = roi[1];
+    if (roi[3] < p.yMax) p.yMax = roi[3];
+    if (p.xMin > p.xMax) std::swap(p.xMin, p.xMax);
+    if (p.yMin > p.yMax) std::swap(p.yMin, p.yMax);
+  }
+  p.xInc = (p.xMax - p.xMin) / nx;
+  p.yInc = (p.yMax - p.yMin) / ny;
+  p.F = F;
+  p.sdl = sdl;
+
+  // Pixel-centre tables by repeated addition (backprojectors.cpp:374-377, 424-428).
+  p.xs.resize(nx);
+  p.ys.resize(ny);
+  {
+    double x = p.xMin + p.xInc / 2;
+    for (int ix = 0; ix < nx; x += p.xInc, ix++) p.xs[ix] = static_cast<float>(x);
+    double y = p.yMin + p.yInc / 2;
+    for (int iy = 0; iy < ny; y += p.yInc, iy++) p.ys[iy] = static_cast<float>(y);
+  }
+  if (p.kind == KIND_TRIG || p.kind == KIND_TABLE || p.kind == KIND_EQUIANGULAR ||
+      p.kind == KIND_EQUILINEAR) {
+    // BackprojectTable ctor; BackprojectTrig evaluates the same r and phi per
+    // view from the same repeated-addition x, y, so it shares the table.
+    p.r.resize(static_cast<size_t>(nx) * ny);
+    p.phi.resize(static_cast<size_t>(nx) * ny);
+    double x, y;
+    int ix, iy;
+    for (x = p.xMin + p.xInc / 2, ix = 0; ix < nx; x += p.xInc, ix++)
+      for (y = p.yMin + p.yInc / 2, iy = 0; iy < ny; y += p.yInc, iy++) {
+        p.r[static_cast<size_t>(ix) * ny + iy] = sqrt(x * x + y * y);
+        p.phi[static_cast<size_t>(ix) * ny + iy] = atan2(y, x);
+      }
+  }
+  if (p.kind == KIND_DIFF || p.kind == KIND_IDIFF) {
+    double x = p.xMin + p.xInc / 2;
+    double y = p.yMin + p.yInc / 2;
+    p.start_r = sqrt(x * x + y * y);
+    p.start_phi = atan2(y, x);
+  }
+  p.windowF64 = bp_window_bound(p, p.tileX, p.tileY);
+  p.windowF32 = p.windowF64;
+  return CTSIM_OK;
+}
+
+// Per-view constants of the FP64 path; the same expressions as
+// BackprojectDiff::BackprojectView (backprojectors.cpp:523-528) and
+// BackprojectIntDiff::BackprojectView (593-597).
+void bp_view_consts_f64(const BpPlan& p, const double* angles, int nView, ViewConstF64* out) {
+  const long scale = 1L << 32;
+  for (int v = 0; v < nView; v++) {
+    double theta = angles[v];
+    ViewConstF64& c = out[v];
+    c.theta = theta;
+    c.a = c.b = c.c = 0;
+    if (p.kind == KIND_DIFF) {
+      c.a = p.xInc * cos(theta) / p.detInc;
+      c.b = p.yInc * sin(theta) / p.detInc;
+      c.c = p.iDetCenter + p.start_r * cos(theta - p.start_phi) / p.detInc;
+    } else if (p.kind == KIND_IDIFF) {
+      long dx = nearest_long(p.xInc * cos(theta) / p.detInc * scale);
+      long dy = nearest_long(p.yInc * sin(theta) / p.detInc * scale);
+      long cs = nearest_long((p.start_r * cos(theta - p.start_phi) / p.detInc + p.iDetCenter) * scale);
+      memcpy(&c.a, &dx, 8);
+      memcpy(&c.b, &dy, 8);
+      memcpy(&c.c, &cs, 8);
+    }
+  }
+}
+
+// Per-view constants of the mixed path: FP64 trig on the host, rounded once.
+void bp_view_consts_f32(const BpPlan& p, const double* angles, int nView, ViewConstF32* out) {
+  for (int v = 0; v < nView; v++) {
+    double c = cos(angles[v]), s = sin(angles[v]);
+    if (p.geom == GEOM_PARALLEL) {
+      out[v].ca = static_cast<float>(c / p.detInc);
+      out[v].sa = static_cast<float>(s / p.detInc);
+    } else {
+      out[v].ca = static_cast<float>(c);
+      out[v].sa = static_cast<float>(s);
+    }
+    out[v].pad0 = out[v].pad1 = 0.f;
+  }
+}
+
+// Upper bound on the detector positions one tile of tileX x tileY pixels can
+// touch in one view, plus the interpolation taps and a rounding margin.
+// Returns 0 when a full detector row must be staged instead.
+int bp_window_bound(const BpPlan& p, int tileX, int tileY) {
+  const double ax = (tileX - 1) * p.xInc, ay = (tileY - 1) * p.yInc;
+  const double diag = sqrt(ax * ax + ay * ay);
+  double span;
+  if (p.geom == GEOM_PARALLEL) {
+    span = diag / p.detInc;
+  } else {
+    double rx = fmax(fabs(p.xMin), fabs(p.xMax)), ry = fmax(fabs(p.yMin), fabs(p.yMax));
+    double R = sqrt(rx * rx + ry * ry);
+    if (!(p.F > 1.05 * R)) return 0;  // source near or inside the image: no monotone window
+    double D = p.F - R;
+    double ang = 2 * asin(fmin(1.0, diag / (2 * D)));
+    if (p.geom == GEOM_EQUIANGULAR) {
+      span = ang / p.detInc;
+    } else {
+      double s = R / p.F;
+      span = p.sdl * ang / ((1 - s * s) * p.detInc);
+    }
+  }
+  double w = ceil(span) + 10;
+  if (!(w < p.nDet + 8)) return 0;
+  return static_cast<int>(w);
+}
+
+}  // namespace ctsim

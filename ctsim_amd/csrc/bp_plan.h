@@ -1,0 +1,67 @@
+// Backprojector plan: host-side geometry of one (Projections, ImageFile,
+// backprojector, interpolation) combination and its device tables.
+#pragma once
+
+#include "ctsim_internal.h"
+
+namespace ctsim {
+
+// Which kernel body a plan runs (the reference's class, with the fan-beam
+// geometries split by geometry).
+enum BpKind {
+  KIND_TRIG = 0,
+  KIND_TABLE = 1,
+  KIND_DIFF = 2,
+  KIND_IDIFF = 3,
+  KIND_EQUIANGULAR = 4,
+  KIND_EQUILINEAR = 5,
+};
+
+// Per-view constants of the FP64 path (one struct per view, 32 bytes).
+struct ViewConstF64 {
+  double theta;     // view angle (Trig, Table, fan)
+  double a, b;      // Diff: det_dx, det_dy       IntDiff: bit patterns of int64 det_dx, det_dy
+  double c;         // Diff: detPosColStart       IntDiff: bit pattern of int64 colStart
+};
+
+// Per-view constants of the mixed path (16 bytes).
+struct ViewConstF32 {
+  float ca, sa;     // parallel: cos/detInc, sin/detInc; fan: cos, sin
+  float pad0, pad1;
+};
+
+struct BpPlan {
+  int geom = 0, family = 0, kind = 0, interp = 0, interpFactor = 1;
+  int nDet = 0, nView = 0, nx = 0, ny = 0, iDetCenter = 0;
+  double detInc = 0, rotScale = 0, xMin = 0, xMax = 0, yMin = 0, yMax = 0, xInc = 0, yInc = 0;
+  double F = 0, sdl = 0, start_r = 0, start_phi = 0, viewDiameter = 0;
+  // Host tables (built exactly as the reference builds them).
+  std::vector<double> r, phi;       // [nx*ny] column-major, Table/fan/Trig
+  std::vector<float> xs, ys;        // pixel-centre coordinates (mixed path)
+  // Device copies.
+  DevBuf d_rphi;                    // r then phi, doubles
+  DevBuf d_xy;                      // xs then ys, floats
+  // Launch shape.
+  int tileX = 16, tileY = 16;       // pixels per block along ix, iy
+  int windowF64 = 0, windowF32 = 0; // detector window per (tile, view); 0 = full row
+};
+
+// Host side (bp_host.cpp).
+int bp_plan_init(BpPlan& p, int geometry, int nDet, double detInc, int nView,
+                 double viewDiameter, double F, double sdl, const char* backprojName,
+                 const char* interpName, int interpFactor, int nx, int ny, const double* roi);
+void bp_view_consts_f64(const BpPlan& p, const double* angles, int nView, ViewConstF64* out);
+void bp_view_consts_f32(const BpPlan& p, const double* angles, int nView, ViewConstF32* out);
+int bp_window_bound(const BpPlan& p, int tileX, int tileY);
+
+// Device side (backproject.hip).
+int bp_upload_tables(BpPlan& p);
+int bp_launch(ctsim_ctx* ctx, const BpPlan& p, int precision, const void* d_filtered,
+              const double* h_angles, int nView, float* d_image);
+int bp_launch_post(ctsim_ctx* ctx, const BpPlan& p, float* d_image);
+
+}  // namespace ctsim
+
+struct ctsim_backprojector {
+  ctsim::BpPlan plan;
+};

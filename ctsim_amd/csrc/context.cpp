@@ -1,0 +1,147 @@
+// Context, error state and device-buffer helpers of libctsim_amd.
+#include <cstdio>
+#include <cstring>
+
+#include "ctsim_internal.h"
+
+namespace ctsim {
+
+static thread_local std::string t_lastError;
+
+void set_error(const std::string& msg) { t_lastError = msg; }
+
+int fail(int code, const std::string& msg) {
+  t_lastError = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  char buf[512];
+  snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e),
+           what, file, line);
+  t_lastError = buf;
+  return CTSIM_ERR_HIP;
+}
+
+int DevBuf::reserve(size_t n) {
+  if (n <= bytes && ptr) return CTSIM_OK;
+  release();
+  size_t want = n < 256 ? 256 : n;
+  CTSIM_HIP(hipMalloc(&ptr, want));
+  bytes = want;
+  return CTSIM_OK;
+}
+
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+int PinnedStage::reserve(size_t n, hipStream_t s) {
+  if (!done) CTSIM_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+  // The previous upload out of this area must have finished before the host
+  // writes into it again.
+  CTSIM_HIP(hipEventSynchronize(done));
+  if (n <= bytes && ptr) return CTSIM_OK;
+  if (ptr) (void)hipHostFree(ptr);
+  ptr = nullptr;
+  size_t want = n < 4096 ? 4096 : n;
+  CTSIM_HIP(hipHostMalloc(&ptr, want, hipHostMallocDefault));
+  bytes = want;
+  (void)s;
+  return CTSIM_OK;
+}
+
+int PinnedStage::upload(DevBuf& dst, size_t n, hipStream_t s) {
+  CTSIM_CHECK(dst.reserve(n));
+  CTSIM_HIP(hipMemcpyAsync(dst.ptr, ptr, n, hipMemcpyHostToDevice, s));
+  CTSIM_HIP(hipEventRecord(done, s));
+  return CTSIM_OK;
+}
+
+void PinnedStage::release() {
+  if (done) {
+    (void)hipEventSynchronize(done);
+    (void)hipEventDestroy(done);
+  }
+  if (ptr) (void)hipHostFree(ptr);
+  ptr = nullptr;
+  done = nullptr;
+  bytes = 0;
+}
+
+}  // namespace ctsim
+
+int ctsim_ctx::activate() const {
+  CTSIM_HIP(hipSetDevice(device));
+  return CTSIM_OK;
+}
+
+using namespace ctsim;
+
+extern "C" {
+
+const char* ctsim_last_error(void) { return t_lastError.c_str(); }
+
+const char* ctsim_version(void) { return "ctsim_amd 0.1 (gfx950)"; }
+
+int ctsim_ctx_create(ctsim_ctx** out, int device, void* stream) {
+  if (!out) return fail(CTSIM_ERR_INVALID, "ctsim_ctx_create: out is NULL");
+  *out = nullptr;
+  int n = 0;
+  CTSIM_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n)
+    return fail(CTSIM_ERR_INVALID, "ctsim_ctx_create: no such device");
+  auto* c = new ctsim_ctx();
+  c->device = device;
+  int rc = c->activate();
+  if (rc) { delete c; return rc; }
+  if (stream) {
+    c->stream = static_cast<hipStream_t>(stream);
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate", __FILE__, __LINE__); }
+    c->ownsStream = true;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->numCUs = prop.multiProcessorCount;
+  *out = c;
+  return CTSIM_OK;
+}
+
+void ctsim_ctx_destroy(ctsim_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->ownsStream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int ctsim_ctx_set_stream(ctsim_ctx* ctx, void* stream) {
+  if (!ctx) return fail(CTSIM_ERR_INVALID, "ctsim_ctx_set_stream: ctx is NULL");
+  CTSIM_CHECK(ctx->activate());
+  if (ctx->ownsStream) {
+    CTSIM_HIP(hipStreamSynchronize(ctx->stream));
+    CTSIM_HIP(hipStreamDestroy(ctx->stream));
+    ctx->ownsStream = false;
+  }
+  if (stream) {
+    ctx->stream = static_cast<hipStream_t>(stream);
+  } else {
+    CTSIM_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->ownsStream = true;
+  }
+  return CTSIM_OK;
+}
+
+void* ctsim_ctx_stream(const ctsim_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int ctsim_ctx_synchronize(ctsim_ctx* ctx) {
+  if (!ctx) return fail(CTSIM_ERR_INVALID, "ctsim_ctx_synchronize: ctx is NULL");
+  CTSIM_CHECK(ctx->activate());
+  CTSIM_HIP(hipStreamSynchronize(ctx->stream));
+  return CTSIM_OK;
+}
+
+}  // extern "C"

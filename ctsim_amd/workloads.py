@@ -1,0 +1,124 @@
+"""Synthetic workloads: seeded phantoms and the scan/reconstruction settings of
+the benchmark configurations.
+
+Phantom elements are tuples ``(type, cx, cy, u, v, rot_deg, atten)`` in
+definition order, the order of the reference's phantom file format
+(libctsim/phantom.cpp:184-213, ``type cx cy u v rot atten`` with the rotation
+in degrees).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+SEED_BASE = 0xC7511
+
+
+def config_seed(config: int, slice_index: int = 0) -> int:
+    """Seed of one slice of one configuration: 0xC7511 + 1000*config + slice."""
+    return SEED_BASE + 1000 * config + slice_index
+
+
+def seeded_elements(n_ellipses: int, seed: int, n_rects: int = 0, radius: float = 1.0):
+    """Random elements with the configs' distributions: centres uniform in the
+    disk of ``radius``, semi-axes U[0.02, 0.4], rotation U[0, pi) (stored in
+    degrees), attenuation +-U[0.01, 1]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    elems = []
+    for kind, count in (("ellipse", n_ellipses), ("rectangle", n_rects)):
+        for _ in range(count):
+            r = radius * np.sqrt(rng.uniform())
+            phi = 2 * np.pi * rng.uniform()
+            u, v = rng.uniform(0.02, 0.4, size=2)
+            rot = np.degrees(rng.uniform(0.0, np.pi))
+            att = rng.uniform(0.01, 1.0) * (1.0 if rng.uniform() < 0.5 else -1.0)
+            elems.append((kind, float(r * np.cos(phi)), float(r * np.sin(phi)), float(u),
+                          float(v), float(rot), float(att)))
+    return elems
+
+
+def mixed_elements(seed: int):
+    """A small phantom exercising every element type (ellipse, rectangle,
+    triangle, sector, segment)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    elems = seeded_elements(10, seed, n_rects=3)
+    for kind in ("triangle", "sector", "segment", "triangle", "sector", "segment"):
+        cx, cy = rng.uniform(-0.6, 0.6, size=2)
+        u, v = rng.uniform(0.1, 0.4, size=2)
+        elems.append((kind, float(cx), float(cy), float(u), float(v),
+                      float(rng.uniform(-180, 180)), float(rng.uniform(-1, 1))))
+    return elems
+
+
+@dataclass
+class SliceSpec:
+    """One slice: phantom plus Scanner and Reconstructor arguments
+    (Scanner ctor libctsim/scanner.cpp:86-91, Reconstructor ctor
+    libctsim/reconstruct.cpp:50-54)."""
+    elems: list = field(default_factory=list)
+    builtin: str | None = None
+    geometry: str = "parallel"
+    nDet: int = 367
+    nView: int = 320
+    nSample: int = 1
+    rotLen: float = np.pi
+    focalRatio: float = 2.0
+    centerRatio: float = 2.0
+    viewRatio: float = 1.0
+    scanRatio: float = 1.0
+    filter: str = "abs_bandlimit"
+    filterParam: float = -1.0
+    method: str = "convolution"
+    zeropad: int = 1
+    generation: str = "direct"
+    interp: str = "linear"
+    backproj: str = "idiff"
+    nx: int = 256
+    ny: int = 256
+
+
+def config1() -> SliceSpec:
+    """Built-in Shepp-Logan (11 ellipses as coded), parallel, 367 det x 320
+    views over pi, 2 rays per detector, abs_bandlimit by convolution, IntDiff
+    linear into 256x256."""
+    return SliceSpec(builtin="shepp-logan", geometry="parallel", nDet=367, nView=320, nSample=2,
+                     rotLen=np.pi, filter="abs_bandlimit", method="convolution", interp="linear",
+                     backproj="idiff", nx=256, ny=256)
+
+
+def config2() -> SliceSpec:
+    """64 seeded ellipses, equiangular fan beam, 1024 det x 1440 views over 2pi,
+    focal-length ratio 2, Hanning-windowed ramp by FFT (zeropad 1 -> 2048
+    points), cubic interpolation into 512x512."""
+    return SliceSpec(elems=seeded_elements(64, config_seed(2)), geometry="equiangular",
+                     nDet=1024, nView=1440, nSample=1, rotLen=2 * np.pi, focalRatio=2.0,
+                     filter="abs_hanning", method="fft", interp="cubic", backproj="idiff",
+                     nx=512, ny=512)
+
+
+def config3() -> SliceSpec:
+    """256 seeded ellipses + 32 rectangles, parallel, 2048 det x 2880 views over
+    pi, 'shepp' spatial filter by convolution, IntDiff linear into 2048x2048."""
+    return SliceSpec(elems=seeded_elements(256, config_seed(3), n_rects=32), geometry="parallel",
+                     nDet=2048, nView=2880, nSample=1, rotLen=np.pi, filter="shepp",
+                     method="convolution", interp="linear", backproj="idiff", nx=2048, ny=2048)
+
+
+def config4(n_slices: int = 256):
+    """Uneven batch: per slice, image size from {128,...,1024}^2, detectors
+    U[183, 1467], views U[90, 1440], geometry 1/3 each, 8-128 ellipses."""
+    out = []
+    for s in range(n_slices):
+        rng = np.random.Generator(np.random.PCG64(config_seed(4, s)))
+        nx = int(rng.choice([128, 256, 384, 512, 768, 1024]))
+        nDet = int(rng.integers(183, 1468))
+        nView = int(rng.integers(90, 1441))
+        geom = ["parallel", "equilinear", "equiangular"][int(rng.integers(0, 3))]
+        ne = int(rng.integers(8, 129))
+        out.append(SliceSpec(elems=seeded_elements(ne, config_seed(4, s) + 500000),
+                             geometry=geom, nDet=nDet, nView=nView, nSample=1,
+                             rotLen=np.pi if geom == "parallel" else 2 * np.pi,
+                             filter="abs_bandlimit", method="convolution", interp="linear",
+                             backproj="idiff", nx=nx, ny=nx))
+    return out

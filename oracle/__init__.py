@@ -235,20 +235,43 @@ def stats(kind: str, ref_img: np.ndarray, img: np.ndarray):
     return tuple(dre)
 
 
-def seeded_phantom(n_ellipses: int, seed: int, n_rects: int = 0, radius: float = 1.0) -> Phantom:
-    """Seeded random phantom with the distributions the benchmark configs name:
-    centres uniform in the unit disk, semi-axes U[0.02, 0.4], rotation
-    U[0, pi) (written in degrees, as the .phm format stores it), attenuation
-    +-U[0.01, 1]."""
-    rng = np.random.Generator(np.random.PCG64(seed))
-    elems = []
-    for kind, count in (("ellipse", n_ellipses), ("rectangle", n_rects)):
-        for _ in range(count):
-            r = radius * np.sqrt(rng.uniform())
-            phi = 2 * np.pi * rng.uniform()
-            u, v = rng.uniform(0.02, 0.4, size=2)
-            rot = np.degrees(rng.uniform(0.0, np.pi))
-            att = rng.uniform(0.01, 1.0) * (1.0 if rng.uniform() < 0.5 else -1.0)
-            elems.append((kind, float(r * np.cos(phi)), float(r * np.sin(phi)), float(u),
-                          float(v), float(rot), float(att)))
-    return Phantom(None, elems)
+def _bind_helpers(L: C.CDLL) -> C.CDLL:
+    d, i, vp, c = C.c_double, C.c_int, C.c_void_p, C.c_char_p
+    L.orc_frequency_response.restype = d
+    L.orc_frequency_response.argtypes = [i, d, d, d]
+    L.orc_spatial_response_analytic.restype = d
+    L.orc_spatial_response_analytic.argtypes = [i, d, d, d]
+    L.orc_spatial_response_calc.restype = d
+    L.orc_spatial_response_calc.argtypes = [i, d, d, d, i]
+    L.orc_integrate_simpson.restype = d
+    L.orc_integrate_simpson.argtypes = [d, d, vp, i]
+    L.orc_cubic_interpolate.restype = d
+    L.orc_cubic_interpolate.argtypes = [vp, i, d]
+    L.orc_add_zeropad_factor.restype = i
+    L.orc_add_zeropad_factor.argtypes = [i, i]
+    L.orc_raysum_line.restype = d
+    L.orc_raysum_line.argtypes = [vp, vp, i, c, d, d, d, d]
+    L.orc_phantom_extent.restype = i
+    L.orc_phantom_extent.argtypes = [vp, vp, i, c, vp]
+    return L
+
+
+def helpers() -> C.CDLL:
+    """The oracle's building blocks (closed-form checks)."""
+    L = lib("oracle")
+    if not getattr(L, "_helpers_bound", False):
+        _bind_helpers(L)
+        L._helpers_bound = True
+    return L
+
+
+def raysum_line(phm: Phantom, x1, y1, x2, y2) -> float:
+    t, p, n = phm.arrays()
+    return helpers().orc_raysum_line(_ptr(t), _ptr(p), n, _b(phm.builtin), x1, y1, x2, y2)
+
+
+def phantom_extent(phm: Phantom):
+    t, p, n = phm.arrays()
+    ext = np.zeros(4, np.float64)
+    helpers().orc_phantom_extent(_ptr(t), _ptr(p), n, _b(phm.builtin), _ptr(ext))
+    return ext
