@@ -4,22 +4,25 @@ filtering and backprojection hot path.
 The compute lives in ``libctsim_amd.so`` (hand-written HIP kernels behind the
 C ABI declared in include/ctsim_hip.h).  This package only loads that library
 and offers thin Python handles over it for tests and benchmarks; there is no
-Python or CPU fallback for any kernel.  Importing fails loudly when the
-library has not been built (``python -m ctsim_amd.build``).
+Python or CPU fallback for any kernel.  Touching any handle below loads the
+library and fails loudly when it has not been built
+(``python -m ctsim_amd.build``); the ``build`` and ``workloads`` submodules
+work without it.
 """
 from __future__ import annotations
 
-import os
+import importlib
 
-from ._lib import LIB_PATH, lib  # noqa: F401  (raises if the library is missing)
-from .api import (  # noqa: F401
-    PREC_F64,
-    PREC_MIXED,
-    Backprojector,
-    Context,
-    CtsimError,
-)
-
-__all__ = ["lib", "LIB_PATH", "Context", "Backprojector", "CtsimError", "PREC_F64", "PREC_MIXED"]
 __version__ = "0.1"
-HERE = os.path.dirname(os.path.abspath(__file__))
+
+_API = {"Context", "Backprojector", "ProcessSignal", "Phantom", "Scanner", "CtsimError",
+        "PREC_F64", "PREC_MIXED",
+        "check"}
+
+
+def __getattr__(name):
+    if name in ("lib", "LIB_PATH"):
+        return getattr(importlib.import_module("._lib", __name__), name)
+    if name in _API:
+        return getattr(importlib.import_module(".api", __name__), name)
+    raise AttributeError(name)

@@ -31,6 +31,20 @@ SIGNATURES = {
     "ctsim_backproject_views": (_i, [_vp, _vp, _i, _vp, _vp, _i, _vp]),
     "ctsim_backproject_postprocess": (_i, [_vp, _vp, _vp]),
     "ctsim_backprojector_info": (_i, [_vp, _vp]),
+    "ctsim_procsignal_create": (_i, [_vp, _pp, _cp, _cp, _d, _d, _i, _d, _cp, _cp, _i, _i, _i,
+                                     _d, _d]),
+    "ctsim_procsignal_destroy": (None, [_vp]),
+    "ctsim_procsignal_info": (_i, [_vp, _vp, _vp]),
+    "ctsim_procsignal_get_filter": (_i, [_vp, _vp, _i]),
+    "ctsim_filter_views": (_i, [_vp, _vp, _i, _vp, _i, _vp]),
+    "ctsim_phantom_create": (_i, [_pp, _cp]),
+    "ctsim_phantom_destroy": (None, [_vp]),
+    "ctsim_phantom_add_element": (_i, [_vp, _cp, _d, _d, _d, _d, _d, _d]),
+    "ctsim_phantom_extent": (_i, [_vp, _vp]),
+    "ctsim_scanner_create": (_i, [_pp, _vp, _cp, _i, _i, _i, _i, _d, _d, _d, _d, _d]),
+    "ctsim_scanner_destroy": (None, [_vp]),
+    "ctsim_scanner_info": (_i, [_vp, _vp]),
+    "ctsim_collect_projections": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp]),
 }
 
 for _name, (_res, _args) in SIGNATURES.items():

@@ -119,3 +119,140 @@ class Backprojector:
 
     def __del__(self):
         self.close()
+
+
+class ProcessSignal:
+    """Filter plan (libctsim/procsignal.cpp:82-442) and filterSignal over
+    batches of views (procsignal.cpp:544-629)."""
+
+    def __init__(self, ctx: Context, filter: str, method: str, bw: float, signalInc: float,
+                 n: int, param: float = -1.0, domain: str = "spatial",
+                 generation: str = "direct", zeropad: int = 1, preinterp: int = 1,
+                 geometry=0, focalLength: float = 0.0, sourceDetectorLength: float = 0.0):
+        self.ctx = ctx
+        g = GEOMETRY_IDS[geometry] if isinstance(geometry, str) else int(geometry)
+        h = C.c_void_p()
+        check(lib.ctsim_procsignal_create(
+            ctx.handle, C.byref(h), filter.encode(), method.encode(), bw, signalInc, n, param,
+            domain.encode(), generation.encode(), zeropad, preinterp, g, focalLength,
+            sourceDetectorLength), "ctsim_procsignal_create")
+        self.handle = h
+        self.n = n
+
+    def info(self):
+        nfp = C.c_int()
+        mmi = np.zeros(3, np.float64)
+        check(lib.ctsim_procsignal_info(self.handle, C.byref(nfp), _hptr(mmi)),
+              "ctsim_procsignal_info")
+        return nfp.value, tuple(mmi)
+
+    def filter_kernel(self):
+        nfp, _ = self.info()
+        out = np.zeros(nfp, np.float64)
+        lib.ctsim_procsignal_get_filter(self.handle, _hptr(out), nfp)
+        return out
+
+    def filter(self, sino, out, precision: int = PREC_F64):
+        """Filter every row of ``sino`` (device float32 [nView, n]) into ``out``
+        (device [nView, n], float64 for F64, float32 for MIXED)."""
+        import torch
+        want = torch.float64 if precision == PREC_F64 else torch.float32
+        if sino.dtype != torch.float32 or sino.shape[-1] != self.n:
+            raise TypeError("sino must be float32 [nView, n]")
+        if out.dtype != want or tuple(out.shape) != tuple(sino.shape):
+            raise TypeError(f"out must be {want} with the shape of sino")
+        check(lib.ctsim_filter_views(self.ctx.handle, self.handle, precision,
+                                     C.c_void_p(_dptr(sino)), sino.shape[0],
+                                     C.c_void_p(_dptr(out))), "ctsim_filter_views")
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.ctsim_procsignal_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+class Phantom:
+    """Phantom (libctsim/phantom.cpp): a built-in name and/or elements
+    (type, cx, cy, u, v, rot_deg, atten) in definition order."""
+
+    def __init__(self, builtin: str | None = None, elems=()):
+        h = C.c_void_p()
+        check(lib.ctsim_phantom_create(C.byref(h), builtin.encode() if builtin else None),
+              "ctsim_phantom_create")
+        self.handle = h
+        for e in elems:
+            self.add(*e)
+
+    def add(self, type: str, cx, cy, u, v, rot_deg, atten):
+        check(lib.ctsim_phantom_add_element(self.handle, type.encode(), cx, cy, u, v, rot_deg,
+                                            atten), "ctsim_phantom_add_element")
+
+    def extent(self):
+        e = np.zeros(4, np.float64)
+        lib.ctsim_phantom_extent(self.handle, _hptr(e))
+        return e
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.ctsim_phantom_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+SCANNER_INFO = ("detInc", "detStart", "rotInc", "viewDiameter", "focalLength",
+                "sourceDetectorLength", "fanBeamAngle", "phmLen", "detLen",
+                "centerDetectorLength", "scanDiameter", "geometry")
+
+
+class Scanner:
+    """Scanner geometry (libctsim/scanner.cpp:86-224) and
+    collectProjections (280-419)."""
+
+    def __init__(self, phantom: Phantom, geometry: str, nDet: int, nView: int,
+                 offsetView: int = 0, nSample: int = 1, rotLen: float = np.pi,
+                 focalRatio: float = 2.0, centerRatio: float = 2.0, viewRatio: float = 1.0,
+                 scanRatio: float = 1.0):
+        h = C.c_void_p()
+        check(lib.ctsim_scanner_create(C.byref(h), phantom.handle, geometry.encode(), nDet, nView,
+                                       offsetView, nSample, rotLen, focalRatio, centerRatio,
+                                       viewRatio, scanRatio), "ctsim_scanner_create")
+        self.handle = h
+        self.phantom = phantom
+        self.nDet, self.nView, self.offsetView = nDet, nView, offsetView
+        o = np.zeros(12, np.float64)
+        lib.ctsim_scanner_info(h, _hptr(o))
+        self.info = dict(zip(SCANNER_INFO, o))
+        self.geometry = geometry
+
+    def collect(self, ctx: Context, sino, precision: int = PREC_F64, iStartView: int = 0,
+                iNumViews: int | None = None, iOffsetView: int | None = None,
+                iStorageOffset: int | None = None):
+        """Ray sums into ``sino`` (device float32 [nView, nDet]); returns the
+        view angles (host float64)."""
+        import torch
+        if sino.dtype != torch.float32 or sino.shape[-1] != self.nDet:
+            raise TypeError("sino must be float32 [nView, nDet]")
+        n = self.nView if iNumViews is None else iNumViews
+        off = self.offsetView if iOffsetView is None else iOffsetView
+        store = iStartView if iStorageOffset is None else iStorageOffset
+        if store + n > sino.shape[0]:
+            raise ValueError("sinogram has too few rows")
+        ang = np.zeros(sino.shape[0], np.float64)
+        check(lib.ctsim_collect_projections(ctx.handle, self.handle, self.phantom.handle,
+                                            precision, iStartView, n, off, store,
+                                            C.c_void_p(_dptr(sino)), _hptr(ang)),
+              "ctsim_collect_projections")
+        return ang
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.ctsim_scanner_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,480 @@
+// Projection filtering kernels for gfx950 (ProcessSignal::filterSignal,
+// libctsim/procsignal.cpp:544-673).
+//
+// Sequential-sum kernels (FP64 convolution and the exact DFT of the
+// "fourier" / "fouier-table" methods): lanes run over views, so every lane of
+// a wave needs the same filter tap or twiddle at the same step (an LDS
+// broadcast or a scalar load) while its own sum runs in the reference's order.
+//
+// FFT kernels (the "fft" method, and every Fourier method in mixed
+// precision): one workgroup filters two views with one complex radix-2
+// Stockham transform in LDS: z = x_a + i x_b, Z = FFT(z), split Z into the two
+// real spectra by Hermitian symmetry, apply the method's spectral multipliers,
+// recombine, inverse FFT; the real part is view a, the imaginary part view b.
+#include <hip/hip_runtime.h>
+
+#include "filter_plan.h"
+
+#pragma clang fp contract(off)
+
+namespace ctsim {
+namespace {
+
+constexpr int kLanes = 64;        // views per wave
+constexpr int kWaves = 4;         // waves per block
+constexpr int kOutPerWave = 8;    // outputs per wave (independent sums)
+constexpr int kJChunk = 64;       // inputs staged per step
+constexpr int kKChunk = 32;       // spectrum entries staged per step (inverse DFT)
+
+// ---------------------------------------------------- exact convolution
+// out[v][i] = dx * sum_{j=0}^{n-1} x[v][j] * h[i - j + n - 1]  (procsignal.cpp:659-673)
+template <class TO>
+__global__ __launch_bounds__(kLanes * kWaves) void conv_seq_kernel(
+    const float* __restrict__ in, const double* __restrict__ w, const double* __restrict__ h,
+    TO* __restrict__ out, int nView, int n, double dx, int outStride) {
+  __shared__ double s_x[kJChunk][kLanes];
+  extern __shared__ double s_h[];  // 2n-1 taps
+  const int lane = threadIdx.x & (kLanes - 1), wave = threadIdx.x / kLanes;
+  const int v = blockIdx.y * kLanes + lane;
+  const int i0 = (blockIdx.x * kWaves + wave) * kOutPerWave;
+  for (int k = threadIdx.x; k < 2 * n - 1; k += blockDim.x) s_h[k] = h[k];
+  double acc[kOutPerWave];
+#pragma unroll
+  for (int o = 0; o < kOutPerWave; o++) acc[o] = 0.0;
+  for (int j0 = 0; j0 < n; j0 += kJChunk) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < kJChunk * kLanes; k += blockDim.x) {
+      const int jj = k / kLanes, l = k % kLanes;
+      const int j = j0 + jj, vv = blockIdx.y * kLanes + l;
+      double xv = 0.0;
+      if (j < n && vv < nView) {
+        xv = static_cast<double>(in[static_cast<size_t>(vv) * n + j]);
+        if (w) xv *= w[j];
+      }
+      s_x[jj][l] = xv;
+    }
+    __syncthreads();
+    const int jn = min(kJChunk, n - j0);
+    for (int jj = 0; jj < jn; jj++) {
+      const double xv = s_x[jj][lane];
+      const int j = j0 + jj;
+#pragma unroll
+      for (int o = 0; o < kOutPerWave; o++) {
+        const int i = i0 + o;
+        const int hk = i - j + n - 1;
+        if (i < n) acc[o] += xv * s_h[hk];
+      }
+    }
+  }
+  if (v < nView) {
+#pragma unroll
+    for (int o = 0; o < kOutPerWave; o++) {
+      const int i = i0 + o;
+      if (i < n) out[static_cast<size_t>(v) * outStride + i] = static_cast<TO>(acc[o] * dx);
+    }
+  }
+}
+
+// ----------------------------------------------------------- exact DFT
+// Forward: Z[v][k] = (1/N) sum_{j<n} x[v][j] * (c[kj] + i s[kj]), then
+// multiplied by filter[k]   (procsignal.cpp:570-576, 706-728, 778-801)
+__global__ __launch_bounds__(kLanes * kWaves) void dft_fwd_kernel(
+    const float* __restrict__ in, const double* __restrict__ w, const double* __restrict__ tc,
+    const double* __restrict__ ts, const double* __restrict__ filt, double2* __restrict__ Z,
+    int nView, int n, int N) {
+  __shared__ double s_x[kJChunk][kLanes];
+  const int lane = threadIdx.x & (kLanes - 1), wave = threadIdx.x / kLanes;
+  const int v = blockIdx.y * kLanes + lane;
+  const int k0 = (blockIdx.x * kWaves + wave) * kOutPerWave;
+  double sr[kOutPerWave], si[kOutPerWave];
+#pragma unroll
+  for (int o = 0; o < kOutPerWave; o++) sr[o] = si[o] = 0.0;
+  for (int j0 = 0; j0 < n; j0 += kJChunk) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < kJChunk * kLanes; k += blockDim.x) {
+      const int jj = k / kLanes, l = k % kLanes;
+      const int j = j0 + jj, vv = blockIdx.y * kLanes + l;
+      double xv = 0.0;
+      if (j < n && vv < nView) {
+        xv = static_cast<double>(in[static_cast<size_t>(vv) * n + j]);
+        if (w) xv *= w[j];
+      }
+      s_x[jj][l] = xv;
+    }
+    __syncthreads();
+    const int jn = min(kJChunk, n - j0);
+    for (int jj = 0; jj < jn; jj++) {
+      const double xv = s_x[jj][lane];
+      const int j = j0 + jj;
+#pragma unroll
+      for (int o = 0; o < kOutPerWave; o++) {
+        const int kk = k0 + o;
+        if (kk < N) {
+          const size_t t = static_cast<size_t>(kk) * j;
+          sr[o] += xv * tc[t];
+          si[o] += xv * ts[t];
+        }
+      }
+    }
+  }
+  if (v < nView) {
+#pragma unroll
+    for (int o = 0; o < kOutPerWave; o++) {
+      const int kk = k0 + o;
+      if (kk < N) {
+        double re = sr[o] / N, im = si[o] / N;
+        re *= filt[kk];
+        im *= filt[kk];
+        Z[static_cast<size_t>(v) * N + kk] = make_double2(re, im);
+      }
+    }
+  }
+}
+
+// Inverse: out[v][i] = sum_{k<N} Re(Z[k]) c[ik] - Im(Z[k]) s[ik], i < n
+// (procsignal.cpp:577-580, 756-774, 843-864)
+template <class TO>
+__global__ __launch_bounds__(kLanes * kWaves) void dft_inv_kernel(
+    const double2* __restrict__ Z, const double* __restrict__ tc, const double* __restrict__ ts,
+    TO* __restrict__ out, int nView, int n, int N, int outStride) {
+  __shared__ double2 s_z[kKChunk][kLanes];
+  const int lane = threadIdx.x & (kLanes - 1), wave = threadIdx.x / kLanes;
+  const int v = blockIdx.y * kLanes + lane;
+  const int i0 = (blockIdx.x * kWaves + wave) * kOutPerWave;
+  double acc[kOutPerWave];
+#pragma unroll
+  for (int o = 0; o < kOutPerWave; o++) acc[o] = 0.0;
+  for (int k0 = 0; k0 < N; k0 += kKChunk) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < kKChunk * kLanes; k += blockDim.x) {
+      const int kk = k / kLanes, l = k % kLanes;
+      const int vv = blockIdx.y * kLanes + l;
+      s_z[kk][l] = (k0 + kk < N && vv < nView) ? Z[static_cast<size_t>(vv) * N + k0 + kk]
+                                               : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int kn = min(kKChunk, N - k0);
+    for (int kk = 0; kk < kn; kk++) {
+      const double2 z = s_z[kk][lane];
+      const int k = k0 + kk;
+#pragma unroll
+      for (int o = 0; o < kOutPerWave; o++) {
+        const int i = i0 + o;
+        if (i < n) {
+          const size_t t = static_cast<size_t>(i) * k;
+          acc[o] += z.x * tc[t] - z.y * ts[t];
+        }
+      }
+    }
+  }
+  if (v < nView) {
+#pragma unroll
+    for (int o = 0; o < kOutPerWave; o++) {
+      const int i = i0 + o;
+      if (i < n) out[static_cast<size_t>(v) * outStride + i] = static_cast<TO>(acc[o]);
+    }
+  }
+}
+
+// ------------------------------------------------------------ LDS FFT
+template <class T>
+struct Cplx {
+  T x, y;
+};
+
+template <class T>
+__device__ __forceinline__ Cplx<T> cmul(Cplx<T> a, Cplx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// In-place radix-2 Stockham pass over z[0..N) held in LDS.  tw[k] =
+// exp(-2 pi i k / N) for k < N/2; dir = -1 forward, +1 inverse (conjugate
+// twiddles).  Every lane first reads its butterflies, then the block
+// synchronises, then writes: the autosort permutation makes the passes
+// write-after-read hazards otherwise.
+template <class T, int R>
+__device__ void stockham(Cplx<T>* z, const Cplx<T>* tw, int N, int logN, int dir) {
+  const int half = N >> 1;
+  for (int s = 0, Ls = 1; s < logN; s++, Ls <<= 1) {
+    Cplx<T> ya[R], yb[R];
+    int idx[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int t = threadIdx.x + r * blockDim.x;
+      idx[r] = -1;
+      if (t < half) {
+        const int k = t & (Ls - 1);
+        Cplx<T> a = z[t], b = z[t + half];
+        Cplx<T> w = tw[k * (half / Ls)];
+        if (dir > 0) w.y = -w.y;
+        const Cplx<T> wb = cmul(w, b);
+        ya[r] = {a.x + wb.x, a.y + wb.y};
+        yb[r] = {a.x - wb.x, a.y - wb.y};
+        idx[r] = ((t - k) << 1) + k;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (idx[r] >= 0) {
+        z[idx[r]] = ya[r];
+        z[idx[r] + Ls] = yb[r];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <class T, class TO, int R>
+__global__ __launch_bounds__(256) void fft_filter_kernel(
+    const float* __restrict__ in, const double* __restrict__ w, const T* __restrict__ hre,
+    const T* __restrict__ him, const Cplx<T>* __restrict__ twg, TO* __restrict__ out, int nView,
+    int n, int N, int logN, int outStride) {
+  extern __shared__ unsigned char smem_raw[];
+  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw);
+  Cplx<T>* tw = z + N;
+  const int va = 2 * blockIdx.x, vb = va + 1;
+  const bool hasB = vb < nView;
+  for (int k = threadIdx.x; k < N / 2; k += blockDim.x) tw[k] = twg[k];
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    T a = 0, b = 0;
+    if (j < n) {
+      double xa = static_cast<double>(in[static_cast<size_t>(va) * n + j]);
+      double xb = hasB ? static_cast<double>(in[static_cast<size_t>(vb) * n + j]) : 0.0;
+      if (w) {
+        xa *= w[j];
+        xb *= w[j];
+      }
+      a = static_cast<T>(xa);
+      b = static_cast<T>(xb);
+    }
+    z[j] = {a, b};
+  }
+  __syncthreads();
+  stockham<T, R>(z, tw, N, logN, -1);
+  // Split the two real spectra, apply the multipliers, recombine.
+  for (int k = threadIdx.x; k <= N / 2; k += blockDim.x) {
+    const int kn = (N - k) & (N - 1);
+    const Cplx<T> zk = z[k], zn = z[kn];
+    // X_a = (Z_k + conj Z_{N-k}) / 2, X_b = (Z_k - conj Z_{N-k}) / (2i)
+    const T ar = (zk.x + zn.x) * T(0.5), ai = (zk.y - zn.y) * T(0.5);
+    const T br = (zk.y + zn.y) * T(0.5), bi = (zn.x - zk.x) * T(0.5);
+    const T hr = hre[k], hi = him[k];
+    const T yar = hr * ar, yai = hi * ai, ybr = hr * br, ybi = hi * bi;
+    // W_k = Y_a + i Y_b, W_{N-k} = conj(Y_a) + i conj(Y_b)
+    const Cplx<T> wk = {yar - ybi, yai + ybr};
+    const Cplx<T> wn = {yar + ybi, ybr - yai};
+    // pairs (k, N-k) for k <= N/2 are disjoint across lanes: no barrier needed
+    z[k] = wk;
+    if (kn != k) z[kn] = wn;
+  }
+  __syncthreads();
+  stockham<T, R>(z, tw, N, logN, +1);
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const Cplx<T> r = z[j];
+    out[static_cast<size_t>(va) * outStride + j] = static_cast<TO>(r.x);
+    if (hasB) out[static_cast<size_t>(vb) * outStride + j] = static_cast<TO>(r.y);
+  }
+}
+
+// ------------------------------------------------------------ launchers
+
+template <class T, class TO>
+int launch_fft(ctsim_ctx* ctx, const FilterPlan& p, const float* d_in, int nView, TO* d_out,
+               int outStride) {
+  const int N = p.nfp;
+  int logN = 0;
+  while ((1 << logN) < N) logN++;
+  if ((1 << logN) != N)
+    return fail(CTSIM_ERR_UNSUPPORTED, "FFT filtering needs a power-of-two transform length "
+                                       "(zeropad >= 1)");
+  const size_t lds = static_cast<size_t>(N) * sizeof(Cplx<T>) + static_cast<size_t>(N / 2) * sizeof(Cplx<T>);
+  if (lds > 64 * 1024) return fail(CTSIM_ERR_UNSUPPORTED, "FFT filtering: transform too long for LDS");
+  const T* hre = (sizeof(T) == 8 ? p.d_hmul64 : p.d_hmul32).template as<T>();
+  const T* him = hre + (N / 2 + 1);
+  const Cplx<T>* tw = (sizeof(T) == 8 ? p.d_tw64 : p.d_tw32).template as<Cplx<T>>();
+  const double* w = p.preweight.empty() ? nullptr : p.d_weight.as<double>();
+  const int grid = (nView + 1) / 2;
+  const int R = (N / 2 + 255) / 256;
+#define CTSIM_FFT_CASE(RR)                                                                     \
+  case RR:                                                                                     \
+    hipLaunchKernelGGL((fft_filter_kernel<T, TO, RR>), dim3(grid), dim3(256), lds, ctx->stream, \
+                       d_in, w, hre, him, tw, d_out, nView, p.n, N, logN, outStride);          \
+    break;
+  switch (R) {
+    CTSIM_FFT_CASE(1)
+    CTSIM_FFT_CASE(2)
+    CTSIM_FFT_CASE(4)
+    CTSIM_FFT_CASE(8)
+    CTSIM_FFT_CASE(16)
+    default: return fail(CTSIM_ERR_UNSUPPORTED, "FFT filtering: unsupported transform length");
+  }
+#undef CTSIM_FFT_CASE
+  CTSIM_HIP(hipGetLastError());
+  return CTSIM_OK;
+}
+
+}  // namespace
+
+int filter_upload(FilterPlan& p) {
+  CTSIM_CHECK(p.d_filter.reserve(p.filter.size() * sizeof(double)));
+  CTSIM_HIP(hipMemcpy(p.d_filter.ptr, p.filter.data(), p.filter.size() * sizeof(double),
+                      hipMemcpyHostToDevice));
+  if (!p.preweight.empty()) {
+    CTSIM_CHECK(p.d_weight.reserve(p.preweight.size() * sizeof(double)));
+    CTSIM_HIP(hipMemcpy(p.d_weight.ptr, p.preweight.data(), p.preweight.size() * sizeof(double),
+                        hipMemcpyHostToDevice));
+  }
+  if (!p.hre.empty()) {
+    const int N = p.nfp, H = N / 2 + 1;
+    std::vector<double> m64(2 * H);
+    std::vector<float> m32(2 * H);
+    for (int k = 0; k < H; k++) {
+      m64[k] = p.hre[k];
+      m64[H + k] = p.him[k];
+      m32[k] = static_cast<float>(p.hre[k]);
+      m32[H + k] = static_cast<float>(p.him[k]);
+    }
+    CTSIM_CHECK(p.d_hmul64.reserve(m64.size() * sizeof(double)));
+    CTSIM_CHECK(p.d_hmul32.reserve(m32.size() * sizeof(float)));
+    CTSIM_HIP(hipMemcpy(p.d_hmul64.ptr, m64.data(), m64.size() * sizeof(double), hipMemcpyHostToDevice));
+    CTSIM_HIP(hipMemcpy(p.d_hmul32.ptr, m32.data(), m32.size() * sizeof(float), hipMemcpyHostToDevice));
+    // exp(-2 pi i k / N) from FP64 host trig
+    std::vector<double> t64(N);
+    std::vector<float> t32(N);
+    for (int k = 0; k < N / 2; k++) {
+      const double a = -2.0 * kPi * k / N;
+      t64[2 * k] = cos(a);
+      t64[2 * k + 1] = sin(a);
+      t32[2 * k] = static_cast<float>(t64[2 * k]);
+      t32[2 * k + 1] = static_cast<float>(t64[2 * k + 1]);
+    }
+    CTSIM_CHECK(p.d_tw64.reserve(t64.size() * sizeof(double)));
+    CTSIM_CHECK(p.d_tw32.reserve(t32.size() * sizeof(float)));
+    CTSIM_HIP(hipMemcpy(p.d_tw64.ptr, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));
+    CTSIM_HIP(hipMemcpy(p.d_tw32.ptr, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return CTSIM_OK;
+}
+
+static int upload_dft_tables(FilterPlan& p) {
+  if (p.d_dft.ptr) return CTSIM_OK;
+  filter_build_dft_tables(p);
+  const size_t nk = p.tfc.size();
+  CTSIM_CHECK(p.d_dft.reserve(4 * nk * sizeof(double)));
+  double* d = p.d_dft.as<double>();
+  CTSIM_HIP(hipMemcpy(d, p.tfc.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+  CTSIM_HIP(hipMemcpy(d + nk, p.tfs.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+  CTSIM_HIP(hipMemcpy(d + 2 * nk, p.tbc.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+  CTSIM_HIP(hipMemcpy(d + 3 * nk, p.tbs.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+  // the host copies are no longer needed
+  std::vector<double>().swap(p.tfc);
+  std::vector<double>().swap(p.tfs);
+  std::vector<double>().swap(p.tbc);
+  std::vector<double>().swap(p.tbs);
+  return CTSIM_OK;
+}
+
+int filter_launch(ctsim_ctx* ctx, FilterPlan& p, int precision, const float* d_in, int nView,
+                  void* d_out, int outStride) {
+  if (nView <= 0) return CTSIM_OK;
+  const hipStream_t s = ctx->stream;
+  const double* w = p.preweight.empty() ? nullptr : p.d_weight.as<double>();
+  const bool f64 = precision == CTSIM_PREC_F64;
+  if (p.method == METHOD_CONVOLUTION) {
+    const dim3 grid((p.n + kWaves * kOutPerWave - 1) / (kWaves * kOutPerWave), (nView + kLanes - 1) / kLanes);
+    const size_t lds = static_cast<size_t>(2 * p.n - 1) * sizeof(double);
+    if (lds + sizeof(double) * kJChunk * kLanes > 64 * 1024)
+      return fail(CTSIM_ERR_UNSUPPORTED, "convolution filter: kernel too long for LDS");
+    if (f64)
+      hipLaunchKernelGGL((conv_seq_kernel<double>), grid, dim3(kLanes * kWaves), lds, s, d_in, w,
+                         p.d_filter.as<double>(), static_cast<double*>(d_out), nView, p.n, p.inc, outStride);
+    else
+      hipLaunchKernelGGL((conv_seq_kernel<float>), grid, dim3(kLanes * kWaves), lds, s, d_in, w,
+                         p.d_filter.as<double>(), static_cast<float*>(d_out), nView, p.n, p.inc, outStride);
+    CTSIM_HIP(hipGetLastError());
+    return CTSIM_OK;
+  }
+  if (filter_is_fourier(p) && f64) {
+    const int N = p.nfp;
+    if (N > 4096)
+      return fail(CTSIM_ERR_UNSUPPORTED, "exact DFT filtering supports transforms up to 4096 points");
+    CTSIM_CHECK(upload_dft_tables(p));
+    const size_t nk = static_cast<size_t>(N - 1) * (N - 1) + 1;
+    const double* tab = p.d_dft.as<double>();
+    CTSIM_CHECK(ctx->scratch.reserve(static_cast<size_t>(nView) * N * sizeof(double2)));
+    double2* Z = ctx->scratch.as<double2>();
+    const int per = kWaves * kOutPerWave;
+    hipLaunchKernelGGL(dft_fwd_kernel, dim3((N + per - 1) / per, (nView + kLanes - 1) / kLanes),
+                       dim3(kLanes * kWaves), 0, s, d_in, w, tab, tab + nk,
+                       p.d_filter.as<double>(), Z, nView, p.n, N);
+    CTSIM_HIP(hipGetLastError());
+    hipLaunchKernelGGL((dft_inv_kernel<double>), dim3((p.n + per - 1) / per, (nView + kLanes - 1) / kLanes),
+                       dim3(kLanes * kWaves), 0, s, Z, tab + 2 * nk, tab + 3 * nk,
+                       static_cast<double*>(d_out), nView, p.n, N, outStride);
+    CTSIM_HIP(hipGetLastError());
+    return CTSIM_OK;
+  }
+  if (f64) return launch_fft<double, double>(ctx, p, d_in, nView, static_cast<double*>(d_out), outStride);
+  return launch_fft<float, float>(ctx, p, d_in, nView, static_cast<float*>(d_out), outStride);
+}
+
+}  // namespace ctsim
+
+using namespace ctsim;
+
+extern "C" {
+
+int ctsim_procsignal_create(ctsim_ctx* ctx, ctsim_procsignal** out, const char* filterName,
+                            const char* methodName, double bw, double signalIncrement, int n,
+                            double param, const char* domainName, const char* generationName,
+                            int zeropad, int preinterpolationFactor, int geometry,
+                            double focalLength, double sourceDetectorLength) {
+  if (!ctx || !out) return fail(CTSIM_ERR_INVALID, "ctsim_procsignal_create: NULL argument");
+  *out = nullptr;
+  CTSIM_CHECK(ctx->activate());
+  auto* ps = new ctsim_procsignal();
+  int rc = filter_plan_init(ps->plan, filterName, methodName, bw, signalIncrement, n, param,
+                            domainName, generationName, zeropad, preinterpolationFactor, geometry,
+                            focalLength, sourceDetectorLength);
+  if (rc == CTSIM_OK) rc = filter_upload(ps->plan);
+  if (rc != CTSIM_OK) {
+    delete ps;
+    return rc;
+  }
+  *out = ps;
+  return CTSIM_OK;
+}
+
+void ctsim_procsignal_destroy(ctsim_procsignal* ps) { delete ps; }
+
+int ctsim_procsignal_info(const ctsim_procsignal* ps, int* nFilterPoints, double* minMaxInc) {
+  if (!ps) return fail(CTSIM_ERR_INVALID, "ctsim_procsignal_info: NULL argument");
+  if (nFilterPoints) *nFilterPoints = ps->plan.nfp;
+  if (minMaxInc) {
+    minMaxInc[0] = ps->plan.fmin;
+    minMaxInc[1] = ps->plan.fmax;
+    minMaxInc[2] = ps->plan.finc;
+  }
+  return CTSIM_OK;
+}
+
+int ctsim_procsignal_get_filter(const ctsim_procsignal* ps, double* h_out, int capacity) {
+  if (!ps || !h_out) return fail(CTSIM_ERR_INVALID, "ctsim_procsignal_get_filter: NULL argument");
+  const int n = static_cast<int>(ps->plan.filter.size());
+  for (int i = 0; i < n && i < capacity; i++) h_out[i] = ps->plan.filter[i];
+  return n;
+}
+
+int ctsim_filter_views(ctsim_ctx* ctx, ctsim_procsignal* ps, int precision, const float* d_in,
+                       int nView, void* d_out) {
+  if (!ctx || !ps || (nView > 0 && (!d_in || !d_out)))
+    return fail(CTSIM_ERR_INVALID, "ctsim_filter_views: NULL argument");
+  if (precision != CTSIM_PREC_F64 && precision != CTSIM_PREC_MIXED)
+    return fail(CTSIM_ERR_INVALID, "ctsim_filter_views: unknown precision");
+  if (nView < 0) return fail(CTSIM_ERR_INVALID, "ctsim_filter_views: nView < 0");
+  CTSIM_CHECK(ctx->activate());
+  return filter_launch(ctx, ps->plan, precision, d_in, nView, d_out, ps->plan.n * ps->plan.preinterp);
+}
+
+}  // extern "C"

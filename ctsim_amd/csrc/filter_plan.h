@@ -1,0 +1,54 @@
+// ProcessSignal plan: the filter the reference generates for one
+// (filter, method, generation, geometry, detector) combination, and the
+// device tables the filtering kernels use.
+#pragma once
+
+#include "ctsim_internal.h"
+
+namespace ctsim {
+
+// Filter method ids, libctsim/procsignal.cpp:34-41 ("fftw"/"rfftw" exist when
+// the reference is built with FFTW).
+enum FilterMethod {
+  METHOD_CONVOLUTION = 0,
+  METHOD_FOURIER = 1,
+  METHOD_FOURIER_TABLE = 2,
+  METHOD_FFT = 3,
+  METHOD_FFTW = 4,
+  METHOD_RFFTW = 5,
+};
+
+struct FilterPlan {
+  int filterId = 0, method = 0, generation = 0, geom = 0;
+  int n = 0, nfp = 0, nOut = 0, preinterp = 1, zeropad = 0;
+  double inc = 0, bw = 0, F = 0, sdl = 0, fmin = 0, fmax = 0, finc = 0, param = 0;
+  std::vector<double> filter;     // m_adFilter, nfp points
+  std::vector<double> preweight;  // fan-beam input weights (empty for parallel)
+  // Spectral multipliers for k = 0..N/2 applied to the real and imaginary
+  // parts of the unnormalised forward transform (1/N folded in).
+  std::vector<double> hre, him;
+  // Exact-DFT twiddle tables (fourier / fouier-table), index i*j.
+  std::vector<double> tfc, tfs, tbc, tbs;
+  bool tablesBuilt = false;
+
+  DevBuf d_filter, d_weight, d_hmul64, d_hmul32, d_tw64, d_tw32, d_dft;
+};
+
+// Host side (filter_host.cpp).
+int filter_plan_init(FilterPlan& p, const char* filterName, const char* methodName, double bw,
+                     double signalInc, int n, double param, const char* domainName,
+                     const char* generationName, int zeropad, int preinterp, int geometry,
+                     double focalLength, double sdl);
+void filter_build_dft_tables(FilterPlan& p);
+bool filter_is_fourier(const FilterPlan& p);
+
+// Device side (filter.hip).
+int filter_upload(FilterPlan& p);
+int filter_launch(ctsim_ctx* ctx, FilterPlan& p, int precision, const float* d_in, int nView,
+                  void* d_out, int outStride);
+
+}  // namespace ctsim
+
+struct ctsim_procsignal {
+  ctsim::FilterPlan plan;
+};

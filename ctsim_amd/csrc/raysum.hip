@@ -1,0 +1,497 @@
+// Ray-sum kernels for gfx950 (Scanner::collectProjections,
+// libctsim/scanner.cpp:280-656).
+//
+// One lane owns one (view, detector): it walks the detector's sample rays and
+// sums every phantom element's clipped chord times its attenuation in FP64,
+// then stores sum / nSample as float (scanner.cpp:522).  All lanes of a wave
+// visit the same elements in the same order, so element data is read with
+// wave-uniform loads.  The detector and source positions are repeated sums
+// along the detector array (scanner.cpp:490-532); a first pass with one lane
+// per view performs that walk so every lane starts from the reference's
+// values.
+//
+//   F64 path    the reference's clipping step order: bounding-box reject by
+//               Cohen-Sutherland on the polygon outline's box, map to the
+//               element frame, circle clip by translate / rotate (atan2, cos,
+//               sin) / clamp / rotate back, map back, length.
+//   mixed path  the same box reject, then closed-form FP64 chords for
+//               ellipses (quadratic in the element frame) and rectangles
+//               (slab clip); triangles, segments and sectors keep the
+//               reference's clip.
+#include <hip/hip_runtime.h>
+
+#include "scan_plan.h"
+
+#pragma clang fp contract(off)
+
+namespace ctsim {
+namespace {
+
+// Per-view constants of projectSingleView, scanner.cpp:449-473.
+struct ViewSetup {
+  double sdx, sdy, ddx, ddy, ddx2, ddy2, aInc, aSampInc;
+};
+
+__device__ __forceinline__ ViewSetup view_setup(const ViewEnds& e, int nDet, unsigned nSample,
+                                                int geom, double angDetInc) {
+  ViewSetup s;
+  s.sdx = (e.xs2 - e.xs1) / nDet;
+  s.sdy = (e.ys2 - e.ys1) / nDet;
+  s.ddx = s.ddy = s.ddx2 = s.ddy2 = s.aInc = s.aSampInc = 0;
+  if (geom == GEOM_EQUIANGULAR) {
+    s.aInc = angDetInc;
+    s.aSampInc = s.aInc / nSample;
+  } else {
+    s.ddx = (e.xd2 - e.xd1) / nDet;
+    s.ddy = (e.yd2 - e.yd1) / nDet;
+    s.ddx2 = s.ddx / nSample;
+    s.ddy2 = s.ddy / nSample;
+  }
+  return s;
+}
+
+struct WalkArgs {
+  const ViewEnds* ends;
+  double4* walk;  // [nView][nDet]: xs_maj, ys_maj, xd_maj (or angle), yd_maj
+  int nView, nDet, geom;
+  unsigned nSample;
+  double angDetInc, angDetLen;
+};
+
+// The detector walk of projectSingleView (scanner.cpp:449-531).
+__global__ void raysum_walk_kernel(WalkArgs a) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= a.nView) return;
+  const ViewEnds e = a.ends[v];
+  const ViewSetup s = view_setup(e, a.nDet, a.nSample, a.geom, a.angDetInc);
+  double xs = e.xs1 + (s.sdx / 2), ys = e.ys1 + (s.sdy / 2);
+  double xd = 0, yd = 0, am = 0;
+  if (a.geom == GEOM_EQUIANGULAR) {
+    const double aOff = s.aSampInc / 2;
+    am = e.detAngle - (a.angDetLen / 2) + aOff;
+  } else {
+    xd = e.xd1 + s.ddx2 / 2;
+    yd = e.yd1 + s.ddy2 / 2;
+  }
+  double4* w = a.walk + static_cast<size_t>(v) * a.nDet;
+  for (int d = 0; d < a.nDet; d++) {
+    w[d] = a.geom == GEOM_EQUIANGULAR ? make_double4(xs, ys, am, 0.0) : make_double4(xs, ys, xd, yd);
+    xs += s.sdx;
+    ys += s.sdy;
+    if (a.geom == GEOM_EQUIANGULAR) am += s.aInc;
+    else { xd += s.ddx; yd += s.ddy; }
+  }
+}
+
+// --------------------------------------------------- reference clipping
+
+__device__ __forceinline__ void xf(const double* m, double& x, double& y) {
+  const double xt = x * m[0] + y * m[2] + m[4];
+  const double yt = x * m[1] + y * m[3] + m[5];
+  x = xt;
+  y = yt;
+}
+
+// normalizeAngle, libctsupport/mathfuncs.cpp:72-81
+__device__ __forceinline__ double normalize_angle(double t) {
+  while (t < 0.) t += kTwoPi;
+  while (t >= kTwoPi) t -= kTwoPi;
+  return t;
+}
+
+// rectcode + clip_rect, libctsupport/clip.cpp:332-395
+__device__ __forceinline__ int rectcode(double x, double y, const double* r) {
+  int c = 0;
+  if (x < r[0]) c = 1;
+  else if (x > r[2]) c = 2;
+  if (y < r[1]) c += 4;
+  else if (y > r[3]) c += 8;
+  return c;
+}
+__device__ bool clip_rect(double& x1, double& y1, double& x2, double& y2, const double* r) {
+  double x = 0, y = 0;
+  int c1 = rectcode(x1, y1, r), c2 = rectcode(x2, y2, r);
+  while (c1 || c2) {
+    if (c1 & c2) return false;
+    const int c = c1 ? c1 : c2;
+    if (c & 1) { y = y1 + (y2 - y1) * (r[0] - x1) / (x2 - x1); x = r[0]; }
+    else if (c & 2) { y = y1 + (y2 - y1) * (r[2] - x1) / (x2 - x1); x = r[2]; }
+    else if (c & 4) { x = x1 + (x2 - x1) * (r[1] - y1) / (y2 - y1); y = r[1]; }
+    else if (c & 8) { x = x1 + (x2 - x1) * (r[3] - y1) / (y2 - y1); y = r[3]; }
+    if (c == c1) { x1 = x; y1 = y; c1 = rectcode(x1, y1, r); }
+    else { x2 = x; y2 = y; c2 = rectcode(x2, y2, r); }
+  }
+  return true;
+}
+
+// clip_circle, libctsupport/clip.cpp:126-198 (rot_mtx2 / xform_mtx2 inlined)
+__device__ bool clip_circle(double& x1, double& y1, double& x2, double& y2, double cx, double cy,
+                            double radius, double t1, double t2) {
+  double xc1 = x1, yc1 = y1, xc2 = x2, yc2 = y2, ccx = cx, ccy = cy;
+  const double xtrans = -xc1, ytrans = -yc1;
+  xc1 += xtrans; yc1 += ytrans; xc2 += xtrans; yc2 += ytrans; ccx += xtrans; ccy += ytrans;
+  const double theta = -atan2(yc2, xc2);
+  double c = cos(theta), s = sin(theta);
+  {  // xform by rot(theta): x' = x c + y (-s) + 0, y' = x s + y c + 0
+    double xt = xc2 * c + yc2 * -s + 0.0, yt = xc2 * s + yc2 * c + 0.0;
+    xc2 = xt; yc2 = yt;
+    xt = ccx * c + ccy * -s + 0.0; yt = ccx * s + ccy * c + 0.0;
+    ccx = xt; ccy = yt;
+  }
+  t1 += theta;
+  t2 += theta;
+  t1 = normalize_angle(t1);
+  t2 = normalize_angle(t2);
+  if (xc2 < -kDEps || fabs(yc2) > kFEps) return false;  // reference: ERR_SEVERE, reject
+  if (fabs(ccy) > radius) return false;
+  const double temp = sqrt(radius * radius - ccy * ccy);
+  const double xcmin = ccx - temp, xcmax = ccx + temp;
+  if (fabs(t2 - t1) < kDEps) {
+    if (xc1 < xcmin) xc1 = xcmin;
+    if (xc2 > xcmax) xc2 = xcmax;
+  } else if (t1 < t2) {
+    if (t1 < kPi && t2 > kPi)
+      if (xc1 < xcmin) xc1 = xcmin;
+  } else if (t1 > t2) {
+    if (t1 < kPi)
+      if (xc1 < xcmin) xc1 = xcmin;
+    if (xc2 > xcmax) xc2 = xcmax;
+  }
+  c = cos(-theta);
+  s = sin(-theta);
+  {
+    double xt = xc1 * c + yc1 * -s + 0.0, yt = xc1 * s + yc1 * c + 0.0;
+    xc1 = xt; yc1 = yt;
+    xt = xc2 * c + yc2 * -s + 0.0; yt = xc2 * s + yc2 * c + 0.0;
+    xc2 = xt; yc2 = yt;
+  }
+  xc1 += -xtrans; yc1 += -ytrans; xc2 += -xtrans; yc2 += -ytrans;
+  x1 = xc1; y1 = yc1; x2 = xc2; y2 = yc2;
+  return true;
+}
+
+// tcode + clip_triangle, libctsupport/clip.cpp:243-316
+__device__ __forceinline__ int tcode(double x, double y, double m, double b, bool clipX) {
+  int c = 0;
+  if (clipX && y < 0.) c = 1;
+  if (y > -m * x + b + kDEps) c += 2;
+  if (y > m * x + b + kDEps) c += 4;
+  return c;
+}
+__device__ bool clip_triangle(double& x1, double& y1, double& x2, double& y2, double u, double v,
+                              bool clipX) {
+  const double m = v / u, b = v;
+  int c1 = tcode(x1, y1, m, b, clipX), c2 = tcode(x2, y2, m, b, clipX);
+  while (c1 || c2) {
+    if (c1 & c2) return false;
+    const int c = c1 ? c1 : c2;
+    double x = 0, y = 0;
+    if (c & 1) {
+      x = x1 + (x2 - x1) * (0.0 - y1) / (y2 - y1);
+      y = 0.0;
+    } else if (c & 2) {
+      const double dx = x2 - x1, dy = y2 - y1;
+      if (fabs(dx) > kDEps) x = (-y1 + b + x1 * dy / dx) / (m + dy / dx);
+      else x = x1;
+      y = -m * x + b;
+    } else if (c & 4) {
+      const double dx = x2 - x1, dy = y2 - y1;
+      if (fabs(dx) > kDEps) {
+        x = (y1 - b - x1 * dy / dx);
+        x /= (m - dy / dx);
+      } else {
+        x = x1;
+      }
+      y = m * x + b;
+    }
+    if (c == c1) { x1 = x; y1 = y; c1 = tcode(x1, y1, m, b, clipX); }
+    else { x2 = x; y2 = y; c2 = tcode(x2, y2, m, b, clipX); }
+  }
+  return true;
+}
+
+// clip_segment, libctsupport/clip.cpp:44-77
+__device__ bool clip_segment(double& x1, double& y1, double& x2, double& y2, double u, double v) {
+  double xc1 = x1 * u, yc1 = y1 * v, xc2 = x2 * u, yc2 = y2 * v;
+  if (yc1 > 0. && yc2 > 0.) return false;
+  const double radius = sqrt(u * u + v * v);
+  if (!clip_circle(xc1, yc1, xc2, yc2, 0.0, v, radius, 0.0, 0.0)) return false;
+  if (yc1 > 0. && yc2 > 0.) return false;
+  if (yc1 > 0.) {
+    xc1 = xc1 + (xc2 - xc1) * (0.0 - yc1) / (yc2 - yc1);
+    yc1 = 0.0;
+  } else if (yc2 > 0.) {
+    xc2 = xc1 + (xc2 - xc1) * (0.0 - yc1) / (yc2 - yc1);
+    yc2 = 0.0;
+  }
+  x1 = xc1 / u; y1 = yc1 / v; x2 = xc2 / u; y2 = yc2 / v;
+  return true;
+}
+
+// clip_sector, libctsupport/clip.cpp:90-111
+__device__ bool clip_sector(double& x1, double& y1, double& x2, double& y2, double u, double v) {
+  double xc1 = x1 * u, yc1 = y1 * v, xc2 = x2 * u, yc2 = y2 * v;
+  const double radius = sqrt(u * u + v * v);
+  if (!clip_circle(xc1, yc1, xc2, yc2, 0.0, v, radius, 0.0, 0.0)) return false;
+  if (!clip_triangle(xc1, yc1, xc2, yc2, u, v, false)) return false;
+  x1 = xc1 / u; y1 = yc1 / v; x2 = xc2 / u; y2 = yc2 / v;
+  return true;
+}
+
+// PhantomElement::clipLineWorldCoords / clipLineNormalizedCoords and
+// Scanner::projectLineAgainstPElem, phantom.cpp:830-898, scanner.cpp:633-656
+__device__ double chord_ref(const ElemGPU& e, double x1, double y1, double x2, double y2) {
+  double cx1 = x1, cy1 = y1, cx2 = x2, cy2 = y2;
+  if (!clip_rect(cx1, cy1, cx2, cy2, e.lim)) return 0.0;
+  xf(e.p2o, x1, y1);
+  xf(e.p2o, x2, y2);
+  bool ok = false;
+  const double unit[4] = {-1.0, -1.0, 1.0, 1.0};
+  switch (e.type) {
+    case PE_RECT: ok = clip_rect(x1, y1, x2, y2, unit); break;
+    case PE_ELLIPSE: ok = clip_circle(x1, y1, x2, y2, 0.0, 0.0, 1.0, 0.0, 0.0); break;
+    case PE_TRI: ok = clip_triangle(x1, y1, x2, y2, 1.0, 1.0, true); break;
+    case PE_SEGMENT: ok = clip_segment(x1, y1, x2, y2, e.u, e.v); break;
+    case PE_SECTOR: ok = clip_sector(x1, y1, x2, y2, e.u, e.v); break;
+    default: break;
+  }
+  if (!ok) return 0.0;
+  xf(e.o2p, x1, y1);
+  xf(e.o2p, x2, y2);
+  const double len = sqrt((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
+  return len * e.atten;
+}
+
+// ------------------------------------------------- closed-form chords
+#pragma clang fp contract(fast)
+
+// Segment-box test by slabs (the reference rejects with Cohen-Sutherland on
+// the same box; the two agree except for rays grazing a box corner).
+__device__ __forceinline__ bool seg_hits_box(double x1, double y1, double dx, double dy,
+                                             const double* r) {
+  double t0 = 0.0, t1 = 1.0;
+  if (dx != 0.0) {
+    const double inv = 1.0 / dx;
+    double ta = (r[0] - x1) * inv, tb = (r[2] - x1) * inv;
+    if (ta > tb) { const double t = ta; ta = tb; tb = t; }
+    t0 = fmax(t0, ta);
+    t1 = fmin(t1, tb);
+  } else if (x1 < r[0] || x1 > r[2]) {
+    return false;
+  }
+  if (dy != 0.0) {
+    const double inv = 1.0 / dy;
+    double ta = (r[1] - y1) * inv, tb = (r[3] - y1) * inv;
+    if (ta > tb) { const double t = ta; ta = tb; tb = t; }
+    t0 = fmax(t0, ta);
+    t1 = fmin(t1, tb);
+  } else if (y1 < r[1] || y1 > r[3]) {
+    return false;
+  }
+  return t0 <= t1;
+}
+
+// Parameter interval of the segment inside the element, in the element frame.
+__device__ double chord_closed(const ElemGPU& e, double x1, double y1, double x2, double y2,
+                               double segLen) {
+  const double dx = x2 - x1, dy = y2 - y1;
+  if (!seg_hits_box(x1, y1, dx, dy, e.lim)) return 0.0;
+  if (e.type == PE_ELLIPSE || e.type == PE_RECT) {
+    const double px = x1 * e.p2o[0] + y1 * e.p2o[2] + e.p2o[4];
+    const double py = x1 * e.p2o[1] + y1 * e.p2o[3] + e.p2o[5];
+    const double qx = dx * e.p2o[0] + dy * e.p2o[2];
+    const double qy = dx * e.p2o[1] + dy * e.p2o[3];
+    double t0, t1;
+    if (e.type == PE_ELLIPSE) {
+      // |p + t q|^2 = 1
+      const double a = qx * qx + qy * qy, b = px * qx + py * qy, c = px * px + py * py - 1.0;
+      const double disc = b * b - a * c;
+      if (!(disc > 0.0)) return 0.0;
+      const double sq = sqrt(disc);
+      t0 = (-b - sq) / a;
+      t1 = (-b + sq) / a;
+    } else {
+      t0 = -1e300;
+      t1 = 1e300;
+      if (qx != 0.0) {
+        double ta = (-1.0 - px) / qx, tb = (1.0 - px) / qx;
+        if (ta > tb) { const double t = ta; ta = tb; tb = t; }
+        t0 = fmax(t0, ta);
+        t1 = fmin(t1, tb);
+      } else if (px < -1.0 || px > 1.0) {
+        return 0.0;
+      }
+      if (qy != 0.0) {
+        double ta = (-1.0 - py) / qy, tb = (1.0 - py) / qy;
+        if (ta > tb) { const double t = ta; ta = tb; tb = t; }
+        t0 = fmax(t0, ta);
+        t1 = fmin(t1, tb);
+      } else if (py < -1.0 || py > 1.0) {
+        return 0.0;
+      }
+    }
+    t0 = fmax(t0, 0.0);
+    t1 = fmin(t1, 1.0);
+    return t1 > t0 ? (t1 - t0) * segLen * e.atten : 0.0;
+  }
+  return chord_ref(e, x1, y1, x2, y2);
+}
+
+#pragma clang fp contract(off)
+
+struct RaysumArgs {
+  const ViewEnds* ends;
+  const double4* walk;
+  const ElemGPU* __restrict__ elems;
+  float* sino;
+  int nView, nDet, nElem, geom, storageOffset;
+  unsigned nSample;
+  double angDetInc, centerDetLength;
+  int unitPulse;
+};
+
+template <bool EXACT>
+__global__ __launch_bounds__(256) void raysum_kernel(RaysumArgs a) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y;
+  if (d >= a.nDet) return;
+  float* out = a.sino + static_cast<size_t>(v + a.storageOffset) * a.nDet;
+  if (a.unitPulse) {  // scanner.cpp:475-478
+    out[d] = d == a.nDet / 2 ? 1.f : 0.f;
+    return;
+  }
+  const ViewEnds e = a.ends[v];
+  const ViewSetup s = view_setup(e, a.nDet, a.nSample, a.geom, a.angDetInc);
+  const double4 w = a.walk[static_cast<size_t>(v) * a.nDet + d];
+  const double xs = w.x, ys = w.y;
+  double xd = w.z, yd = w.w, ang = w.z;
+  double sum = 0.0;
+  for (unsigned i = 0; i < a.nSample; i++) {
+    if (a.geom == GEOM_EQUIANGULAR) {
+      xd = a.centerDetLength * cos(ang);
+      yd = a.centerDetLength * sin(ang);
+    }
+    // projectSingleLine: the element list is newest-first (phantom.cpp:247)
+    double rsum = 0.0;
+    if (EXACT) {
+      for (int k = a.nElem - 1; k >= 0; k--) rsum += chord_ref(a.elems[k], xd, yd, xs, ys);
+    } else {
+      const double segLen = sqrt((xs - xd) * (xs - xd) + (ys - yd) * (ys - yd));
+      for (int k = a.nElem - 1; k >= 0; k--) rsum += chord_closed(a.elems[k], xd, yd, xs, ys, segLen);
+    }
+    sum += rsum;
+    if (a.geom == GEOM_EQUIANGULAR) ang += s.aSampInc;
+    else { xd += s.ddx2; yd += s.ddy2; }
+  }
+  out[d] = static_cast<float>(sum / a.nSample);
+}
+
+}  // namespace
+
+int raysum_launch(ctsim_ctx* ctx, const ScannerPlan& s, PhantomPlan& p, int precision,
+                  int iStartView, int iNumViews, int iOffsetView, int iStorageOffset,
+                  float* d_sino, double* h_angles) {
+  if (iNumViews <= 0) return CTSIM_OK;
+  const hipStream_t st = ctx->stream;
+  if (p.dirty || !p.d_elems.ptr) {
+    CTSIM_CHECK(p.d_elems.reserve(p.elems.size() * sizeof(ElemGPU)));
+    CTSIM_HIP(hipMemcpy(p.d_elems.ptr, p.elems.data(), p.elems.size() * sizeof(ElemGPU),
+                        hipMemcpyHostToDevice));
+    p.dirty = false;
+  }
+  const size_t eb = static_cast<size_t>(iNumViews) * sizeof(ViewEnds);
+  CTSIM_CHECK(ctx->stage.reserve(eb, st));
+  ViewEnds* ends = static_cast<ViewEnds*>(ctx->stage.ptr);
+  scanner_view_ends(s, iStartView, iNumViews, iOffsetView, ends);
+  if (h_angles)
+    for (int v = 0; v < iNumViews; v++) h_angles[v + iStorageOffset] = ends[v].viewAngle;
+  CTSIM_CHECK(ctx->stage.upload(ctx->viewConsts, eb, st));
+  CTSIM_CHECK(ctx->scratch.reserve(static_cast<size_t>(iNumViews) * s.nDet * sizeof(double4)));
+  WalkArgs wa{ctx->viewConsts.as<ViewEnds>(), ctx->scratch.as<double4>(), iNumViews, s.nDet,
+              s.geom, s.nSample, s.angDetInc, s.angDetLen};
+  hipLaunchKernelGGL(raysum_walk_kernel, dim3((iNumViews + 63) / 64), dim3(64), 0, st, wa);
+  CTSIM_HIP(hipGetLastError());
+  RaysumArgs ra{ctx->viewConsts.as<ViewEnds>(), ctx->scratch.as<double4>(), p.d_elems.as<ElemGPU>(),
+                d_sino, iNumViews, s.nDet, static_cast<int>(p.elems.size()), s.geom, iStorageOffset,
+                s.nSample, s.angDetInc, s.centerDetLength, p.unitPulse ? 1 : 0};
+  const dim3 grid((s.nDet + 255) / 256, iNumViews);
+  if (precision == CTSIM_PREC_F64)
+    hipLaunchKernelGGL((raysum_kernel<true>), grid, dim3(256), 0, st, ra);
+  else
+    hipLaunchKernelGGL((raysum_kernel<false>), grid, dim3(256), 0, st, ra);
+  CTSIM_HIP(hipGetLastError());
+  return CTSIM_OK;
+}
+
+}  // namespace ctsim
+
+using namespace ctsim;
+
+extern "C" {
+
+int ctsim_phantom_create(ctsim_phantom** out, const char* builtinName) {
+  if (!out) return fail(CTSIM_ERR_INVALID, "ctsim_phantom_create: NULL argument");
+  *out = nullptr;
+  auto* p = new ctsim_phantom();
+  if (builtinName && builtinName[0]) {
+    const int rc = phantom_builtin(p->plan, builtinName);
+    if (rc) { delete p; return rc; }
+  }
+  *out = p;
+  return CTSIM_OK;
+}
+
+void ctsim_phantom_destroy(ctsim_phantom* p) { delete p; }
+
+int ctsim_phantom_add_element(ctsim_phantom* p, const char* type, double cx, double cy, double u,
+                              double v, double rotDeg, double atten) {
+  if (!p) return fail(CTSIM_ERR_INVALID, "ctsim_phantom_add_element: NULL phantom");
+  return phantom_add(p->plan, type, cx, cy, u, v, rotDeg, atten);
+}
+
+int ctsim_phantom_extent(const ctsim_phantom* p, double* ext4) {
+  if (!p || !ext4) return fail(CTSIM_ERR_INVALID, "ctsim_phantom_extent: NULL argument");
+  ext4[0] = p->plan.xmin; ext4[1] = p->plan.xmax; ext4[2] = p->plan.ymin; ext4[3] = p->plan.ymax;
+  return static_cast<int>(p->plan.elems.size());
+}
+
+int ctsim_scanner_create(ctsim_scanner** out, const ctsim_phantom* phm, const char* geometry,
+                         int nDet, int nView, int offsetView, int nSample, double rotLen,
+                         double focalLengthRatio, double centerDetectorRatio, double viewRatio,
+                         double scanRatio) {
+  if (!out || !phm) return fail(CTSIM_ERR_INVALID, "ctsim_scanner_create: NULL argument");
+  *out = nullptr;
+  auto* s = new ctsim_scanner();
+  const int rc = scanner_init(s->plan, phm->plan, geometry, nDet, nView, offsetView, nSample, rotLen,
+                              focalLengthRatio, centerDetectorRatio, viewRatio, scanRatio);
+  if (rc) { delete s; return rc; }
+  *out = s;
+  return CTSIM_OK;
+}
+
+void ctsim_scanner_destroy(ctsim_scanner* s) { delete s; }
+
+int ctsim_scanner_info(const ctsim_scanner* sc, double* o) {
+  if (!sc || !o) return fail(CTSIM_ERR_INVALID, "ctsim_scanner_info: NULL argument");
+  const ScannerPlan& s = sc->plan;
+  o[0] = s.detInc; o[1] = s.detStart; o[2] = s.rotInc; o[3] = s.viewDiameter;
+  o[4] = s.focalLength; o[5] = s.sdl; o[6] = s.fanBeamAngle; o[7] = s.viewDiameter / kSqrt2;
+  o[8] = s.detLen; o[9] = s.centerDetLength; o[10] = s.scanDiameter; o[11] = s.geom;
+  return CTSIM_OK;
+}
+
+int ctsim_collect_projections(ctsim_ctx* ctx, const ctsim_scanner* sc, ctsim_phantom* phm,
+                              int precision, int iStartView, int iNumViews, int iOffsetView,
+                              int iStorageOffset, float* d_sino, double* h_angles) {
+  if (!ctx || !sc || !phm || (iNumViews > 0 && !d_sino))
+    return fail(CTSIM_ERR_INVALID, "ctsim_collect_projections: NULL argument");
+  if (precision != CTSIM_PREC_F64 && precision != CTSIM_PREC_MIXED)
+    return fail(CTSIM_ERR_INVALID, "ctsim_collect_projections: unknown precision");
+  if (iNumViews < 0 || iStartView < 0 || iStorageOffset < 0)
+    return fail(CTSIM_ERR_INVALID, "ctsim_collect_projections: negative view range");
+  CTSIM_CHECK(ctx->activate());
+  return raysum_launch(ctx, sc->plan, phm->plan, precision, iStartView, iNumViews, iOffsetView,
+                       iStorageOffset, d_sino, h_angles);
+}
+
+}  // extern "C"

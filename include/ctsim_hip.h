@@ -105,6 +105,70 @@ int ctsim_backproject_postprocess(ctsim_ctx* ctx, const ctsim_backprojector* bp,
  * rotScale, iDetCenter (8 doubles). */
 int ctsim_backprojector_info(const ctsim_backprojector* bp, double* out8);
 
+/* ------------------------------------------------------------ filtering */
+
+/* Replaces ProcessSignal::ProcessSignal -> init (libctsim/procsignal.cpp:82-442)
+ * with the same arguments (include/procsignal.h:66-69): the filter is
+ * generated on the host in FP64 exactly as the reference generates it
+ * (SignalFilter, libctsim/filter.cpp:110-584).  methodName may be
+ * "convolution", "fourier", "fouier-table", or "fft"/"rfftw" (the reference's
+ * FFTW real-to-halfcomplex path, procsignal.cpp:412-440, 600-613) or "fftw"
+ * (its complex FFTW path, 614-626). */
+int ctsim_procsignal_create(ctsim_ctx* ctx, ctsim_procsignal** out, const char* filterName,
+                            const char* methodName, double bw, double signalIncrement, int n,
+                            double param, const char* domainName, const char* generationName,
+                            int zeropad, int preinterpolationFactor, int geometry,
+                            double focalLength, double sourceDetectorLength);
+void ctsim_procsignal_destroy(ctsim_procsignal* ps);
+/* getNFilterPoints / getFilterMin / getFilterMax / getFilterIncrement
+ * (include/procsignal.h:80-84). */
+int ctsim_procsignal_info(const ctsim_procsignal* ps, int* nFilterPoints, double* minMaxInc3);
+/* getFilter (include/procsignal.h:85-86): copies up to capacity values,
+ * returns the number of filter points. */
+int ctsim_procsignal_get_filter(const ctsim_procsignal* ps, double* h_out, int capacity);
+/* Replaces ProcessSignal::filterSignal (procsignal.cpp:544-629) over nView
+ * rows of d_in (float [nView][n]) into d_out ([nView][n]; double for F64,
+ * float for MIXED). */
+int ctsim_filter_views(ctsim_ctx* ctx, ctsim_procsignal* ps, int precision, const float* d_in,
+                       int nView, void* d_out);
+
+/* ------------------------------------------------------- phantom, scanner */
+
+/* Replaces Phantom::Phantom / createFromPhantom (libctsim/phantom.cpp:58-170):
+ * builtinName is NULL/"" (empty phantom) or "herman", "shepp-logan",
+ * "unit-pulse". */
+int ctsim_phantom_create(ctsim_phantom** out, const char* builtinName);
+void ctsim_phantom_destroy(ctsim_phantom* phm);
+/* Replaces Phantom::addPElem (phantom.cpp:236-256): type is "rectangle",
+ * "triangle", "ellipse", "sector" or "segment", rotDeg in degrees, as in the
+ * phantom file format (phantom.cpp:184-213). */
+int ctsim_phantom_add_element(ctsim_phantom* phm, const char* type, double cx, double cy,
+                              double u, double v, double rotDeg, double atten);
+/* xmin(), xmax(), ymin(), ymax() (include/phantom.h:196-199) into ext4;
+ * returns the number of elements. */
+int ctsim_phantom_extent(const ctsim_phantom* phm, double* ext4);
+
+/* Replaces Scanner::Scanner (libctsim/scanner.cpp:86-224; include/scanner.h:75-77),
+ * rotLen in radians.  nSample < 1 is rejected (the reference divides by
+ * zero). */
+int ctsim_scanner_create(ctsim_scanner** out, const ctsim_phantom* phm, const char* geometry,
+                         int nDet, int nView, int offsetView, int nSample, double rotLen,
+                         double focalLengthRatio, double centerDetectorRatio, double viewRatio,
+                         double scanRatio);
+void ctsim_scanner_destroy(ctsim_scanner* sc);
+/* Scanner accessors (include/scanner.h:92-108): detInc, detStart, rotInc,
+ * viewDiameter, focalLength, sourceDetectorLength, fanBeamAngle, phmLen,
+ * detLen, centerDetectorLength, scanDiameter, geometry (12 doubles). */
+int ctsim_scanner_info(const ctsim_scanner* sc, double* out12);
+/* Replaces Scanner::collectProjections(proj, phm, iStartView, iNumViews,
+ * iOffsetView, iStorageOffset, trace) (scanner.cpp:296-419): writes rows
+ * [iStorageOffset, iStorageOffset + iNumViews) of d_sino (float [*][nDet])
+ * and, if h_angles is not NULL, the same entries of h_angles (the
+ * DetectorArray view angles). */
+int ctsim_collect_projections(ctsim_ctx* ctx, const ctsim_scanner* sc, ctsim_phantom* phm,
+                              int precision, int iStartView, int iNumViews, int iOffsetView,
+                              int iStorageOffset, float* d_sino, double* h_angles);
+
 #ifdef __cplusplus
 }
 #endif

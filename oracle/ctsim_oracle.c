@@ -296,7 +296,7 @@ static void elem_init(elem_t* e, int type, double cx, double cy, double u, doubl
   scale_m(t, 1 / u, 1 / v);
   mult_m(e->phmToObj, t, e->phmToObj);
 
-  double xs[400], ys[400];
+  double xs[400] = {0}, ys[400] = {0};
   int n = 0, cpts;
   double radius, theta, start, stop;
   switch (type) {
@@ -883,7 +883,66 @@ int orc_add_zeropad_factor(int n, int zeropad) {
   return n;
 }
 
-enum { M_CONVOLUTION = 0, M_FOURIER = 1, M_FOURIER_TABLE = 2, M_FFT = 3 };
+enum { M_CONVOLUTION = 0, M_FOURIER = 1, M_FOURIER_TABLE = 2, M_FFT = 3, M_FFTW = 4, M_RFFTW = 5 };
+
+/* FFTW's real-to-halfcomplex transform and its inverse, by their published
+ * definitions (FFTW 3 manual, "The Halfcomplex-format DFT"): R2HC stores
+ * Re X_k at k (k <= N/2) and Im X_k at N-k (0 < k < N/2) for the unnormalised
+ * forward DFT X_k = sum_j x_j e^{-2 pi i jk/N}; HC2R is its unnormalised
+ * inverse.  Angles are reduced exactly, (j*k mod N) * 2 pi / N. */
+static void fftw_r2hc(const double* x, double* hc, int N) {
+  for (int k = 0; k <= N / 2; k++) {
+    long double re = 0, im = 0;
+    for (int j = 0; j < N; j++) {
+      long m = ((long)j * k) % N;
+      long double a = 6.283185307179586476925286766559L * m / N;
+      re += x[j] * cosl(a);
+      im -= x[j] * sinl(a);
+    }
+    hc[k] = (double)re;
+    if (k > 0 && k < N - k) hc[N - k] = (double)im;
+  }
+}
+static void fftw_hc2r(const double* hc, double* y, int nOut, int N) {
+  for (int j = 0; j < nOut; j++) {
+    long double s = hc[0];
+    for (int k = 1; k < N - k; k++) {
+      long m = ((long)j * k) % N;
+      long double a = 6.283185307179586476925286766559L * m / N;
+      s += 2.0L * (hc[k] * cosl(a) - hc[N - k] * sinl(a));
+    }
+    if (N % 2 == 0) s += (j % 2 ? -1.0L : 1.0L) * hc[N / 2];
+    y[j] = (double)s;
+  }
+}
+/* FFTW's complex DFT of a real input, multiplied by a real filter, inverse,
+ * real part (procsignal.cpp:614-626). */
+static void fftw_complex_filter(const double* x, const double* filt, double* y, int nOut, int N) {
+  long double* re = (long double*)malloc(sizeof(long double) * N);
+  long double* im = (long double*)malloc(sizeof(long double) * N);
+  for (int k = 0; k < N; k++) {
+    long double r = 0, i = 0;
+    for (int j = 0; j < N; j++) {
+      long m = ((long)j * k) % N;
+      long double a = 6.283185307179586476925286766559L * m / N;
+      r += x[j] * cosl(a);
+      i -= x[j] * sinl(a);
+    }
+    re[k] = r * filt[k];
+    im[k] = i * filt[k];
+  }
+  for (int j = 0; j < nOut; j++) {
+    long double s = 0;
+    for (int k = 0; k < N; k++) {
+      long m = ((long)j * k) % N;
+      long double a = 6.283185307179586476925286766559L * m / N;
+      s += re[k] * cosl(a) - im[k] * sinl(a);
+    }
+    y[j] = (double)s;
+  }
+  free(re);
+  free(im);
+}
 
 typedef struct {
   int filterId, method, generation, geom, n, nfp, nOut, preinterp;
@@ -903,11 +962,12 @@ static int procsig_init(procsig_t* ps, const char* filterName, const char* metho
                         double inc, int n, double param, const char* domainName,
                         const char* genName, int zeropad, int preinterp, int geom, double F,
                         double sdl) {
-  static const char* methods[] = {"convolution", "fourier", "fouier-table", "fft"};
+  /* "fftw"/"rfftw" exist in FFTW builds of the reference (procsignal.cpp:38-41) */
+  static const char* methods[] = {"convolution", "fourier", "fouier-table", "fft", "fftw", "rfftw"};
   static const char* gens[] = {"direct", "inverse-fourier"};
   static const char* domains[] = {"frequency", "spatial"};
   memset(ps, 0, sizeof *ps);
-  ps->method = lookup(methodName, methods, 4);
+  ps->method = lookup(methodName, methods, 6);
   if (ps->method < 0) return fail(1, "Invalid filter method name");
   ps->generation = lookup(genName, gens, 2);
   if (ps->generation < 0) return fail(1, "Invalid frequency filter name");
@@ -921,7 +981,9 @@ static int procsig_init(procsig_t* ps, const char* filterName, const char* metho
     ps->inc /= sc;
     ps->bw *= sc;
   }
-  if (ps->method == M_FFT) return fail(1, "FFT not yet implemented");
+  /* With FFTW the reference remaps "fft" to its real FFTW path
+   * (procsignal.cpp:160-168); that is the behaviour restated here. */
+  if (ps->method == M_FFT) ps->method = M_RFFTW;
   int freq = ps->method != M_CONVOLUTION;
   int i;
   if (!freq) {
@@ -1016,6 +1078,8 @@ static int procsig_init(procsig_t* ps, const char* filterName, const char* metho
     free(im);
     free(sp);
   }
+  if (ps->method == M_FFTW || ps->method == M_RFFTW) /* procsignal.cpp:413-416 */
+    for (i = 0; i < ps->nfp; i++) ps->filter[i] /= ps->nfp;
   if (ps->method == M_FOURIER_TABLE) {
     int m = ps->nfp > ps->nOut ? ps->nfp : ps->nOut;
     int nF = m * m + 1;
@@ -1055,6 +1119,24 @@ static void procsig_filter(const procsig_t* ps, const float* cin, double* out) {
       for (int j = 0; j < n; j++) sum += *f1++ * *f2--;
       out[i] = sum * ps->inc;
     }
+  } else if (ps->method == M_RFFTW || ps->method == M_FFTW) {
+    /* procsignal.cpp:600-626 */
+    int N = ps->nfp;
+    double* x = (double*)calloc(N, sizeof(double));
+    double* y = (double*)malloc(sizeof(double) * N);
+    for (i = 0; i < n; i++) x[i] = in[i];
+    if (ps->method == M_RFFTW) {
+      double* hc = (double*)malloc(sizeof(double) * N);
+      fftw_r2hc(x, hc, N);
+      for (i = 0; i < N; i++) hc[i] = ps->filter[i] * hc[i];
+      fftw_hc2r(hc, y, n, N);
+      free(hc);
+    } else {
+      fftw_complex_filter(x, ps->filter, y, n, N);
+    }
+    for (i = 0; i < n; i++) out[i] = y[i];
+    free(x);
+    free(y);
   } else {
     int N = ps->nfp;
     double* x = (double*)calloc(N, sizeof(double));

@@ -145,11 +145,58 @@ def test_random_pipelines_match_reference(oracle_lib, seed):
 @needs_ref
 def test_invalid_names_fail_like_reference(oracle_lib):
     sig = np.zeros((1, 9), np.float32)
-    for bad in [dict(filter="nope", method="convolution"), dict(filter="shepp", method="nope"),
-                dict(filter="shepp", method="fft")]:
+    for bad in [dict(filter="nope", method="convolution"), dict(filter="shepp", method="nope")]:
         for kind in ("oracle", "ref"):
             with pytest.raises(RuntimeError):
                 O.filter_views(kind, sig, bw=10, inc=0.1, **bad)
+    # built without FFTW (as here), the reference refuses "fft"; the oracle
+    # restates the FFTW build's behaviour instead (see test_fft_methods_*)
+    with pytest.raises(RuntimeError):
+        O.filter_views("ref", sig, "shepp", "fft", bw=10, inc=0.1)
+
+
+def _halfcomplex_filter_numpy(x, H, n):
+    """FFTW R2HC -> multiply the halfcomplex array by H -> HC2R, by numpy."""
+    N = len(H)
+    X = np.fft.rfft(x)                       # X_k, k = 0..N/2
+    hc = np.zeros(N)
+    hc[:N // 2 + 1] = X.real
+    for k in range(1, (N + 1) // 2):
+        hc[N - k] = X.imag[k]
+    hc = hc * H
+    Y = np.zeros(N // 2 + 1, complex)
+    Y.real = hc[:N // 2 + 1]
+    for k in range(1, (N + 1) // 2):
+        Y.imag[k] = hc[N - k]
+    return np.fft.irfft(Y, N)[:n] * N
+
+
+def test_fft_methods_follow_fftw_definitions(oracle_lib):
+    rng = np.random.default_rng(12)
+    for geom in (0, 1, 2):
+        for n in (21, 32):
+            sig = rng.standard_normal((2, n)).astype(np.float32)
+            args = dict(bw=1 / 0.02, inc=0.02, zeropad=1, geometry=geom, focal=2.0, sdl=4.0)
+            out, H, _ = O.filter_views("oracle", sig, "abs_hanning", "fft", **args)
+            cx, Hc, _ = O.filter_views("oracle", sig, "abs_hanning", "fftw", **args)
+            four, Hf, _ = O.filter_views("oracle", sig, "abs_hanning", "fourier", **args)
+            N = len(H)
+            np.testing.assert_array_equal(H, Hf / N)  # the FFTW paths pre-divide by N
+            inc = 0.02 / (4.0 / 2.0) if geom == 2 else 0.02
+            for v in range(2):
+                x = np.zeros(N)
+                x[:n] = sig[v]
+                k = np.arange(n) - n // 2
+                if geom == 1:
+                    x[:n] *= 2.0 * np.cos(k * inc)
+                elif geom == 2:
+                    x[:n] *= 2.0 / np.sqrt(4.0 + (k * inc) ** 2)
+                ref = _halfcomplex_filter_numpy(x, H, n)
+                np.testing.assert_allclose(out[v], ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+            # the complex FFTW path is the 'fourier' mathematics
+            np.testing.assert_allclose(cx, four, rtol=1e-9, atol=1e-11 * np.abs(four).max())
+            if geom == 0:  # symmetric filter: the halfcomplex product agrees too
+                np.testing.assert_allclose(out, four, rtol=1e-9, atol=1e-11 * np.abs(four).max())
 
 
 # -------------------------------------------------------------- closed forms
