@@ -17,6 +17,8 @@
 //               per-view constants, Kahan-compensated accumulation.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "bp_plan.h"
 
 // FP64 parity kernels keep the reference's rounding: no contraction.
@@ -571,6 +573,30 @@ void launch_f32_interp(int interp, bool lastIncl, const BpArgs& a, int grid, siz
   }
 }
 
+// Per-view constants for this launch: reuse the plan's device copy when the
+// angles are those of the previous launch, else compute them on the host and
+// copy them in stream order (after any kernel still reading the old ones).
+template <class VC, class F>
+int view_consts(ctsim_ctx* ctx, const BpPlan& p, int kind, const double* h_angles, int nView,
+                F compute, const void** out) {
+  const bool same = p.cachedKind == kind && static_cast<int>(p.cachedAngles.size()) == nView &&
+                    memcmp(p.cachedAngles.data(), h_angles, sizeof(double) * nView) == 0;
+  if (!same) {
+    const size_t bytes = static_cast<size_t>(nView) * sizeof(VC);
+    CTSIM_CHECK(ctx->stage.reserve(bytes, ctx->stream));
+    compute(static_cast<VC*>(ctx->stage.ptr));
+    if (p.d_vc.bytes < bytes) {
+      CTSIM_HIP(hipStreamSynchronize(ctx->stream));
+      CTSIM_CHECK(p.d_vc.reserve(bytes));
+    }
+    CTSIM_CHECK(ctx->stage.upload(p.d_vc, bytes, ctx->stream));
+    p.cachedAngles.assign(h_angles, h_angles + nView);
+    p.cachedKind = kind;
+  }
+  *out = p.d_vc.ptr;
+  return CTSIM_OK;
+}
+
 }  // namespace
 
 int bp_upload_tables(BpPlan& p) {
@@ -616,11 +642,9 @@ int bp_launch(ctsim_ctx* ctx, const BpPlan& p, int precision, const void* d_filt
   const bool f64 = precision == CTSIM_PREC_F64 || p.kind == KIND_DIFF || p.interp == INTERP_NEAREST;
   const bool inF32 = precision == CTSIM_PREC_MIXED;
   if (f64) {
-    const size_t bytes = static_cast<size_t>(nView) * sizeof(ViewConstF64);
-    CTSIM_CHECK(ctx->stage.reserve(bytes, s));
-    bp_view_consts_f64(p, h_angles, nView, static_cast<ViewConstF64*>(ctx->stage.ptr));
-    CTSIM_CHECK(ctx->stage.upload(ctx->viewConsts, bytes, s));
-    a.vc = ctx->viewConsts.ptr;
+    CTSIM_CHECK(view_consts<ViewConstF64>(
+        ctx, p, 0, h_angles, nView,
+        [&](ViewConstF64* o) { bp_view_consts_f64(p, h_angles, nView, o); }, &a.vc));
     if (p.kind == KIND_DIFF) {
       const int g = (p.nx + 63) / 64;
       if (inF32) launch_diff<float>(p.interp, a, g, s);
@@ -643,11 +667,9 @@ int bp_launch(ctsim_ctx* ctx, const BpPlan& p, int precision, const void* d_filt
     return CTSIM_OK;
   }
 
-  const size_t bytes = static_cast<size_t>(nView) * sizeof(ViewConstF32);
-  CTSIM_CHECK(ctx->stage.reserve(bytes, s));
-  bp_view_consts_f32(p, h_angles, nView, static_cast<ViewConstF32*>(ctx->stage.ptr));
-  CTSIM_CHECK(ctx->stage.upload(ctx->viewConsts, bytes, s));
-  a.vc = ctx->viewConsts.ptr;
+  CTSIM_CHECK(view_consts<ViewConstF32>(
+      ctx, p, 1, h_angles, nView,
+      [&](ViewConstF32* o) { bp_view_consts_f32(p, h_angles, nView, o); }, &a.vc));
   a.W = p.windowF32;
   if (p.kind == KIND_TRIG) a.endScale = static_cast<float>(p.rotScale);
   const int Wv = a.W > 0 ? a.W : p.nDet + 8;

@@ -44,6 +44,11 @@ struct BpPlan {
   // Launch shape.
   int tileX = 16, tileY = 16;       // pixels per block along ix, iy
   int windowF64 = 0, windowF32 = 0; // detector window per (tile, view); 0 = full row
+  // Per-view constants of the last angle set, kept on the device so repeated
+  // reconstructions with the same views upload nothing.
+  mutable std::vector<double> cachedAngles;
+  mutable int cachedKind = -1;      // 0: FP64 constants, 1: FP32 constants
+  mutable DevBuf d_vc;
 };
 
 // Host side (bp_host.cpp).

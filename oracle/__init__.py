@@ -275,3 +275,28 @@ def phantom_extent(phm: Phantom):
     ext = np.zeros(4, np.float64)
     helpers().orc_phantom_extent(_ptr(t), _ptr(p), n, _b(phm.builtin), _ptr(ext))
     return ext
+
+
+def fbp_sample(kind: str, phm: Phantom, geometry: str, nDet: int, nView: int, nSample: int,
+               rotLen: float, focalRatio: float, centerRatio: float, filter: str, method: str,
+               zeropad: int, interp: str, backproj: str, nx: int, ny: int, sino: np.ndarray,
+               angles: np.ndarray, nUse: int):
+    """Filter + backproject the first nUse views on the CPU; returns
+    (image, filter seconds, backprojection seconds)."""
+    L = lib(kind)
+    pre = "orc" if kind == "oracle" else "ref"
+    f = getattr(L, f"{pre}_fbp_sample")
+    c, i, d, vp = C.c_char_p, C.c_int, C.c_double, C.c_void_p
+    f.restype = i
+    f.argtypes = [vp, vp, i, c, c, i, i, i, d, d, d, c, c, i, c, c, i, i, vp, vp, i, vp, vp]
+    t, p, n = phm.arrays()
+    img = np.zeros((nx, ny), np.float32)
+    tim = np.zeros(2, np.float64)
+    sino = np.ascontiguousarray(sino, np.float32)
+    angles = np.ascontiguousarray(angles, np.float64)
+    rc = f(_ptr(t), _ptr(p), n, _b(phm.builtin), _b(geometry), nDet, nView, nSample, rotLen,
+           focalRatio, centerRatio, _b(filter), _b(method), zeropad, _b(interp), _b(backproj), nx,
+           ny, _ptr(sino), _ptr(angles), nUse, _ptr(img), _ptr(tim))
+    if rc:
+        raise RuntimeError(f"{kind} fbp_sample failed: {getattr(L, pre + '_last_error')().decode()}")
+    return img, tim[0], tim[1]

@@ -1536,6 +1536,59 @@ int orc_pipeline(const int* types, const double* params, int nElem, const char* 
   return 0;
 }
 
+/* Filter + backproject the first nUse views of a given sinogram
+ * (Reconstructor::reconstructView, reconstruct.cpp:156-239), timing the two
+ * stages; the CPU-baseline leg of the benchmark when the reference library is
+ * not available. */
+int orc_fbp_sample(const int* types, const double* params, int nElem, const char* builtin,
+                   const char* geometry, int nDet, int nView, int nSample, double rotLen,
+                   double focalRatio, double centerRatio, const char* filterName,
+                   const char* methodName, int zeropad, const char* interpName,
+                   const char* backprojName, int nx, int ny, const float* sino,
+                   const double* angles, int nUse, float* image, double* timings) {
+  phantom_t p;
+  scanner_t s;
+  if (phm_build(&p, types, params, nElem, builtin)) return 1;
+  if (scanner_init(&s, &p, geometry, nDet, nView, 0, nSample, rotLen, focalRatio, centerRatio,
+                   1.0, 1.0)) {
+    free(p.e);
+    return 2;
+  }
+  procsig_t ps;
+  if (procsig_init(&ps, filterName, methodName, 1. / s.detInc, s.detInc, nDet, -1.0, "spatial",
+                   "direct", zeropad, 1, s.geom, s.focalLength, s.sdl)) {
+    procsig_free(&ps);
+    free(p.e);
+    return 3;
+  }
+  bp_t b;
+  if (bp_init(&b, s.geom, nDet, s.detInc, nView, s.viewDiameter, s.focalLength, s.sdl,
+              backprojName, interpName, nx, ny, NULL)) {
+    procsig_free(&ps);
+    free(p.e);
+    return 4;
+  }
+  memset(image, 0, sizeof(float) * nx * ny);
+  double* buf = (double*)calloc(nDet, sizeof(double));
+  double tf = 0, tb = 0;
+  for (int v = 0; v < nUse; v++) {
+    double a = now_s();
+    procsig_filter(&ps, sino + (size_t)v * nDet, buf);
+    double c = now_s();
+    bp_view(&b, image, buf, angles[v]);
+    tf += c - a;
+    tb += now_s() - c;
+  }
+  bp_post(&b, image);
+  timings[0] = tf;
+  timings[1] = tb;
+  free(buf);
+  bp_free(&b);
+  procsig_free(&ps);
+  free(p.e);
+  return 0;
+}
+
 /* ImageFile::comparativeStatistics, libctsim/imagefile.cpp:185-240 */
 int orc_stats(int nx, int ny, const float* a, const float* b, double* dre) {
   if (nx == 0 || ny == 0) return 1;

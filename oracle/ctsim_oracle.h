@@ -54,6 +54,13 @@ int orc_backproject(const int* types, const double* params, int nElem, const cha
 
 int orc_stats(int nx, int ny, const float* refImage, const float* cmpImage, double* dre);
 
+int orc_fbp_sample(const int* types, const double* params, int nElem, const char* builtin,
+                   const char* geometry, int nDet, int nView, int nSample, double rotLen,
+                   double focalRatio, double centerRatio, const char* filterName,
+                   const char* methodName, int zeropad, const char* interpName,
+                   const char* backprojName, int nx, int ny, const float* sino,
+                   const double* angles, int nUse, float* image, double* timings);
+
 const char* orc_last_error(void);
 
 /* Building blocks exposed for closed-form checks. */

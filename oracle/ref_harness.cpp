@@ -188,3 +188,47 @@ int ref_stats(int nx, int ny, const float* refImage, const float* cmpImage, doub
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Filter + backproject the first nUse views of a given sinogram, as
+// Reconstructor::reconstructView does (reconstruct.cpp:156-239), timing the
+// two stages.  The geometry comes from a Scanner built on the same phantom.
+int ref_fbp_sample(const int* types, const double* params, int nElem, const char* builtin,
+                   const char* geometry, int nDet, int nView, int nSample, double rotLen,
+                   double focalRatio, double centerRatio, const char* filterName,
+                   const char* methodName, int zeropad, const char* interpName,
+                   const char* backprojName, int nx, int ny, const float* sino,
+                   const double* angles, int nUse, float* image, double* timings) {
+  Phantom phm;
+  buildPhantom(phm, types, params, nElem, builtin);
+  Scanner scanner(phm, geometry, nDet, nView, 0, nSample, rotLen, focalRatio, centerRatio, 1.0, 1.0);
+  if (scanner.fail()) { g_err = scanner.failMessage(); return 2; }
+  Projections pj(scanner);
+  ImageFile im(nx, ny);
+  ProcessSignal ps(filterName, methodName, 1. / pj.detInc(), pj.detInc(), pj.nDet(), -1.0,
+                   "spatial", "direct", zeropad, 1, Trace::TRACE_NONE, pj.geometry(),
+                   pj.focalLength(), pj.sourceDetectorLength());
+  if (ps.fail()) { g_err = ps.failMessage(); return 3; }
+  Backprojector bp(pj, im, backprojName, interpName, 1, NULL);
+  if (bp.fail()) { g_err = bp.failMessage(); return 4; }
+  double* buf = new double[nDet];
+  double tf = 0, tb = 0;
+  for (int v = 0; v < nUse; v++) {
+    auto a = std::chrono::steady_clock::now();
+    ps.filterSignal(sino + (size_t)v * nDet, buf);
+    auto b = std::chrono::steady_clock::now();
+    bp.BackprojectView(buf, angles[v]);
+    auto c = std::chrono::steady_clock::now();
+    tf += std::chrono::duration<double>(b - a).count();
+    tb += std::chrono::duration<double>(c - b).count();
+  }
+  bp.PostProcessing();
+  delete[] buf;
+  copyImage(im, image);
+  timings[0] = tf;
+  timings[1] = tb;
+  return 0;
+}
+
+}  // extern "C"
