@@ -4,7 +4,8 @@ Host sources (*.cpp) are compiled by g++ with -O2 -ffp-contract=off, the
 reference's own code-generation class, so the FP64 geometry, filter and trig
 tables they produce are bit-identical to the reference's.  Device sources
 (*.hip) are compiled by hipcc for gfx950 only, with packed-FP32 instructions
-disabled.  Everything links into ctsim_amd/libctsim_amd.so.
+disabled.  Everything links into ctsim_amd/libctsim_amd.so; the command-line tools of
+ctsim_amd/tools are linked against it into ctsim_amd/bin.
 
     python -m ctsim_amd.build [-j N] [--verbose]
 """
@@ -21,6 +22,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libctsim_amd.so")
+TOOLS = os.path.join(PKG, "tools")
+BIN = os.path.join(PKG, "bin")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
@@ -66,7 +69,21 @@ def build(jobs: int = 8, verbose: bool = False) -> str:
     _run(["hipcc", "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs,
           f"-L{ROCM}/lib", "-lamdhip64", "-Wl,-z,defs", "-lm", "-lpthread"], verbose)
     os.replace(tmp, LIB)
+    _build_tools(verbose)
     return LIB
+
+
+def _build_tools(verbose: bool) -> None:
+    """Command-line drop-ins (pjrec, phm2pj, if2) linked to the library with
+    an rpath of $ORIGIN/.., so they run from the tree."""
+    os.makedirs(BIN, exist_ok=True)
+    for f in sorted(os.listdir(TOOLS)):
+        if not f.endswith(".cpp"):
+            continue
+        exe = os.path.join(BIN, f[:-4])
+        _run(["g++", *HOST_FLAGS, *INCLUDES, os.path.join(TOOLS, f), "-o", exe, f"-L{PKG}",
+              "-lctsim_amd", "-Wl,-rpath,$ORIGIN/..", f"-L{ROCM}/lib", "-lamdhip64",
+              f"-Wl,-rpath,{ROCM}/lib"], verbose)
 
 
 if __name__ == "__main__":

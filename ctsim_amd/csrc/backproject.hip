@@ -386,10 +386,24 @@ __device__ __forceinline__ float cubic_f32(const float* w, int lo, int Wv, int n
 // atan for FP32: odd polynomial t*P(t^2) on [0,1] (weighted least-squares fit
 // of atan(t)/t in t^2, max abs error 1.3e-7), with atan(t) = pi/2 - atan(1/t)
 // beyond 1.
+template <bool SMALL_T>
 __device__ __forceinline__ float atan_f32(float t) {
+  if constexpr (SMALL_T) {  // |t| <= 1 is guaranteed by the geometry
+    const float z = t * t;
+    float p = 0.0028499013278633356f;
+    p = p * z - 0.01606869138777256f;
+    p = p * z + 0.042691655457019806f;
+    p = p * z - 0.07504309713840485f;
+    p = p * z + 0.10640943795442581f;
+    p = p * z - 0.14203648269176483f;
+    p = p * z + 0.1999261975288391f;
+    p = p * z - 0.3333307206630707f;
+    p = p * z + 1.0f;
+    return t * p;
+  }
   const float at = fabsf(t);
   const bool inv = at > 1.f;
-  const float u = inv ? __frcp_rn(at) : at;
+  const float u = inv ? __builtin_amdgcn_rcpf(at) : at;
   const float z = u * u;
   float p = 0.0028499013278633356f;
   p = p * z - 0.01606869138777256f;
@@ -406,33 +420,44 @@ __device__ __forceinline__ float atan_f32(float t) {
 }
 
 
-template <int GK>  // 0 parallel, 1 equiangular, 2 equilinear
-__device__ __forceinline__ void pos_f32(const BpArgs& a, const ViewConstF32& c, float x, float y,
-                                        float& pos, float& wgt) {
+// Detector position and weight of one pixel in one view (mixed path).
+// ca, sa are cos/detInc, sin/detInc (parallel) or cos, sin (fan).
+template <int GK, bool SMALL_T>
+__device__ __forceinline__ void pos_f32(float center, float F, float posScale, float ca, float sa,
+                                        float x, float y, float& pos, float& wgt) {
   if constexpr (GK == 0) {
-    pos = static_cast<float>(a.iDetCenter) + x * c.ca + y * c.sa;
+    pos = fmaf(y, sa, fmaf(x, ca, center));
     wgt = 1.f;
   } else {
-    const float rc = x * c.ca + y * c.sa;
-    const float rs = x * c.sa - y * c.ca;
-    const float F = static_cast<float>(a.F);
-    const float fl = F + rs;
-    const float q = __frcp_rn(fl);
+    const float rc = fmaf(y, sa, x * ca);
+    const float fl = fmaf(x, sa, fmaf(-y, ca, F));  // F + r sin(beta - phi)
+    const float q = __builtin_amdgcn_rcpf(fl);
     if constexpr (GK == 1) {
-      pos = atan_f32(rc * q) * static_cast<float>(1.0 / a.detInc) + static_cast<float>(a.iDetCenter);
-      wgt = __frcp_rn(fl * fl + rc * rc);
+      pos = fmaf(atan_f32<SMALL_T>(rc * q), posScale, center);
+      wgt = __builtin_amdgcn_rcpf(fmaf(fl, fl, rc * rc));
     } else {
-      pos = rc * q * static_cast<float>(a.sdl / a.detInc) + static_cast<float>(a.iDetCenter);
+      pos = fmaf(rc * q, posScale, center);
       const float u = F * q;
       wgt = u * u;
     }
   }
 }
 
-template <int GK, int INTERP, bool LAST_INCL>
+// Cubic through y0..y3 at nodes -1, 0, 1, 2, evaluated at t in [0, 1), in
+// Horner form (the same polynomial as the reference's Lagrange form).
+__device__ __forceinline__ float cubic_horner(float y0, float y1, float y2, float y3, float t) {
+  const float a2 = fmaf(0.5f, y0 + y2, -y1);
+  const float a3 = fmaf(1.f / 6.f, y3 - y0, 0.5f * (y1 - y2));
+  const float a1 = fmaf(-1.f / 6.f, y3, fmaf(-0.5f, y1, fmaf(-1.f / 3.f, y0, y2)));
+  return fmaf(t, fmaf(t, fmaf(t, a3, a2), a1), y1);
+}
+
+template <int GK, int INTERP, bool LAST_INCL, bool SMALL_T>
 __global__ __launch_bounds__(kBlock) void bp_f32_kernel(BpArgs a) {
   extern __shared__ float s_g[];
   __shared__ int s_lo2[kMaxChunk];
+  __shared__ int s_in[kMaxChunk];
+  __shared__ float2 s_cs[kMaxChunk];
   const int t = threadIdx.x;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int ix0 = (tile / a.tilesY) * a.tileX, iy0 = (tile % a.tilesY) * a.tileY;
@@ -445,24 +470,34 @@ __global__ __launch_bounds__(kBlock) void bp_f32_kernel(BpArgs a) {
   const float* frows = static_cast<const float*>(a.f);
   const float x = a.xs[active ? ix : ix0], y = a.ys[active ? iy : iy0];
   const int nDet = a.nDet;
+  const float center = static_cast<float>(a.iDetCenter);
+  const float F = static_cast<float>(a.F);
+  const float posScale = GK == 1 ? static_cast<float>(1.0 / a.detInc)
+                                 : static_cast<float>(a.sdl / a.detInc);
   float sum = 0.f, comp = 0.f;
 
   for (int v0 = 0; v0 < a.nView; v0 += a.VC) {
     const int nv = min(a.VC, a.nView - v0);
     if (t < nv) {
-      int lo = -4;
+      const ViewConstF32 c = vcs[v0 + t];
+      int lo = -4, interior = 0;
       if (a.W > 0) {
-        const ViewConstF32 c = vcs[v0 + t];
         float p0, p1, p2, p3, wd;
-        pos_f32<GK>(a, c, a.xs[ix0], a.ys[iy0], p0, wd);
-        pos_f32<GK>(a, c, a.xs[cx1], a.ys[iy0], p1, wd);
-        pos_f32<GK>(a, c, a.xs[ix0], a.ys[cy1], p2, wd);
-        pos_f32<GK>(a, c, a.xs[cx1], a.ys[cy1], p3, wd);
+        pos_f32<GK, SMALL_T>(center, F, posScale, c.ca, c.sa, a.xs[ix0], a.ys[iy0], p0, wd);
+        pos_f32<GK, SMALL_T>(center, F, posScale, c.ca, c.sa, a.xs[cx1], a.ys[iy0], p1, wd);
+        pos_f32<GK, SMALL_T>(center, F, posScale, c.ca, c.sa, a.xs[ix0], a.ys[cy1], p2, wd);
+        pos_f32<GK, SMALL_T>(center, F, posScale, c.ca, c.sa, a.xs[cx1], a.ys[cy1], p3, wd);
         float pmin = fminf(fminf(p0, p1), fminf(p2, p3));
+        float pmax = fmaxf(fmaxf(p0, p1), fmaxf(p2, p3));
         if (!(pmin > -1e9f && pmin < 1e9f)) pmin = -4.f;
         lo = static_cast<int>(floorf(pmin)) - 4;
+        // every tap of every pixel is an interior Lagrange tap
+        interior = (pmin >= 2.f && pmax <= static_cast<float>(nDet - 4) &&
+                    pmax - pmin <= static_cast<float>(Wv - 10)) ? 1 : 0;
       }
       s_lo2[t] = lo;
+      s_in[t] = interior;
+      s_cs[t] = make_float2(c.ca, c.sa);
     }
     __syncthreads();
     for (int vv = 0; vv < nv; vv++) {
@@ -477,11 +512,11 @@ __global__ __launch_bounds__(kBlock) void bp_f32_kernel(BpArgs a) {
     __syncthreads();
     if (active) {
       for (int vv = 0; vv < nv; vv++) {
-        const ViewConstF32 c = vcs[v0 + vv];
+        const float2 cs = s_cs[vv];
         const float* w = s_g + vv * Wv;
-        const int lo = s_lo2[vv];
+        const int lo = __builtin_amdgcn_readfirstlane(s_lo2[vv]);
         float pos, wgt;
-        pos_f32<GK>(a, c, x, y, pos, wgt);
+        pos_f32<GK, SMALL_T>(center, F, posScale, cs.x, cs.y, x, y, pos, wgt);
         float cval = 0.f;
         if constexpr (INTERP == INTERP_NEAREST) {
           const int i = static_cast<int>(floorf(pos + 0.5f));
@@ -494,14 +529,20 @@ __global__ __launch_bounds__(kBlock) void bp_f32_kernel(BpArgs a) {
           if (i >= 0 && i <= hiOk) {
             const float f0 = wget(w, lo, Wv, i);
             const float f1 = (i < nDet - 1) ? wget(w, lo, Wv, i + 1) : f0;
-            cval = f0 + fr * (f1 - f0);
+            cval = fmaf(fr, f1 - f0, f0);
           }
         } else {
-          cval = cubic_f32(w, lo, Wv, nDet, pos);
+          if (__builtin_amdgcn_readfirstlane(s_in[vv])) {
+            const float pf = floorf(pos);
+            int k = static_cast<int>(pf) - 1 - lo;
+            k = min(max(k, 0), Wv - 4);
+            cval = cubic_horner(w[k], w[k + 1], w[k + 2], w[k + 3], pos - pf);
+          } else {
+            cval = cubic_f32(w, lo, Wv, nDet, pos);
+          }
         }
-        cval *= wgt;
         // Kahan-compensated accumulation across views
-        const float yk = cval - comp;
+        const float yk = cval * wgt - comp;
         const float tk = sum + yk;
         comp = (tk - sum) - yk;
         sum = tk;
@@ -556,20 +597,27 @@ void launch_diff(int interp, const BpArgs& a, int grid, hipStream_t s) {
 }
 
 template <int GK, int INTERP>
-void launch_f32(bool lastIncl, const BpArgs& a, int grid, size_t lds, hipStream_t s) {
-  if (lastIncl)
-    hipLaunchKernelGGL((bp_f32_kernel<GK, INTERP, true>), dim3(grid), dim3(kBlock), lds, s, a);
-  else
-    hipLaunchKernelGGL((bp_f32_kernel<GK, INTERP, false>), dim3(grid), dim3(kBlock), lds, s, a);
+void launch_f32(bool lastIncl, bool smallT, const BpArgs& a, int grid, size_t lds, hipStream_t s) {
+  if (GK == 1 && smallT) {
+    if (lastIncl)
+      hipLaunchKernelGGL((bp_f32_kernel<GK, INTERP, true, true>), dim3(grid), dim3(kBlock), lds, s, a);
+    else
+      hipLaunchKernelGGL((bp_f32_kernel<GK, INTERP, false, true>), dim3(grid), dim3(kBlock), lds, s, a);
+  } else {
+    if (lastIncl)
+      hipLaunchKernelGGL((bp_f32_kernel<GK, INTERP, true, false>), dim3(grid), dim3(kBlock), lds, s, a);
+    else
+      hipLaunchKernelGGL((bp_f32_kernel<GK, INTERP, false, false>), dim3(grid), dim3(kBlock), lds, s, a);
+  }
 }
 
 template <int GK>
-void launch_f32_interp(int interp, bool lastIncl, const BpArgs& a, int grid, size_t lds,
-                       hipStream_t s) {
+void launch_f32_interp(int interp, bool lastIncl, bool smallT, const BpArgs& a, int grid,
+                       size_t lds, hipStream_t s) {
   switch (interp) {
-    case INTERP_NEAREST: launch_f32<GK, INTERP_NEAREST>(lastIncl, a, grid, lds, s); break;
-    case INTERP_LINEAR: launch_f32<GK, INTERP_LINEAR>(lastIncl, a, grid, lds, s); break;
-    default: launch_f32<GK, INTERP_CUBIC>(lastIncl, a, grid, lds, s); break;
+    case INTERP_NEAREST: launch_f32<GK, INTERP_NEAREST>(lastIncl, smallT, a, grid, lds, s); break;
+    case INTERP_LINEAR: launch_f32<GK, INTERP_LINEAR>(lastIncl, smallT, a, grid, lds, s); break;
+    default: launch_f32<GK, INTERP_CUBIC>(lastIncl, smallT, a, grid, lds, s); break;
   }
 }
 
@@ -682,9 +730,9 @@ int bp_launch(ctsim_ctx* ctx, const BpPlan& p, int precision, const void* d_filt
   if (lds > 64 * 1024) return fail(CTSIM_ERR_UNSUPPORTED, "backprojector: detector row too long for LDS");
   const bool lastIncl = p.kind == KIND_IDIFF;
   const int gk = p.geom == GEOM_PARALLEL ? 0 : (p.geom == GEOM_EQUIANGULAR ? 1 : 2);
-  if (gk == 0) launch_f32_interp<0>(p.interp, lastIncl, a, grid, lds, s);
-  else if (gk == 1) launch_f32_interp<1>(p.interp, lastIncl, a, grid, lds, s);
-  else launch_f32_interp<2>(p.interp, lastIncl, a, grid, lds, s);
+  if (gk == 0) launch_f32_interp<0>(p.interp, lastIncl, p.smallTan, a, grid, lds, s);
+  else if (gk == 1) launch_f32_interp<1>(p.interp, lastIncl, p.smallTan, a, grid, lds, s);
+  else launch_f32_interp<2>(p.interp, lastIncl, p.smallTan, a, grid, lds, s);
   CTSIM_HIP(hipGetLastError());
   return CTSIM_OK;
 }

@@ -108,6 +108,13 @@ int bp_plan_init(BpPlan& p, int geometry, int nDet, double detInc, int nView,
   }
   p.windowF64 = bp_window_bound(p, p.tileX, p.tileY);
   p.windowF32 = p.windowF64;
+  {
+    // tan(gamma) = r cos / (F + r sin) stays within [-1, 1] when every image
+    // point is closer to the origin than F / sqrt(2) (with a margin)
+    const double rx = fmax(fabs(p.xMin), fabs(p.xMax)), ry = fmax(fabs(p.yMin), fabs(p.yMax));
+    const double R = sqrt(rx * rx + ry * ry);
+    p.smallTan = p.F > 0 && R * 1.4143 * 1.001 < p.F;
+  }
   return CTSIM_OK;
 }
 

@@ -44,6 +44,7 @@ struct BpPlan {
   // Launch shape.
   int tileX = 16, tileY = 16;       // pixels per block along ix, iy
   int windowF64 = 0, windowF32 = 0; // detector window per (tile, view); 0 = full row
+  bool smallTan = false;            // fan beam: |tan gamma| <= 1 for every pixel
   // Per-view constants of the last angle set, kept on the device so repeated
   // reconstructions with the same views upload nothing.
   mutable std::vector<double> cachedAngles;

@@ -97,13 +97,8 @@ int ctsim_ctx_create(ctsim_ctx** out, int device, void* stream) {
   c->device = device;
   int rc = c->activate();
   if (rc) { delete c; return rc; }
-  if (stream) {
-    c->stream = static_cast<hipStream_t>(stream);
-  } else {
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate", __FILE__, __LINE__); }
-    c->ownsStream = true;
-  }
+  // NULL is HIP's null stream, so work is ordered with other null-stream work
+  c->stream = static_cast<hipStream_t>(stream);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->numCUs = prop.multiProcessorCount;
   *out = c;
@@ -113,25 +108,14 @@ int ctsim_ctx_create(ctsim_ctx** out, int device, void* stream) {
 void ctsim_ctx_destroy(ctsim_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  if (ctx->ownsStream) (void)hipStreamDestroy(ctx->stream);
+  (void)hipStreamSynchronize(ctx->stream);
   delete ctx;
 }
 
 int ctsim_ctx_set_stream(ctsim_ctx* ctx, void* stream) {
   if (!ctx) return fail(CTSIM_ERR_INVALID, "ctsim_ctx_set_stream: ctx is NULL");
   CTSIM_CHECK(ctx->activate());
-  if (ctx->ownsStream) {
-    CTSIM_HIP(hipStreamSynchronize(ctx->stream));
-    CTSIM_HIP(hipStreamDestroy(ctx->stream));
-    ctx->ownsStream = false;
-  }
-  if (stream) {
-    ctx->stream = static_cast<hipStream_t>(stream);
-  } else {
-    CTSIM_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    ctx->ownsStream = true;
-  }
+  ctx->stream = static_cast<hipStream_t>(stream);
   return CTSIM_OK;
 }
 

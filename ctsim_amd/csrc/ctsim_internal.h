@@ -73,8 +73,7 @@ struct PinnedStage {
 // The context: one device, one stream, its scratch.
 struct ctsim_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
-  bool ownsStream = false;
+  hipStream_t stream = nullptr;  // nullptr is the null stream
   ctsim::PinnedStage stage;
   ctsim::DevBuf viewConsts;   // per-view constants of the current launch
   ctsim::DevBuf scratch;      // kernel scratch (partials, counters)

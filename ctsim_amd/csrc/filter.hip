@@ -277,6 +277,106 @@ __global__ __launch_bounds__(256) void fft_filter_kernel(
   }
 }
 
+
+// ------------------------------------------- Toeplitz GEMM on the MFMA units
+// Mixed-precision convolution (procsignal.cpp:659-673 as a matrix product):
+//   Y[v][i] = sum_j X[v][j] * T[j][i],  T[j][i] = dx * h[i - j + n - 1] = g[j - i + n - 1]
+// M = views, N = outputs, K = inputs.  X and g are split into bf16 hi + lo
+// (hi = bf16(x), lo = bf16(x - hi)) and the product is hi*hi + hi*lo + lo*hi
+// with FP32 accumulation, three v_mfma_f32_32x32x16_bf16 per tile and k-step.
+// A workgroup (4 waves) computes a 64-view x 128-output tile; wave (wm, wn)
+// owns rows wm*32.. and two 32-column MFMA tiles at wn*64.  The Toeplitz
+// operand is never materialised: for a k-chunk the window of g it needs is
+// staged in LDS as 8 copies shifted by one element, so the 8 consecutive
+// values a lane needs always start at an 8-element (16-byte) boundary of one
+// copy.
+#pragma clang fp contract(fast)
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx16_t __attribute__((ext_vector_type(16)));
+typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
+
+constexpr int kTM = 64, kTN = 128, kTK = 32;
+constexpr int kWin = kTN + kTK - 1;           // window of g per k-chunk
+constexpr int kWinPad = ((kWin + 7) / 8) * 8 + 8;
+
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+  __bf16 b = static_cast<__bf16>(x);
+  return __builtin_bit_cast(unsigned short, b);
+}
+
+__global__ __launch_bounds__(256) void toeplitz_mfma_kernel(
+    const float* __restrict__ in, const float* __restrict__ w32,
+    const unsigned short* __restrict__ gh, const unsigned short* __restrict__ gl,
+    float* __restrict__ out, int nView, int n, int outStride) {
+  __shared__ __attribute__((aligned(16))) unsigned short s_gh[8][kWinPad];
+  __shared__ __attribute__((aligned(16))) unsigned short s_gl[8][kWinPad];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
+  const int row = m0 + wm * 32 + (lane & 31);
+  const int kg = lane >> 5;
+  floatx16_t acc0 = {}, acc1 = {};
+  const int nq = 2 * n - 1;
+  for (int j0 = 0; j0 < n; j0 += kTK) {
+    const int qb = j0 - n0 - (kTN - 1) + n - 1;  // g index of window position 0
+    __syncthreads();
+    for (int idx = tid; idx < 8 * kWinPad; idx += 256) {
+      const int c = idx / kWinPad, m = idx % kWinPad;
+      const int q = qb + c + m;
+      const bool ok = q >= 0 && q < nq;
+      s_gh[c][m] = ok ? gh[q] : 0;
+      s_gl[c][m] = ok ? gl[q] : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kTK / 16; s++) {
+      // A fragment: X[row][j0 + 16 s + 8 kg + t], t = 0..7
+      const int k0 = j0 + 16 * s + 8 * kg;
+      bf16x8_t ah, al;
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        float x = 0.f;
+        const int k = k0 + t;
+        if (row < nView && k < n) {
+          x = in[static_cast<size_t>(row) * n + k];
+          if (w32) x *= w32[k];
+        }
+        const __bf16 h = static_cast<__bf16>(x);
+        ah[t] = h;
+        al[t] = static_cast<__bf16>(x - static_cast<float>(h));
+      }
+#pragma unroll
+      for (int tile = 0; tile < 2; tile++) {
+        // B fragment: g[j - i + n - 1] for j = k0 + t, i = column of this lane
+        const int cbase = wn * 64 + tile * 32;
+        const int p0 = (kTN - 1) + 16 * s + 8 * kg - cbase - (lane & 31);
+        const int c = p0 & 7, mpos = p0 - c;
+        const u16x8_t bh_bits = *reinterpret_cast<const u16x8_t*>(&s_gh[c][mpos]);
+        const u16x8_t bl_bits = *reinterpret_cast<const u16x8_t*>(&s_gl[c][mpos]);
+        const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, bh_bits);
+        const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, bl_bits);
+        floatx16_t& acc = tile == 0 ? acc0 : acc1;
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+      }
+    }
+  }
+  // D layout of the 32x32 tile: register r of lane l is row 8(r/4) + 4(l/32) + r%4,
+  // column l%32.
+#pragma unroll
+  for (int tile = 0; tile < 2; tile++) {
+    const floatx16_t& acc = tile == 0 ? acc0 : acc1;
+    const int col = n0 + wn * 64 + tile * 32 + (lane & 31);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int rr = m0 + wm * 32 + 8 * (r / 4) + 4 * kg + (r % 4);
+      if (rr < nView && col < n) out[static_cast<size_t>(rr) * outStride + col] = acc[r];
+    }
+  }
+}
+#pragma clang fp contract(off)
+
 // ------------------------------------------------------------ launchers
 
 template <class T, class TO>
@@ -323,6 +423,24 @@ int filter_upload(FilterPlan& p) {
   if (!p.preweight.empty()) {
     CTSIM_CHECK(p.d_weight.reserve(p.preweight.size() * sizeof(double)));
     CTSIM_HIP(hipMemcpy(p.d_weight.ptr, p.preweight.data(), p.preweight.size() * sizeof(double),
+                        hipMemcpyHostToDevice));
+    std::vector<float> w32(p.preweight.begin(), p.preweight.end());
+    CTSIM_CHECK(p.d_weight32.reserve(w32.size() * sizeof(float)));
+    CTSIM_HIP(hipMemcpy(p.d_weight32.ptr, w32.data(), w32.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  if (p.method == METHOD_CONVOLUTION) {
+    // Toeplitz operand of the mixed convolution: g[q] = dx * h[2n-2-q]
+    const int nq = 2 * p.n - 1;
+    std::vector<unsigned short> g16(2 * static_cast<size_t>(nq));
+    for (int q = 0; q < nq; q++) {
+      const float g = static_cast<float>(p.inc * p.filter[2 * p.n - 2 - q]);
+      const __bf16 hi = static_cast<__bf16>(g);
+      const __bf16 lo = static_cast<__bf16>(g - static_cast<float>(hi));
+      g16[q] = __builtin_bit_cast(unsigned short, hi);
+      g16[nq + q] = __builtin_bit_cast(unsigned short, lo);
+    }
+    CTSIM_CHECK(p.d_toeplitz.reserve(g16.size() * sizeof(unsigned short)));
+    CTSIM_HIP(hipMemcpy(p.d_toeplitz.ptr, g16.data(), g16.size() * sizeof(unsigned short),
                         hipMemcpyHostToDevice));
   }
   if (!p.hre.empty()) {
@@ -381,6 +499,16 @@ int filter_launch(ctsim_ctx* ctx, FilterPlan& p, int precision, const float* d_i
   const hipStream_t s = ctx->stream;
   const double* w = p.preweight.empty() ? nullptr : p.d_weight.as<double>();
   const bool f64 = precision == CTSIM_PREC_F64;
+  if (p.method == METHOD_CONVOLUTION && !f64) {
+    const int nq = 2 * p.n - 1;
+    const unsigned short* g = p.d_toeplitz.as<unsigned short>();
+    const float* w32 = p.preweight.empty() ? nullptr : p.d_weight32.as<float>();
+    hipLaunchKernelGGL(toeplitz_mfma_kernel, dim3((p.n + kTN - 1) / kTN, (nView + kTM - 1) / kTM),
+                       dim3(256), 0, s, d_in, w32, g, g + nq, static_cast<float*>(d_out), nView,
+                       p.n, outStride);
+    CTSIM_HIP(hipGetLastError());
+    return CTSIM_OK;
+  }
   if (p.method == METHOD_CONVOLUTION) {
     const dim3 grid((p.n + kWaves * kOutPerWave - 1) / (kWaves * kOutPerWave), (nView + kLanes - 1) / kLanes);
     const size_t lds = static_cast<size_t>(2 * p.n - 1) * sizeof(double);

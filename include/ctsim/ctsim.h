@@ -1,0 +1,320 @@
+// ctsim/ctsim.h -- C++ drop-in for the CTSim hot-path classes, backed by the
+// MI355X kernels of libctsim_amd.so through the C ABI of ctsim_hip.h.
+//
+// The classes keep the reference's names, constructor arguments, accessors
+// and fail()/failMessage() convention so tools written against CTSim
+// (tools/pjrec.cpp, tools/phm2pj.cpp) compile against this header:
+//   Phantom        include/phantom.h:133-228
+//   DetectorArray  include/scanner.h:41-62
+//   Scanner        include/scanner.h:64-156
+//   Projections    include/projections.h:103-219 (.pj files, Appendix B)
+//   ImageFile      include/imagefile.h:170-241 (float32 images, .if files)
+//   ProcessSignal  include/procsignal.h:37-160
+//   Backprojector  include/backprojectors.h:40-106
+//   Reconstructor  include/reconstruct.h:43-78
+// GUI-only arguments (SGP*) are accepted and ignored.  Compute runs on the
+// process's device (ctsim_amd::setDevice, default 0) in the precision chosen
+// by ctsim_amd::setPrecision or the CTSIM_PRECISION environment variable
+// ("f64", the default, reproduces the reference; "mixed" is the fast path).
+#ifndef CTSIM_CTSIM_H
+#define CTSIM_CTSIM_H
+
+#include <string>
+#include <vector>
+
+#include "../ctsim_hip.h"
+
+class SGP;  // the reference's graphics port; unused
+
+namespace ctsim_amd {
+void setDevice(int device);
+void setPrecision(int precision);  // CTSIM_PREC_F64 or CTSIM_PREC_MIXED
+int precision();
+ctsim_ctx* context();              // the process-wide context (created on first use)
+}  // namespace ctsim_amd
+
+// Trace levels (include/trace.h); only TRACE_NONE / TRACE_CONSOLE have effect.
+struct Trace {
+  enum { TRACE_INVALID = -1, TRACE_NONE = 0, TRACE_CONSOLE = 1, TRACE_PHANTOM = 2,
+         TRACE_PROJECTIONS = 3, TRACE_PLOT = 4, TRACE_CLIPPING = 5 };
+  static int convertTraceNameToID(const char* name);
+};
+
+struct ReconstructionROI {  // include/reconstruct.h:36-41
+  double m_dXMin;
+  double m_dYMin;
+  double m_dXMax;
+  double m_dYMax;
+};
+
+class Phantom {
+ public:
+  Phantom();
+  explicit Phantom(const char* phmName);
+  ~Phantom();
+  Phantom(const Phantom&) = delete;
+  Phantom& operator=(const Phantom&) = delete;
+  bool createFromPhantom(const char* phmName);
+  bool createFromFile(const char* fname);
+  void addPElem(const char* type, double cx, double cy, double u, double v, double rot,
+                double atten);
+  bool fail() const { return m_fail; }
+  const std::string& failMessage() const { return m_failMessage; }
+  const std::string& name() const { return m_name; }
+  int nPElem() const { return m_nPElem; }
+  double xmin() const;
+  double xmax() const;
+  double ymin() const;
+  double ymax() const;
+  double maxAxisLength() const;
+  double getDiameterBoundaryCircle() const;
+  ctsim_phantom* handle() const { return m_hip; }
+
+ private:
+  ctsim_phantom* m_hip = nullptr;
+  bool m_fail = false;
+  std::string m_failMessage, m_name;
+  int m_nPElem = 0;
+};
+
+class DetectorArray {
+ public:
+  int nDet() const { return m_nDet; }
+  double viewAngle() const { return m_viewAngle; }
+  float* detValues() { return m_detValues; }
+  const float* detValues() const { return m_detValues; }
+  void setViewAngle(double a) { m_viewAngle = a; }
+
+ private:
+  friend class Projections;
+  float* m_detValues = nullptr;
+  int m_nDet = 0;
+  double m_viewAngle = 0;
+};
+
+class Scanner;
+
+class Projections {
+ public:
+  Projections();
+  Projections(int nView, int nDet);
+  explicit Projections(const Scanner& scanner);
+  void initFromScanner(const Scanner& scanner);
+  bool read(const char* fname);
+  bool read(const std::string& fname) { return read(fname.c_str()); }
+  bool write(const char* fname);
+  bool write(const std::string& fname) { return write(fname.c_str()); }
+  DetectorArray& getDetectorArray(int iview) { return m_views[iview]; }
+  const DetectorArray& getDetectorArray(int iview) const { return m_views[iview]; }
+  int nDet() const { return m_nDet; }
+  int nView() const { return m_nView; }
+  int geometry() const { return m_geometry; }
+  double detInc() const { return m_detInc; }
+  double rotInc() const { return m_rotInc; }
+  double detStart() const { return m_detStart; }
+  double rotStart() const { return m_rotStart; }
+  double focalLength() const { return m_dFocalLength; }
+  double sourceDetectorLength() const { return m_dSourceDetectorLength; }
+  double viewDiameter() const { return m_dViewDiameter; }
+  double phmLen() const;
+  double fanBeamAngle() const { return m_dFanBeamAngle; }
+  double calcTime() const { return m_calcTime; }
+  const char* remark() const { return m_remark.c_str(); }
+  void setCalcTime(double t) { m_calcTime = t; }
+  void setRemark(const char* r) { m_remark = r; }
+  void setRemark(const std::string& r) { m_remark = r; }
+  void setRotInc(double r) { m_rotInc = r; }
+  void setDetInc(double d) { m_detInc = d; }
+  void setPhmLen(double p);
+  // Contiguous [nView][nDet] float data (the DetectorArrays point into it).
+  const float* data() const { return m_data.data(); }
+  float* data() { return m_data.data(); }
+
+ private:
+  void init(int nView, int nDet);
+  int m_nView = 0, m_nDet = 0, m_geometry = 0;
+  double m_detInc = 0, m_rotInc = 0, m_detStart = 0, m_rotStart = 0, m_dFocalLength = 0;
+  double m_dSourceDetectorLength = 0, m_dViewDiameter = 0, m_dFanBeamAngle = 0, m_calcTime = 0;
+  int m_year = 0, m_month = 0, m_day = 0, m_hour = 0, m_minute = 0, m_second = 0;
+  std::string m_remark;
+  std::vector<float> m_data;
+  std::vector<DetectorArray> m_views;
+};
+
+class Scanner {
+ public:
+  static const int GEOMETRY_INVALID = -1, GEOMETRY_PARALLEL = 0, GEOMETRY_EQUIANGULAR = 1,
+                   GEOMETRY_EQUILINEAR = 2, GEOMETRY_LINOGRAM = 3;
+  Scanner(const Phantom& phm, const char* geometryName, int nDet, int nView, int iOffsetView,
+          int nSample, double rot_anglen, double dFocalLengthRatio, double dCenterDetectorRatio,
+          double dViewRatio, double dScanRatio);
+  ~Scanner();
+  Scanner(const Scanner&) = delete;
+  Scanner& operator=(const Scanner&) = delete;
+  void collectProjections(Projections& proj, const Phantom& phm, int trace = Trace::TRACE_NONE,
+                          SGP* pSGP = nullptr);
+  void collectProjections(Projections& proj, const Phantom& phm, int iStartView, int iNumViews,
+                          int iOffsetView, bool bStoreAtViewPosition, int trace = Trace::TRACE_NONE,
+                          SGP* pSGP = nullptr);
+  void collectProjections(Projections& proj, const Phantom& phm, int iStartView, int iNumViews,
+                          int iOffsetView, int iStorageOffset, int trace = Trace::TRACE_NONE,
+                          SGP* pSGP = nullptr);
+  bool fail() const { return m_fail; }
+  const std::string& failMessage() const { return m_failMessage; }
+  unsigned int nDet() const { return m_nDet; }
+  unsigned int nView() const { return m_nView; }
+  unsigned int offsetView() const { return m_offsetView; }
+  unsigned int startView() const { return 0; }
+  double rotInc() const { return m_info[2]; }
+  double detInc() const { return m_info[0]; }
+  double detLen() const { return m_info[8]; }
+  double detStart() const { return m_info[1]; }
+  double focalLength() const { return m_info[4]; }
+  double sourceDetectorLength() const { return m_info[5]; }
+  double centerDetectorLength() const { return m_info[9]; }
+  double viewDiameter() const { return m_info[3]; }
+  double scanDiameter() const { return m_info[10]; }
+  double fanBeamAngle() const { return m_info[6]; }
+  int geometry() const { return m_geometry; }
+  static int convertGeometryNameToID(const char* name);
+  static const char* convertGeometryIDToName(int id);
+
+ private:
+  ctsim_scanner* m_hip = nullptr;
+  bool m_fail = false;
+  std::string m_failMessage;
+  int m_nDet = 0, m_nView = 0, m_offsetView = 0, m_geometry = -1;
+  double m_info[12] = {};
+};
+
+class ImageFile {
+ public:
+  ImageFile();
+  ImageFile(int nx, int ny);
+  bool setArraySize(int nx, int ny);
+  unsigned int nx() const { return m_nx; }
+  unsigned int ny() const { return m_ny; }
+  float** getArray() { return m_cols.empty() ? nullptr : m_cols.data(); }
+  const float* const* getArray() const { return m_cols.empty() ? nullptr : m_cols.data(); }
+  float* data() { return m_data.data(); }          // column-major [nx][ny]
+  const float* data() const { return m_data.data(); }
+  void arrayDataClear();
+  void setAxisIncrement(double incX, double incY);
+  void setAxisExtent(double minX, double maxX, double minY, double maxY);
+  void labelAdd(const char* label, double calcTime = 0);
+  bool fileWrite(const char* fname);
+  bool fileWrite(const std::string& fname) { return fileWrite(fname.c_str()); }
+  bool fileRead(const char* fname);
+  bool fileRead(const std::string& fname) { return fileRead(fname.c_str()); }
+  // Herman's d, r, e with this image as the reference (imagefile.cpp:185-240).
+  bool comparativeStatistics(const ImageFile& imComp, double& d, double& r, double& e) const;
+
+ private:
+  struct Label {
+    int type;
+    double calcTime;
+    std::string text;
+    int date[6];
+  };
+  unsigned int m_nx = 0, m_ny = 0;
+  std::vector<float> m_data;
+  std::vector<float*> m_cols;
+  bool m_axisIncKnown = false, m_axisExtentKnown = false;
+  double m_incX = 0, m_incY = 0, m_minX = 0, m_maxX = 0, m_minY = 0, m_maxY = 0;
+  std::vector<Label> m_labels;
+};
+
+class ProcessSignal {
+ public:
+  ProcessSignal(const char* szFilterName, const char* szFilterMethodName, double dBandwidth,
+                double dSignalIncrement, int nSignalPoints, double dFilterParam,
+                const char* szDomainName, const char* szFilterGenerationName, int iZeropad,
+                int iPreinterpolationFactor, int iTraceLevel, int iGeometry, double dFocalLength,
+                double dSourceDetectorLength, SGP* pSGP = nullptr);
+  ~ProcessSignal();
+  ProcessSignal(const ProcessSignal&) = delete;
+  ProcessSignal& operator=(const ProcessSignal&) = delete;
+  // One view, FP64 (procsignal.cpp:544-629).
+  void filterSignal(const float input[], double output[]) const;
+  bool fail() const { return m_fail; }
+  const std::string& failMessage() const { return m_failMessage; }
+  int getNFilterPoints() const { return m_nFilterPoints; }
+  double getFilterMin() const { return m_minMaxInc[0]; }
+  double getFilterMax() const { return m_minMaxInc[1]; }
+  double getFilterIncrement() const { return m_minMaxInc[2]; }
+  const double* getFilter() const { return m_filter.data(); }
+  ctsim_procsignal* handle() const { return m_hip; }
+  int nSignalPoints() const { return m_n; }
+
+ private:
+  ctsim_procsignal* m_hip = nullptr;
+  bool m_fail = false;
+  std::string m_failMessage;
+  int m_n = 0, m_nFilterPoints = 0;
+  double m_minMaxInc[3] = {};
+  std::vector<double> m_filter;
+  mutable void* m_dIn = nullptr;
+  mutable void* m_dOut = nullptr;
+};
+
+class Backprojector {
+ public:
+  Backprojector(const Projections& proj, ImageFile& im, const char* backprojName,
+                const char* interpName, int interpFactor, const ReconstructionROI* pROI = nullptr);
+  ~Backprojector();
+  Backprojector(const Backprojector&) = delete;
+  Backprojector& operator=(const Backprojector&) = delete;
+  // Views are queued and backprojected in order on the device; the host image
+  // is updated by PostProcessing() (or the destructor, unscaled).
+  void BackprojectView(const double* viewData, double viewAngle);
+  void PostProcessing();
+  bool fail() const { return m_fail; }
+  const std::string& failMessage() const { return m_failMessage; }
+  // Device-side batch: nView filtered rows already on the device.
+  int backprojectDevice(const void* d_filtered, const double* angles, int nView, int precision);
+  static int convertBackprojectNameToID(const char* name);
+  static int convertInterpNameToID(const char* name);
+
+ private:
+  int flush();
+  int download();
+  ctsim_backprojector* m_hip = nullptr;
+  ImageFile& m_im;
+  bool m_fail = false, m_postDone = false;
+  std::string m_failMessage;
+  int m_nDet = 0;
+  float* m_dImage = nullptr;
+  void* m_dViews = nullptr;
+  int m_capacity = 0;
+  std::vector<double> m_queue, m_angles;
+};
+
+class Reconstructor {
+ public:
+  Reconstructor(const Projections& rProj, ImageFile& rIF, const char* filterName,
+                double filt_param, const char* filterMethodName, int zeropad,
+                const char* filterGenerationName, const char* interpName, int interpFactor,
+                const char* backprojectName, int iTrace, ReconstructionROI* pROI = nullptr,
+                bool bRebinToParallel = false, SGP* pSGP = nullptr);
+  ~Reconstructor();
+  Reconstructor(const Reconstructor&) = delete;
+  Reconstructor& operator=(const Reconstructor&) = delete;
+  bool fail() const { return m_bFail; }
+  const std::string& failMessage() const { return m_strFailMessage; }
+  void reconstructAllViews();
+  void reconstructView(int iStartView = 0, int iViewCount = -1, SGP* pSGP = nullptr,
+                       bool bBackprojectView = true, double dGraphWidth = 1.);
+  void postProcessing();
+
+ private:
+  const Projections& m_proj;
+  ProcessSignal* m_pProcessSignal = nullptr;
+  Backprojector* m_pBackprojector = nullptr;
+  int m_iTrace = 0;
+  bool m_bFail = false;
+  std::string m_strFailMessage;
+  float* m_dSino = nullptr;
+  void* m_dFiltered = nullptr;
+};
+
+#endif  // CTSIM_CTSIM_H

@@ -52,10 +52,10 @@ typedef struct ctsim_backprojector ctsim_backprojector;
 
 /* ------------------------------------------------------------- context */
 
-/* Bind a context to a device and a stream (NULL: the context creates its own
- * default-priority stream).  No counterpart in the reference, which is
- * single-threaded CPU code; it carries the device state the classes below
- * need. */
+/* Bind a context to a device and a stream (a hipStream_t; NULL is the null
+ * stream).  Every call on the context is ordered on that stream.  No
+ * counterpart in the reference, which is single-threaded CPU code; it
+ * carries the device state the classes below need. */
 int ctsim_ctx_create(ctsim_ctx** out, int device, void* stream);
 void ctsim_ctx_destroy(ctsim_ctx* ctx);
 int ctsim_ctx_set_stream(ctsim_ctx* ctx, void* stream);

@@ -300,3 +300,63 @@ def fbp_sample(kind: str, phm: Phantom, geometry: str, nDet: int, nView: int, nS
     if rc:
         raise RuntimeError(f"{kind} fbp_sample failed: {getattr(L, pre + '_last_error')().decode()}")
     return img, tim[0], tim[1]
+
+
+def ref_write_pj(path: str, builtin: str, geometry: str, nDet: int, nView: int, nSample: int,
+                 rotLen: float, focalRatio: float, centerRatio: float, data: np.ndarray,
+                 angles: np.ndarray, calcTime: float, remark: str) -> None:
+    """Write a .pj file with the reference's Projections::write, geometry
+    from the reference's Scanner on a built-in phantom."""
+    L = lib("ref")
+    f = L.ref_write_pj
+    c, i, d, vp = C.c_char_p, C.c_int, C.c_double, C.c_void_p
+    f.restype = i
+    f.argtypes = [c, c, c, i, i, i, d, d, d, vp, vp, d, c]
+    data = np.ascontiguousarray(data, np.float32)
+    angles = np.ascontiguousarray(angles, np.float64)
+    rc = f(_b(path), _b(builtin), _b(geometry), nDet, nView, nSample, rotLen, focalRatio,
+           centerRatio, _ptr(data), _ptr(angles), calcTime, _b(remark))
+    if rc:
+        raise RuntimeError(f"ref_write_pj failed ({rc}): {L.ref_last_error().decode()}")
+
+
+def ref_read_pj(path: str, cap_views: int, nDet: int):
+    """Read a .pj file with the reference's Projections::read; returns
+    (info[12], data[nView][nDet], angles)."""
+    L = lib("ref")
+    f = L.ref_read_pj
+    f.restype = C.c_int
+    f.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    info = np.zeros(12, np.float64)
+    data = np.zeros((cap_views, nDet), np.float32)
+    ang = np.zeros(cap_views, np.float64)
+    if f(_b(path), _ptr(info), _ptr(data), _ptr(ang), cap_views):
+        raise RuntimeError(f"ref_read_pj failed: {L.ref_last_error().decode()}")
+    return info, data, ang
+
+
+def ref_write_if(path: str, image: np.ndarray, inc, extent, label: str, calcTime: float) -> None:
+    """Write an [nx][ny] float32 image with the reference's ImageFile::fileWrite."""
+    L = lib("ref")
+    f = L.ref_write_if
+    d = C.c_double
+    f.restype = C.c_int
+    f.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_void_p, d, d, d, d, d, d, C.c_char_p, d]
+    im = np.ascontiguousarray(image, np.float32)
+    nx, ny = im.shape
+    if f(_b(path), nx, ny, _ptr(im), inc[0], inc[1], *extent, _b(label), calcTime):
+        raise RuntimeError("ref_write_if failed")
+
+
+def ref_read_if(path: str, cap: int) -> np.ndarray:
+    """Read a float32 .if image with the reference's ImageFile::fileRead; [nx][ny]."""
+    L = lib("ref")
+    f = L.ref_read_if
+    f.restype = C.c_int
+    f.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_int]
+    nxny = np.zeros(2, np.int32)
+    buf = np.zeros(cap, np.float32)
+    if f(_b(path), _ptr(nxny), _ptr(buf), cap):
+        raise RuntimeError(f"ref_read_if failed: {L.ref_last_error().decode()}")
+    nx, ny = int(nxny[0]), int(nxny[1])
+    return buf[: nx * ny].reshape(nx, ny)

@@ -232,3 +232,72 @@ int ref_fbp_sample(const int* types, const double* params, int nElem, const char
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// File formats through the reference's own writers and readers
+// (Projections::write/read, projections.cpp:486-546 and 817-889;
+// Array2dFile::fileWrite/fileRead, array2dfile.cpp:443-633), so tests can
+// check the drop-in's .pj and .if files byte for byte against them.
+int ref_write_pj(const char* path, const char* builtin, const char* geometry, int nDet, int nView,
+                 int nSample, double rotLen, double focalRatio, double centerRatio,
+                 const float* data, const double* angles, double calcTime, const char* remark) {
+  Phantom phm;
+  buildPhantom(phm, nullptr, nullptr, 0, builtin);
+  Scanner scanner(phm, geometry, nDet, nView, 0, nSample, rotLen, focalRatio, centerRatio, 1.0, 1.0);
+  if (scanner.fail()) { g_err = scanner.failMessage(); return 2; }
+  Projections pj(scanner);
+  pj.setCalcTime(calcTime);
+  pj.setRemark(remark);
+  for (int v = 0; v < nView; v++) {
+    DetectorArray& d = pj.getDetectorArray(v);
+    d.setViewAngle(angles[v]);
+    for (int i = 0; i < nDet; i++) d.detValues()[i] = data[(size_t)v * nDet + i];
+  }
+  return pj.write(path) ? 0 : 1;
+}
+
+// Reads a .pj file: info = {nView, nDet, geometry, calcTime, rotStart,
+// rotInc, detStart, detInc, viewDiameter, focalLength, SDL, 0} (the fan-beam
+// angle has no public accessor);
+// data/angles receive at most cap views.
+int ref_read_pj(const char* path, double* info, float* data, double* angles, int cap) {
+  Projections pj;
+  if (!pj.read(path)) { g_err = "read failed"; return 1; }
+  const double v[12] = {(double)pj.nView(), (double)pj.nDet(), (double)pj.geometry(),
+                        pj.calcTime(), pj.rotStart(), pj.rotInc(), pj.detStart(), pj.detInc(),
+                        pj.viewDiameter(), pj.focalLength(), pj.sourceDetectorLength(), 0.0};
+  for (int i = 0; i < 12; i++) info[i] = v[i];
+  for (int k = 0; k < pj.nView() && k < cap; k++) {
+    const DetectorArray& d = pj.getDetectorArray(k);
+    angles[k] = d.viewAngle();
+    for (int i = 0; i < pj.nDet(); i++) data[(size_t)k * pj.nDet() + i] = d.detValues()[i];
+  }
+  return 0;
+}
+
+// image is column-major [nx][ny], as copyImage writes it.
+int ref_write_if(const char* path, int nx, int ny, const float* image, double incX, double incY,
+                 double minX, double maxX, double minY, double maxY, const char* label,
+                 double calcTime) {
+  ImageFile im(nx, ny);
+  ImageFileArray a = im.getArray();
+  for (int ix = 0; ix < nx; ix++)
+    for (int iy = 0; iy < ny; iy++) a[ix][iy] = image[(size_t)ix * ny + iy];
+  im.setAxisIncrement(incX, incY);
+  im.setAxisExtent(minX, maxX, minY, maxY);
+  im.labelAdd(label, calcTime);
+  return im.fileWrite(path) ? 0 : 1;
+}
+
+int ref_read_if(const char* path, int* nxny, float* image, int cap) {
+  ImageFile im;
+  if (!im.fileRead(path)) { g_err = "read failed"; return 1; }
+  nxny[0] = im.nx();
+  nxny[1] = im.ny();
+  if ((long)im.nx() * im.ny() > cap) return 2;
+  copyImage(im, image);
+  return 0;
+}
+
+}  // extern "C"

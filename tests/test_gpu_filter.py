@@ -144,3 +144,25 @@ def test_fourier_hanning_cubic_end_to_end(gpu_ctx, geom, nDet):
         assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
     else:
         assert np.abs(img - ref).max() <= 4 * np.spacing(np.float32(np.abs(ref).max()))
+
+
+@pytest.mark.parametrize("n", [33, 130, 367, 1024])
+def test_mixed_convolution_toeplitz_mfma(gpu_ctx, oracle_lib, n):
+    """Mixed convolution runs as a split-BF16 Toeplitz product on the MFMA
+    units: its error against the FP64 oracle is bounded by the bf16 hi+lo
+    split (about 2^-16 relative per product term)."""
+    import ctsim_amd
+    O = oracle_lib
+    rng = np.random.default_rng(n)
+    for geom, f in itertools.product((0, 1, 2), ("abs_bandlimit", "shepp")):
+        sig = rng.standard_normal((70, n)).astype(np.float32)
+        inc = 2.0 / n
+        ref, _, _ = O.filter_views("oracle", sig, f, "convolution", bw=1 / inc, inc=inc,
+                                   geometry=geom, focal=2.0, sdl=4.0)
+        mix, _, _ = _filter(gpu_ctx, sig, ctsim_amd.PREC_MIXED, filter=f, method="convolution",
+                            bw=1 / inc, signalInc=inc, n=n, geometry=geom, focalLength=2.0,
+                            sourceDetectorLength=4.0)
+        err = np.abs(mix - ref).max() / np.abs(ref).max()
+        rms = np.sqrt(np.mean((mix - ref) ** 2) / np.mean(ref ** 2))
+        print(f"  n={n} geom={geom} {f}: max rel {err:.2e}, rms rel {rms:.2e}")
+        assert err <= 1e-4 and rms <= 2e-5
