@@ -20,7 +20,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import platform
 import subprocess
@@ -44,16 +43,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def herman_dre(ref, img):
-    import numpy as np
-    ref = ref.astype(np.float64)
-    img = img.astype(np.float64)
-    d = math.sqrt(((ref - img) ** 2).sum() / ((ref - ref.mean()) ** 2).sum())
-    r = np.abs(ref - img).sum() / np.abs(ref).sum()
-    nx, ny = ref.shape
-    a = ref[: nx // 2 * 2, : ny // 2 * 2].reshape(nx // 2, 2, ny // 2, 2).mean(axis=(1, 3))
-    b = img[: nx // 2 * 2, : ny // 2 * 2].reshape(nx // 2, 2, ny // 2, 2).mean(axis=(1, 3))
-    return d, r, float(np.abs(a - b).max())
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "bp_mixed_equiangular_cubic_512_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the main backprojection kernel, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or
+    None when the summary is absent."""
+    try:
+        with open(TRAFFIC_SUMMARY) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_model():
@@ -165,47 +169,54 @@ def main():
     img64 = torch.empty((spec.nx, spec.ny), dtype=torch.float32, device=dev)
     img32 = torch.empty((spec.nx, spec.ny), dtype=torch.float32, device=dev)
 
-    def fbp(prec, filt, img, ev=None):
+    def fbp(prec, filt, img):
         img.zero_()
         ps.filter(sino, filt, prec)
-        if ev is not None:
-            ev[0].record()
         bp.backproject(filt, ang, img, prec)
-        if ev is not None:
-            ev[1].record()
         bp.postprocess(img)
 
     # --- plain FP64 GPU path
     fbp(ctsim_amd.PREC_F64, f64, img64)
     torch.cuda.synchronize()
+    ctx.profile(True)
     t0 = time.perf_counter()
     for _ in range(args.fp64_steps):
         fbp(ctsim_amd.PREC_F64, f64, img64)
     torch.cuda.synchronize()
     t_f64 = (time.perf_counter() - t0) / args.fp64_steps
-    log(f"[bench] FP64 path: {t_f64 * 1e3:.2f} ms/slice")
+    k64_ms, k64_n = ctx.profile_read()
+    ctx.profile(False)
+    log(f"[bench] FP64 path: {t_f64 * 1e3:.2f} ms/slice (backprojection kernel {k64_ms / k64_n:.3f} ms)")
+
+    # --- mixed path: launch shape from the autotuner (setup, untimed)
+    ps.filter(sino, f32, ctsim_amd.PREC_MIXED)
+    shape = bp.autotune(f32, ang)
+    log(f"[bench] autotuned mixed shape: lanesY={shape[0]} P={shape[1]} split={shape[2]} "
+        f"chunk={shape[3]} window={shape[4]} ({shape[5]:.3f} ms)")
 
     # --- mixed path: warmup, then exactly K timed steps
     for _ in range(args.warmup):
         fbp(ctsim_amd.PREC_MIXED, f32, img32)
-    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(args.steps)]
     torch.cuda.synchronize()
+    ctx.profile(True)
     t0 = time.perf_counter()
     for k in range(args.steps):
-        fbp(ctsim_amd.PREC_MIXED, f32, img32, events[k])
+        fbp(ctsim_amd.PREC_MIXED, f32, img32)
     torch.cuda.synchronize()
     t_mixed = (time.perf_counter() - t0) / args.steps
-    bp_ms = float(np.mean([a.elapsed_time(b) for a, b in events]))
+    kms, kn = ctx.profile_read()
+    ctx.profile(False)
+    bp_ms = kms / kn
     log(f"[bench] mixed path: {t_mixed * 1e3:.3f} ms/slice (backprojection kernel {bp_ms:.3f} ms)")
 
-    d, r, e = herman_dre(img64.cpu().numpy(), img32.cpu().numpy())
+    d, r, e = ctsim_amd.image_stats(ctx, img64, img32)
     log(f"[bench] mixed vs FP64: d={d:.3e} r={r:.3e} e={e:.3e}")
 
     pvu = spec.nx * spec.ny * spec.nView
     flops = FLOPS_PER_PVU_EQUIANGULAR_CUBIC_KAHAN * pvu
     achieved = flops / (bp_ms * 1e-3) / 1e12
     hbm_bytes = 4 * spec.nView * spec.nDet + 2 * 4 * spec.nx * spec.ny
+    traffic = measured_traffic()
 
     out = {
         "metric": METRIC,
@@ -224,16 +235,20 @@ def main():
                                "abs_hanning by FFT (2048-point), cubic, 512x512",
                    "slices_per_step": 1, "precision": "mixed"},
         "fp64_gpu": {"slices_per_s": 1.0 / t_f64, "ms_per_slice": t_f64 * 1e3,
+                     "backprojection_kernel_ms": k64_ms / k64_n,
                      "speedup_mixed_over_fp64": t_f64 / t_mixed},
         "accuracy_vs_fp64": {"d": d, "r": r, "e": e, "target_d": 1e-4},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "bp_f32_kernel<equiangular, cubic>", "kernel_ms": bp_ms,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": traffic,
+                     "kernel": "bp_mixed_kernel<equiangular, cubic>", "kernel_ms": bp_ms,
                      "flops_per_pvu": FLOPS_PER_PVU_EQUIANGULAR_CUBIC_KAHAN,
                      "pvu_per_launch": pvu,
                      "hbm_algorithmic_bytes": hbm_bytes,
                      "hbm_frac": hbm_bytes / (bp_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                     "pvu_per_s": pvu / (bp_ms * 1e-3)},
+                     "pvu_per_s": pvu / (bp_ms * 1e-3),
+                     "launch_shape": {"lanesY": shape[0], "P": shape[1], "split": shape[2],
+                                      "chunk": shape[3], "window": shape[4]}},
         "raysum": {"fp64_ms": t_raysum64 * 1e3, "closed_form_ms": t_raysum_m * 1e3,
                    "ray_element_pairs": n_re},
     }

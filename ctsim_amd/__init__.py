@@ -16,8 +16,8 @@ import importlib
 __version__ = "0.1"
 
 _API = {"Context", "Backprojector", "ProcessSignal", "Phantom", "Scanner", "CtsimError",
-        "PREC_F64", "PREC_MIXED",
-        "check"}
+        "PREC_F64", "PREC_MIXED", "image_stats", "helical180li",
+        "halfscan_feather", "check"}
 
 
 def __getattr__(name):

@@ -45,6 +45,13 @@ SIGNATURES = {
     "ctsim_scanner_destroy": (None, [_vp]),
     "ctsim_scanner_info": (_i, [_vp, _vp]),
     "ctsim_collect_projections": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp]),
+    "ctsim_image_stats": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
+    "ctsim_helical180li": (_i, [_vp, _i, _i, _i, _d, _d, _d, _d, _i, _vp, _vp, _i, _vp, _vp]),
+    "ctsim_halfscan_feather": (_i, [_vp, _i, _i, _i, _d, _d, _d, _d, _vp, _vp]),
+    "ctsim_ctx_profile": (_i, [_vp, _i]),
+    "ctsim_ctx_profile_read": (_i, [_vp, _vp, _vp]),
+    "ctsim_backprojector_autotune": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp]),
+    "ctsim_backprojector_set_shape": (_i, [_vp, _vp]),
 }
 
 for _name, (_res, _args) in SIGNATURES.items():

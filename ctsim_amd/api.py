@@ -57,6 +57,17 @@ class Context:
     def synchronize(self):
         check(lib.ctsim_ctx_synchronize(self.handle), "ctsim_ctx_synchronize")
 
+    def profile(self, enable: bool = True):
+        """Bracket every main backprojection kernel with HIP events."""
+        check(lib.ctsim_ctx_profile(self.handle, int(enable)), "ctsim_ctx_profile")
+
+    def profile_read(self) -> tuple[float, int]:
+        """(summed kernel ms, launches) since the last read."""
+        ms = np.zeros(1, np.float64)
+        n = np.zeros(1, np.int32)
+        check(lib.ctsim_ctx_profile_read(self.handle, _hptr(ms), _hptr(n)), "ctsim_ctx_profile_read")
+        return float(ms[0]), int(n[0])
+
     def close(self):
         if getattr(self, "handle", None):
             lib.ctsim_ctx_destroy(self.handle)
@@ -64,6 +75,53 @@ class Context:
 
     def __del__(self):
         self.close()
+
+
+def image_stats(ctx: Context, ref, cmp) -> tuple[float, float, float]:
+    """Herman's d, r, e of two device images [nx][ny] float32 (ref first),
+    ImageFile::comparativeStatistics (libctsim/imagefile.cpp:185-240)."""
+    if ref.shape != cmp.shape or ref.dim() != 2:
+        raise ValueError("image_stats: images must be 2-D and of one shape")
+    out = np.zeros(3, np.float64)
+    nx, ny = ref.shape
+    check(lib.ctsim_image_stats(ctx.handle, C.c_void_p(_dptr(ref)), C.c_void_p(_dptr(cmp)), nx,
+                                ny, _hptr(out)), "ctsim_image_stats")
+    return float(out[0]), float(out[1]), float(out[2])
+
+
+def helical180li(ctx: Context, geometry, sino, rotInc: float, detInc: float, fanBeamAngle: float,
+                 sourceDetectorLength: float, interpView: int = -1):
+    """Projections::Helical180LI (libctsim/projections.cpp:200-358) of a device
+    sinogram float32 [nView, nDet]; returns (device [L+1, nDet], angles)."""
+    import torch
+    g = GEOMETRY_IDS[geometry] if isinstance(geometry, str) else int(geometry)
+    if sino.dtype != torch.float32 or sino.dim() != 2:
+        raise TypeError("sino must be float32 [nView, nDet]")
+    nView, nDet = sino.shape
+    cap = int((np.pi + fanBeamAngle) / rotInc) + 2
+    out = torch.zeros((cap, nDet), dtype=torch.float32, device=sino.device)
+    ang = np.zeros(cap, np.float64)
+    n = C.c_int()
+    check(lib.ctsim_helical180li(ctx.handle, g, nDet, nView, rotInc, detInc, fanBeamAngle,
+                                 sourceDetectorLength, interpView, C.c_void_p(_dptr(sino)),
+                                 C.c_void_p(_dptr(out)), cap, C.byref(n), _hptr(ang)),
+          "ctsim_helical180li")
+    return out[: n.value], ang[: n.value].copy()
+
+
+def halfscan_feather(ctx: Context, geometry, views, rotInc: float, detInc: float,
+                     fanBeamAngle: float, sourceDetectorLength: float):
+    """Projections::HalfScanFeather (projections.cpp:369-439); returns a new
+    device array."""
+    import torch
+    g = GEOMETRY_IDS[geometry] if isinstance(geometry, str) else int(geometry)
+    src = views.contiguous()
+    out = torch.empty_like(src)
+    nView, nDet = src.shape
+    check(lib.ctsim_halfscan_feather(ctx.handle, g, nDet, nView, rotInc, detInc, fanBeamAngle,
+                                     sourceDetectorLength, C.c_void_p(_dptr(src)),
+                                     C.c_void_p(_dptr(out))), "ctsim_halfscan_feather")
+    return out
 
 
 class Backprojector:
@@ -82,7 +140,7 @@ class Backprojector:
             sourceDetectorLength, backproj.encode(), interp.encode(), interpFactor, nx, ny,
             None if roi_a is None else _hptr(roi_a)), "ctsim_backprojector_create")
         self.handle = h
-        self.nx, self.ny, self.nDet = nx, ny, nDet
+        self.nx, self.ny, self.nDet, self.interpFactor = nx, ny, nDet, interpFactor
         self._roi = roi_a
 
     def info(self):
@@ -99,6 +157,8 @@ class Backprojector:
             raise TypeError(f"filtered must be {want} for this precision")
         if image.dtype != torch.float32 or tuple(image.shape) != (self.nx, self.ny):
             raise TypeError("image must be float32 [nx, ny]")
+        if filtered.dim() != 2 or filtered.shape[1] != self.nDet * self.interpFactor:
+            raise ValueError("filtered must be [nView, nDet * interpFactor]")
         ang = np.ascontiguousarray(angles, dtype=np.float64)
         nView = filtered.shape[0]
         if ang.shape[0] < nView:
@@ -106,6 +166,28 @@ class Backprojector:
         check(lib.ctsim_backproject_views(self.ctx.handle, self.handle, precision,
                                           C.c_void_p(_dptr(filtered)), _hptr(ang), nView,
                                           C.c_void_p(_dptr(image))), "ctsim_backproject_views")
+
+    def autotune(self, filtered, angles: np.ndarray):
+        """Times the MIXED kernel's launch shapes on these float32 rows and
+        keeps the fastest; returns (lanesY, P, split, chunk, window, ms)."""
+        import torch
+        if filtered.dtype != torch.float32 or filtered.dim() != 2 or \
+                filtered.shape[1] != self.nDet * self.interpFactor:
+            raise TypeError("filtered must be float32 [nView, nDet * interpFactor]")
+        ang = np.ascontiguousarray(angles, dtype=np.float64)
+        if ang.shape[0] < filtered.shape[0]:
+            raise ValueError("not enough view angles")
+        shape = np.zeros(5, np.int32)
+        ms = np.zeros(1, np.float64)
+        check(lib.ctsim_backprojector_autotune(self.ctx.handle, self.handle,
+                                               C.c_void_p(_dptr(filtered)), _hptr(ang),
+                                               filtered.shape[0], _hptr(shape), _hptr(ms)),
+              "ctsim_backprojector_autotune")
+        return (*map(int, shape), float(ms[0]))
+
+    def set_shape(self, lanesY: int, P: int, split: int, chunk: int):
+        s = np.array([lanesY, P, split, chunk], np.int32)
+        check(lib.ctsim_backprojector_set_shape(self.handle, _hptr(s)), "ctsim_backprojector_set_shape")
 
     def postprocess(self, image):
         check(lib.ctsim_backproject_postprocess(self.ctx.handle, self.handle,

@@ -29,6 +29,8 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxChunk = 64;
+// Dynamic LDS of the mixed kernel: 64 KB per workgroup less its static arrays.
+constexpr size_t kMixedLdsMax = 62 * 1024;
 
 struct BpArgs {
   int nx, ny, nDet, iDetCenter, nView, fstride;
@@ -352,55 +354,73 @@ __global__ __launch_bounds__(64) void bp_diff_kernel(BpArgs a) {
 // ----------------------------------------------------------- mixed path
 #pragma clang fp contract(fast)
 
-// Same interpolation rules in FP32 (mixed path).
-__device__ __forceinline__ float cubic_f32(const float* w, int lo, int Wv, int n, float x) {
-  const float fl = floorf(x);
-  const int l = static_cast<int>(fl) - 1;
-  const int hi = l + 3;
-  if (l < -1 || hi > n) return 0.f;
-  if (l == -1) {
-    const float y0 = wget(w, lo, Wv, 0), y1 = wget(w, lo, Wv, 1);
-    return y0 + x * (y1 - y0);
+// The mixed path interpolates from per-interval coefficients: on detector
+// interval i (position p in [i, i+1), u = p - i) the reference's interpolant
+// is the polynomial a0 + u (a1 + u (a2 + u a3)), zero outside the detector,
+// with its end rules folded in (CubicPolyInterpolator::interpolate,
+// interpolator.cpp:36-75; linear with the IntDiff last-cell rule,
+// backprojectors.cpp:635-638).  coef_kernel writes them once per view, in
+// FP64 rounded to FP32, so the backprojection inner loop is a window lookup
+// and three FMAs.
+template <int INTERP, bool LAST_INCL>
+__global__ __launch_bounds__(256) void coef_kernel(const float* __restrict__ f, int fstride,
+                                                   int nDet, int nView, int blocksPerRow,
+                                                   void* __restrict__ out) {
+  const int v = blockIdx.x / blocksPerRow;
+  const int i = (blockIdx.x - v * blocksPerRow) * 256 + threadIdx.x;
+  if (i >= nDet || v >= nView) return;
+  const float* row = f + static_cast<size_t>(v) * fstride;
+  const size_t o = static_cast<size_t>(v) * nDet + i;
+  const int n = nDet;
+  if constexpr (INTERP == INTERP_CUBIC) {
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    if (i == 0) {
+      if (n >= 2) { a0 = row[0]; a1 = static_cast<double>(row[1]) - row[0]; }
+    } else if (i + 2 > n) {
+      // beyond the last full interval: zero
+    } else if (i + 2 == n) {
+      a0 = row[n - 2];
+      a1 = static_cast<double>(row[n - 1]) - row[n - 2];
+    } else {
+      const double y0 = row[i - 1], y1 = row[i], y2 = row[i + 1], y3 = row[i + 2];
+      a0 = y1;
+      a1 = -y0 / 3 - y1 / 2 + y2 - y3 / 6;
+      a2 = (y0 + y2) / 2 - y1;
+      a3 = (y3 - y0) / 6 + (y1 - y2) / 2;
+    }
+    static_cast<float4*>(out)[o] = make_float4(static_cast<float>(a0), static_cast<float>(a1),
+                                               static_cast<float>(a2), static_cast<float>(a3));
+  } else {
+    const int hiOk = LAST_INCL ? n - 1 : n - 2;
+    float a0 = 0.f, a1 = 0.f;
+    if (i <= hiOk) {
+      a0 = row[i];
+      a1 = i < n - 1 ? static_cast<float>(static_cast<double>(row[i + 1]) - row[i]) : 0.f;
+    }
+    static_cast<float2*>(out)[o] = make_float2(a0, a1);
   }
-  if (hi == n) {
-    const float frac = x - fl;
-    const float a = wget(w, lo, Wv, n - 2), b = wget(w, lo, Wv, n - 1);
-    return a + frac * (b - a);
-  }
-  // Lagrange basis on nodes l..l+3 with t = x - (l+1) in [0,1).
-  const float t = x - fl;
-  const float tm1 = t - 1.f, tp1 = t + 1.f, tm2 = t - 2.f;
-  const float c0 = -(1.f / 6.f) * t * tm1 * tm2;
-  const float c1 = 0.5f * tp1 * tm1 * tm2;
-  const float c2 = -0.5f * tp1 * t * tm2;
-  const float c3 = (1.f / 6.f) * tp1 * t * tm1;
-  const float* p = w + (l - lo);
-  const int k = l - lo;
-  if (k < 0 || k + 3 >= Wv) {
-    return c0 * wget(w, lo, Wv, l) + c1 * wget(w, lo, Wv, l + 1) + c2 * wget(w, lo, Wv, l + 2) +
-           c3 * wget(w, lo, Wv, l + 3);
-  }
-  return c0 * p[0] + c1 * p[1] + c2 * p[2] + c3 * p[3];
 }
 
-// atan for FP32: odd polynomial t*P(t^2) on [0,1] (weighted least-squares fit
-// of atan(t)/t in t^2, max abs error 1.3e-7), with atan(t) = pi/2 - atan(1/t)
-// beyond 1.
-template <bool SMALL_T>
+struct MixedArgs {
+  int nx, ny, nDet, nView, lanesY, tilesX, tilesY, W, chunk, split, viewsPerGroup;
+  float center, F, twoF, posScale, endScale;
+  float atan[9];
+  const float* xs;
+  const float* ys;
+  const ViewConstF32* vc;
+  const void* coef;
+  float* image;     // split == 1: image += endScale * sum
+  float* partial;   // split > 1: partial[group][pixel] = sum
+};
+
+template <int INTERP>
+using CoefT = typename std::conditional<INTERP == INTERP_CUBIC, float4, float2>::type;
+
+// atan on the whole line for FP32: odd polynomial t*P(t^2) on [0,1]
+// (weighted least-squares fit of atan(t)/t in t^2, max abs error 1.3e-7),
+// with atan(t) = pi/2 - atan(1/t) beyond 1.  Used when the fan is wider than
+// the fitted range.
 __device__ __forceinline__ float atan_f32(float t) {
-  if constexpr (SMALL_T) {  // |t| <= 1 is guaranteed by the geometry
-    const float z = t * t;
-    float p = 0.0028499013278633356f;
-    p = p * z - 0.01606869138777256f;
-    p = p * z + 0.042691655457019806f;
-    p = p * z - 0.07504309713840485f;
-    p = p * z + 0.10640943795442581f;
-    p = p * z - 0.14203648269176483f;
-    p = p * z + 0.1999261975288391f;
-    p = p * z - 0.3333307206630707f;
-    p = p * z + 1.0f;
-    return t * p;
-  }
   const float at = fabsf(t);
   const bool inv = at > 1.f;
   const float u = inv ? __builtin_amdgcn_rcpf(at) : at;
@@ -414,143 +434,188 @@ __device__ __forceinline__ float atan_f32(float t) {
   p = p * z + 0.1999261975288391f;
   p = p * z - 0.3333307206630707f;
   p = p * z + 1.0f;
-  float a = u * p;
-  a = inv ? 1.5707963267948966f - a : a;
-  return copysignf(a, t);
+  float r = u * p;
+  r = inv ? 1.5707963267948966f - r : r;
+  return copysignf(r, t);
 }
 
+// Detector offset (before the centre) of the equiangular geometry:
+// atan(t) / detInc with t = tan(gamma).
+template <int DEG>
+__device__ __forceinline__ float fan_angle(const float* c, float t) {
+  if constexpr (DEG > 0) {
+    const float z = t * t;
+    float p = c[DEG];
+#pragma unroll
+    for (int k = DEG - 1; k >= 0; k--) p = fmaf(p, z, c[k]);
+    return t * p;
+  } else {
+    return atan_f32(t) * c[0];
+  }
+}
 
-// Detector position and weight of one pixel in one view (mixed path).
-// ca, sa are cos/detInc, sin/detInc (parallel) or cos, sin (fan).
-template <int GK, bool SMALL_T>
-__device__ __forceinline__ void pos_f32(float center, float F, float posScale, float ca, float sa,
-                                        float x, float y, float& pos, float& wgt) {
+// Position of pixel (x, y) in one view (window-relative when cshift is the
+// centre minus the window start) and its weight.  Parallel: ca, sa carry
+// 1/detInc.  Fan: ca, sa = cos, sin of the view angle.
+template <int GK, int DEG>
+__device__ __forceinline__ void pos_mixed(const MixedArgs& a, float ca, float sa, float cshift,
+                                          float x, float y, float r2mF2, float& pos, float& wgt) {
   if constexpr (GK == 0) {
-    pos = fmaf(y, sa, fmaf(x, ca, center));
+    pos = fmaf(y, sa, fmaf(x, ca, cshift));
     wgt = 1.f;
   } else {
-    const float rc = fmaf(y, sa, x * ca);
-    const float fl = fmaf(x, sa, fmaf(-y, ca, F));  // F + r sin(beta - phi)
-    const float q = __builtin_amdgcn_rcpf(fl);
+    const float rc = fmaf(y, sa, x * ca);             // r cos(beta - phi)
+    const float fl = fmaf(x, sa, fmaf(-y, ca, a.F));  // F + r sin(beta - phi)
     if constexpr (GK == 1) {
-      pos = fmaf(atan_f32<SMALL_T>(rc * q), posScale, center);
-      wgt = __builtin_amdgcn_rcpf(fmaf(fl, fl, rc * rc));
+      // one reciprocal for both 1/fl and 1/L^2, L^2 = fl^2 + rc^2 = 2 F fl + r^2 - F^2
+      const float L2 = fmaf(a.twoF, fl, r2mF2);
+      const float rr = __builtin_amdgcn_rcpf(fl * L2);
+      pos = fan_angle<DEG>(a.atan, rc * (L2 * rr)) + cshift;
+      wgt = fl * rr;
     } else {
-      pos = fmaf(rc * q, posScale, center);
-      const float u = F * q;
+      const float q = __builtin_amdgcn_rcpf(fl);
+      pos = fmaf(rc * q, a.posScale, cshift);
+      const float u = a.F * q;
       wgt = u * u;
     }
   }
 }
 
-// Cubic through y0..y3 at nodes -1, 0, 1, 2, evaluated at t in [0, 1), in
-// Horner form (the same polynomial as the reference's Lagrange form).
-__device__ __forceinline__ float cubic_horner(float y0, float y1, float y2, float y3, float t) {
-  const float a2 = fmaf(0.5f, y0 + y2, -y1);
-  const float a3 = fmaf(1.f / 6.f, y3 - y0, 0.5f * (y1 - y2));
-  const float a1 = fmaf(-1.f / 6.f, y3, fmaf(-0.5f, y1, fmaf(-1.f / 3.f, y0, y2)));
-  return fmaf(t, fmaf(t, fmaf(t, a3, a2), a1), y1);
-}
-
-template <int GK, int INTERP, bool LAST_INCL, bool SMALL_T>
-__global__ __launch_bounds__(kBlock) void bp_f32_kernel(BpArgs a) {
-  extern __shared__ float s_g[];
-  __shared__ int s_lo2[kMaxChunk];
-  __shared__ int s_in[kMaxChunk];
-  __shared__ float2 s_cs[kMaxChunk];
+// One workgroup: a tile of (256 / lanesY) columns x (lanesY * P) rows and
+// one group of views.  Lane (cx, ly) owns column ix0 + cx and rows
+// iy0 + P*ly .. + P-1 (contiguous, so the final read-modify-write is a
+// vector access).  Per chunk of views: lanes < nv compute each view's window
+// start from the tile corners, the workgroup stages the window's interval
+// coefficients in LDS, then every lane walks the chunk's views for its P
+// pixels, summing the chunk in FP32 and folding the chunk sum into a
+// Kahan-compensated total.
+template <int GK, int INTERP, int P, int DEG>
+__global__ __launch_bounds__(256) void bp_mixed_kernel(MixedArgs a) {
+  using CT = CoefT<INTERP>;
+  extern __shared__ float4 s_dyn[];
+  CT* s_w = reinterpret_cast<CT*>(s_dyn);
+  __shared__ float4 s_v[kMaxChunk];
+  __shared__ int s_lo[kMaxChunk];
   const int t = threadIdx.x;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int ix0 = (tile / a.tilesY) * a.tileX, iy0 = (tile % a.tilesY) * a.tileY;
-  const int ix = ix0 + t / a.tileY, iy = iy0 + t % a.tileY;
-  const bool active = (t < a.tileX * a.tileY) && ix < a.nx && iy < a.ny;
-  const size_t pix = static_cast<size_t>(ix) * a.ny + iy;
-  const int cx1 = min(ix0 + a.tileX - 1, a.nx - 1), cy1 = min(iy0 + a.tileY - 1, a.ny - 1);
-  const int Wv = a.W > 0 ? a.W : a.nDet + 8;
-  const ViewConstF32* vcs = static_cast<const ViewConstF32*>(a.vc);
-  const float* frows = static_cast<const float*>(a.f);
-  const float x = a.xs[active ? ix : ix0], y = a.ys[active ? iy : iy0];
-  const int nDet = a.nDet;
-  const float center = static_cast<float>(a.iDetCenter);
-  const float F = static_cast<float>(a.F);
-  const float posScale = GK == 1 ? static_cast<float>(1.0 / a.detInc)
-                                 : static_cast<float>(a.sdl / a.detInc);
-  float sum = 0.f, comp = 0.f;
+  const int nTiles = a.tilesX * a.tilesY;
+  int tile, grp;
+  if ((a.split & 7) == 0) {
+    // View groups are dealt to XCDs: XCD k (block ids k, k+8, ...) runs
+    // groups [k*split/8, (k+1)*split/8) over every tile, so it reads only
+    // its own views' coefficients, which stay in its L2.
+    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+    grp = xcd * (a.split >> 3) + local / nTiles;
+    tile = local % nTiles;
+  } else {
+    const int bid = xcd_tile(blockIdx.x, gridDim.x);
+    tile = bid % nTiles;
+    grp = bid / nTiles;
+  }
+  const int tx = 256 / a.lanesY, ty = a.lanesY * P;
+  const int ix0 = (tile / a.tilesY) * tx, iy0 = (tile % a.tilesY) * ty;
+  const int ix = ix0 + t / a.lanesY, iyb = iy0 + (t % a.lanesY) * P;
+  const int cx1 = min(ix0 + tx - 1, a.nx - 1), cy1 = min(iy0 + ty - 1, a.ny - 1);
+  const int W = a.W;
+  const float x = a.xs[min(ix, a.nx - 1)];
+  float y[P], r2mF2[P];
+#pragma unroll
+  for (int k = 0; k < P; k++) {
+    y[k] = a.ys[min(iyb + k, a.ny - 1)];
+    r2mF2[k] = fmaf(x, x, fmaf(y[k], y[k], -a.F * a.F));
+  }
+  const int vBegin = grp * a.viewsPerGroup;
+  const int vEnd = min(a.nView, vBegin + a.viewsPerGroup);
+  float sum[P], comp[P];
+#pragma unroll
+  for (int k = 0; k < P; k++) sum[k] = comp[k] = 0.f;
+  const CT* coef = static_cast<const CT*>(a.coef);
 
-  for (int v0 = 0; v0 < a.nView; v0 += a.VC) {
-    const int nv = min(a.VC, a.nView - v0);
+  for (int v0 = vBegin; v0 < vEnd; v0 += a.chunk) {
+    const int nv = min(a.chunk, vEnd - v0);
     if (t < nv) {
-      const ViewConstF32 c = vcs[v0 + t];
-      int lo = -4, interior = 0;
-      if (a.W > 0) {
+      const ViewConstF32 c = a.vc[v0 + t];
+      int lo = -2;
+      if (a.W < a.nDet + 4) {
         float p0, p1, p2, p3, wd;
-        pos_f32<GK, SMALL_T>(center, F, posScale, c.ca, c.sa, a.xs[ix0], a.ys[iy0], p0, wd);
-        pos_f32<GK, SMALL_T>(center, F, posScale, c.ca, c.sa, a.xs[cx1], a.ys[iy0], p1, wd);
-        pos_f32<GK, SMALL_T>(center, F, posScale, c.ca, c.sa, a.xs[ix0], a.ys[cy1], p2, wd);
-        pos_f32<GK, SMALL_T>(center, F, posScale, c.ca, c.sa, a.xs[cx1], a.ys[cy1], p3, wd);
+        const float xa = a.xs[ix0], xb = a.xs[cx1], ya = a.ys[iy0], yb = a.ys[cy1];
+        pos_mixed<GK, DEG>(a, c.ca, c.sa, a.center, xa, ya, fmaf(xa, xa, fmaf(ya, ya, -a.F * a.F)), p0, wd);
+        pos_mixed<GK, DEG>(a, c.ca, c.sa, a.center, xb, ya, fmaf(xb, xb, fmaf(ya, ya, -a.F * a.F)), p1, wd);
+        pos_mixed<GK, DEG>(a, c.ca, c.sa, a.center, xa, yb, fmaf(xa, xa, fmaf(yb, yb, -a.F * a.F)), p2, wd);
+        pos_mixed<GK, DEG>(a, c.ca, c.sa, a.center, xb, yb, fmaf(xb, xb, fmaf(yb, yb, -a.F * a.F)), p3, wd);
         float pmin = fminf(fminf(p0, p1), fminf(p2, p3));
-        float pmax = fmaxf(fmaxf(p0, p1), fmaxf(p2, p3));
-        if (!(pmin > -1e9f && pmin < 1e9f)) pmin = -4.f;
-        lo = static_cast<int>(floorf(pmin)) - 4;
-        // every tap of every pixel is an interior Lagrange tap
-        interior = (pmin >= 2.f && pmax <= static_cast<float>(nDet - 4) &&
-                    pmax - pmin <= static_cast<float>(Wv - 10)) ? 1 : 0;
+        if (!(pmin > -1e9f && pmin < 1e9f)) pmin = 0.f;
+        lo = static_cast<int>(floorf(pmin)) - 2;
       }
-      s_lo2[t] = lo;
-      s_in[t] = interior;
-      s_cs[t] = make_float2(c.ca, c.sa);
+      s_lo[t] = lo;
+      s_v[t] = make_float4(c.ca, c.sa, a.center - static_cast<float>(lo), 0.f);
     }
     __syncthreads();
     for (int vv = 0; vv < nv; vv++) {
-      const int lo = s_lo2[vv];
-      const float* frow = frows + static_cast<size_t>(v0 + vv) * a.fstride;
-      float* w = s_g + vv * Wv;
-      for (int j = t; j < Wv; j += kBlock) {
+      const int lo = s_lo[vv];
+      const CT* crow = coef + static_cast<size_t>(v0 + vv) * a.nDet;
+      for (int j = t; j < W; j += 256) {
         const int d = lo + j;
-        w[j] = (d >= 0 && d < nDet) ? frow[d] : 0.f;
+        CT val;
+        if (d >= 0 && d < a.nDet) val = crow[d];
+        else if constexpr (INTERP == INTERP_CUBIC) val = make_float4(0.f, 0.f, 0.f, 0.f);
+        else val = make_float2(0.f, 0.f);
+        s_w[vv * W + j] = val;
       }
     }
     __syncthreads();
-    if (active) {
-      for (int vv = 0; vv < nv; vv++) {
-        const float2 cs = s_cs[vv];
-        const float* w = s_g + vv * Wv;
-        const int lo = __builtin_amdgcn_readfirstlane(s_lo2[vv]);
+    float part[P];
+#pragma unroll
+    for (int k = 0; k < P; k++) part[k] = 0.f;
+    for (int vv = 0; vv < nv; vv++) {
+      const float4 c = s_v[vv];
+      const CT* w = s_w + vv * W;
+#pragma unroll
+      for (int k = 0; k < P; k++) {
         float pos, wgt;
-        pos_f32<GK, SMALL_T>(center, F, posScale, cs.x, cs.y, x, y, pos, wgt);
-        float cval = 0.f;
-        if constexpr (INTERP == INTERP_NEAREST) {
-          const int i = static_cast<int>(floorf(pos + 0.5f));
-          if (i >= 0 && i < nDet) cval = wget(w, lo, Wv, i);
-        } else if constexpr (INTERP == INTERP_LINEAR) {
-          const float pf = floorf(pos);
-          const int i = static_cast<int>(pf);
-          const float fr = pos - pf;
-          const int hiOk = LAST_INCL ? nDet - 1 : nDet - 2;
-          if (i >= 0 && i <= hiOk) {
-            const float f0 = wget(w, lo, Wv, i);
-            const float f1 = (i < nDet - 1) ? wget(w, lo, Wv, i + 1) : f0;
-            cval = fmaf(fr, f1 - f0, f0);
-          }
-        } else {
-          if (__builtin_amdgcn_readfirstlane(s_in[vv])) {
-            const float pf = floorf(pos);
-            int k = static_cast<int>(pf) - 1 - lo;
-            k = min(max(k, 0), Wv - 4);
-            cval = cubic_horner(w[k], w[k + 1], w[k + 2], w[k + 3], pos - pf);
-          } else {
-            cval = cubic_f32(w, lo, Wv, nDet, pos);
-          }
-        }
-        // Kahan-compensated accumulation across views
-        const float yk = cval * wgt - comp;
-        const float tk = sum + yk;
-        comp = (tk - sum) - yk;
-        sum = tk;
+        pos_mixed<GK, DEG>(a, c.x, c.y, c.z, x, y[k], r2mF2[k], pos, wgt);
+        const float pf = floorf(pos);
+        const float u = pos - pf;
+        const int j = min(max(static_cast<int>(pf), 0), W - 1);
+        const CT q = w[j];
+        float val;
+        if constexpr (INTERP == INTERP_CUBIC) val = fmaf(u, fmaf(u, fmaf(u, q.w, q.z), q.y), q.x);
+        else val = fmaf(u, q.y, q.x);
+        part[k] = fmaf(val, wgt, part[k]);
       }
+    }
+#pragma unroll
+    for (int k = 0; k < P; k++) {  // Kahan step with the chunk's sum
+      const float yk = part[k] - comp[k];
+      const float tk = sum[k] + yk;
+      comp[k] = (tk - sum[k]) - yk;
+      sum[k] = tk;
     }
     __syncthreads();
   }
-  if (active) a.image[pix] = a.image[pix] + a.endScale * (sum - comp);
+  if (ix >= a.nx) return;
+  const size_t base = static_cast<size_t>(ix) * a.ny + iyb;
+  if (a.split == 1) {
+#pragma unroll
+    for (int k = 0; k < P; k++)
+      if (iyb + k < a.ny) a.image[base + k] = fmaf(a.endScale, sum[k] - comp[k], a.image[base + k]);
+  } else {
+    float* out = a.partial + static_cast<size_t>(grp) * a.nx * a.ny;
+#pragma unroll
+    for (int k = 0; k < P; k++)
+      if (iyb + k < a.ny) out[base + k] = sum[k] - comp[k];
+  }
+}
+
+// image += endScale * (sum of the view groups' partial images), groups added
+// in order in FP64.
+__global__ void bp_combine_kernel(float* __restrict__ image, const float* __restrict__ partial,
+                                  size_t n, int split, float endScale) {
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    double s = 0;
+    for (int g = 0; g < split; g++) s += partial[static_cast<size_t>(g) * n + i];
+    image[i] = static_cast<float>(static_cast<double>(image[i]) + endScale * s);
+  }
 }
 
 #pragma clang fp contract(off)
@@ -596,28 +661,24 @@ void launch_diff(int interp, const BpArgs& a, int grid, hipStream_t s) {
   }
 }
 
-template <int GK, int INTERP>
-void launch_f32(bool lastIncl, bool smallT, const BpArgs& a, int grid, size_t lds, hipStream_t s) {
-  if (GK == 1 && smallT) {
-    if (lastIncl)
-      hipLaunchKernelGGL((bp_f32_kernel<GK, INTERP, true, true>), dim3(grid), dim3(kBlock), lds, s, a);
-    else
-      hipLaunchKernelGGL((bp_f32_kernel<GK, INTERP, false, true>), dim3(grid), dim3(kBlock), lds, s, a);
-  } else {
-    if (lastIncl)
-      hipLaunchKernelGGL((bp_f32_kernel<GK, INTERP, true, false>), dim3(grid), dim3(kBlock), lds, s, a);
-    else
-      hipLaunchKernelGGL((bp_f32_kernel<GK, INTERP, false, false>), dim3(grid), dim3(kBlock), lds, s, a);
+template <int GK, int INTERP, int DEG>
+void launch_mixed_p(int P, const MixedArgs& a, int grid, size_t lds, hipStream_t s) {
+  switch (P) {
+    case 1: hipLaunchKernelGGL((bp_mixed_kernel<GK, INTERP, 1, DEG>), dim3(grid), dim3(256), lds, s, a); break;
+    case 2: hipLaunchKernelGGL((bp_mixed_kernel<GK, INTERP, 2, DEG>), dim3(grid), dim3(256), lds, s, a); break;
+    default: hipLaunchKernelGGL((bp_mixed_kernel<GK, INTERP, 4, DEG>), dim3(grid), dim3(256), lds, s, a); break;
   }
 }
 
-template <int GK>
-void launch_f32_interp(int interp, bool lastIncl, bool smallT, const BpArgs& a, int grid,
-                       size_t lds, hipStream_t s) {
-  switch (interp) {
-    case INTERP_NEAREST: launch_f32<GK, INTERP_NEAREST>(lastIncl, smallT, a, grid, lds, s); break;
-    case INTERP_LINEAR: launch_f32<GK, INTERP_LINEAR>(lastIncl, smallT, a, grid, lds, s); break;
-    default: launch_f32<GK, INTERP_CUBIC>(lastIncl, smallT, a, grid, lds, s); break;
+template <int INTERP>
+void launch_mixed(int gk, int deg, int P, const MixedArgs& a, int grid, size_t lds, hipStream_t s) {
+  if (gk == 0) return launch_mixed_p<0, INTERP, 0>(P, a, grid, lds, s);
+  if (gk == 2) return launch_mixed_p<2, INTERP, 0>(P, a, grid, lds, s);
+  switch (deg) {
+    case 4: launch_mixed_p<1, INTERP, 4>(P, a, grid, lds, s); break;
+    case 6: launch_mixed_p<1, INTERP, 6>(P, a, grid, lds, s); break;
+    case 8: launch_mixed_p<1, INTERP, 8>(P, a, grid, lds, s); break;
+    default: launch_mixed_p<1, INTERP, 0>(P, a, grid, lds, s); break;
   }
 }
 
@@ -687,7 +748,12 @@ int bp_launch(ctsim_ctx* ctx, const BpPlan& p, int precision, const void* d_filt
   // interpolation (a rounding tie picks a different cell) and Diff (its
   // column walk decides which pixel a sample lands in).  The mixed variant
   // reads float filtered views.
-  const bool f64 = precision == CTSIM_PREC_F64 || p.kind == KIND_DIFF || p.interp == INTERP_NEAREST;
+  // A mixed launch whose full-row staging cannot fit one view in LDS (very
+  // long rows with the source close to the image) also takes the index path.
+  const size_t mixedView = static_cast<size_t>(p.mixed.W > 0 ? p.mixed.W : p.nDet + 4) *
+                           (p.interp == INTERP_CUBIC ? sizeof(float4) : sizeof(float2));
+  const bool f64 = precision == CTSIM_PREC_F64 || p.kind == KIND_DIFF ||
+                   p.interp == INTERP_NEAREST || mixedView > kMixedLdsMax;
   const bool inF32 = precision == CTSIM_PREC_MIXED;
   if (f64) {
     CTSIM_CHECK(view_consts<ViewConstF64>(
@@ -709,8 +775,10 @@ int bp_launch(ctsim_ctx* ctx, const BpPlan& p, int precision, const void* d_filt
     a.VC = vc;
     const size_t lds = static_cast<size_t>(vc) * Wv * sizeof(double);
     if (lds > 64 * 1024) return fail(CTSIM_ERR_UNSUPPORTED, "backprojector: detector row too long for LDS");
+    ctx->prof_begin();
     if (inF32) launch_f64_interp<float>(p.interp, p.kind, a, grid, lds, s);
     else launch_f64_interp<double>(p.interp, p.kind, a, grid, lds, s);
+    ctx->prof_end();
     CTSIM_HIP(hipGetLastError());
     return CTSIM_OK;
   }
@@ -718,22 +786,142 @@ int bp_launch(ctsim_ctx* ctx, const BpPlan& p, int precision, const void* d_filt
   CTSIM_CHECK(view_consts<ViewConstF32>(
       ctx, p, 1, h_angles, nView,
       [&](ViewConstF32* o) { bp_view_consts_f32(p, h_angles, nView, o); }, &a.vc));
-  a.W = p.windowF32;
-  if (p.kind == KIND_TRIG) a.endScale = static_cast<float>(p.rotScale);
-  const int Wv = a.W > 0 ? a.W : p.nDet + 8;
-  const size_t budget = 32 * 1024;
-  int vc = static_cast<int>(budget / (Wv * sizeof(float)));
-  if (vc < 1) vc = 1;
-  if (vc > kMaxChunk) vc = kMaxChunk;
-  a.VC = vc;
-  const size_t lds = static_cast<size_t>(vc) * Wv * sizeof(float);
-  if (lds > 64 * 1024) return fail(CTSIM_ERR_UNSUPPORTED, "backprojector: detector row too long for LDS");
+  return bp_launch_mixed(ctx, p, p.mixed, static_cast<const float*>(d_filtered),
+                         static_cast<const ViewConstF32*>(a.vc), nView, d_image);
+}
+
+// The mixed path: interval coefficients of every view, then the tiled
+// kernel, then (with view groups) the ordered combine.
+int bp_launch_mixed(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const float* d_filtered,
+                    const ViewConstF32* d_vc, int nView, float* d_image) {
+  const hipStream_t s = ctx->stream;
+  const bool cubic = p.interp == INTERP_CUBIC;
   const bool lastIncl = p.kind == KIND_IDIFF;
+  const size_t elem = cubic ? sizeof(float4) : sizeof(float2);
+  CTSIM_CHECK(ctx->scratch2.reserve(static_cast<size_t>(nView) * p.nDet * elem));
+  const int perRow = (p.nDet + 255) / 256;
+  const int cgrid = nView * perRow;
+  const int fstride = p.nDet * p.interpFactor;
+  if (cubic)
+    hipLaunchKernelGGL((coef_kernel<INTERP_CUBIC, false>), dim3(cgrid), dim3(256), 0, s, d_filtered,
+                       fstride, p.nDet, nView, perRow, ctx->scratch2.ptr);
+  else if (lastIncl)
+    hipLaunchKernelGGL((coef_kernel<INTERP_LINEAR, true>), dim3(cgrid), dim3(256), 0, s, d_filtered,
+                       fstride, p.nDet, nView, perRow, ctx->scratch2.ptr);
+  else
+    hipLaunchKernelGGL((coef_kernel<INTERP_LINEAR, false>), dim3(cgrid), dim3(256), 0, s, d_filtered,
+                       fstride, p.nDet, nView, perRow, ctx->scratch2.ptr);
+
+  MixedArgs a{};
+  a.nx = p.nx; a.ny = p.ny; a.nDet = p.nDet; a.nView = nView;
+  a.lanesY = m.lanesY;
+  const int P = m.P, tx = 256 / m.lanesY, ty = m.lanesY * P;
+  a.tilesX = (p.nx + tx - 1) / tx;
+  a.tilesY = (p.ny + ty - 1) / ty;
+  a.W = m.W > 0 ? m.W : p.nDet + 4;
+  a.split = m.split < 1 ? 1 : (m.split > nView ? nView : m.split);
+  a.viewsPerGroup = (nView + a.split - 1) / a.split;
+  a.split = (nView + a.viewsPerGroup - 1) / a.viewsPerGroup;
+  a.chunk = m.chunk < 1 ? 1 : (m.chunk > kMaxChunk ? kMaxChunk : m.chunk);
+  a.center = static_cast<float>(p.iDetCenter);
+  a.F = static_cast<float>(p.F);
+  a.twoF = static_cast<float>(2 * p.F);
+  a.endScale = p.kind == KIND_TRIG ? static_cast<float>(p.rotScale) : 1.f;
   const int gk = p.geom == GEOM_PARALLEL ? 0 : (p.geom == GEOM_EQUIANGULAR ? 1 : 2);
-  if (gk == 0) launch_f32_interp<0>(p.interp, lastIncl, p.smallTan, a, grid, lds, s);
-  else if (gk == 1) launch_f32_interp<1>(p.interp, lastIncl, p.smallTan, a, grid, lds, s);
-  else launch_f32_interp<2>(p.interp, lastIncl, p.smallTan, a, grid, lds, s);
+  a.posScale = gk == 2 ? static_cast<float>(p.sdl / p.detInc) : static_cast<float>(1.0 / p.detInc);
+  const int deg = gk == 1 ? p.atanDeg : 0;
+  if (deg > 0) for (int k = 0; k <= deg; k++) a.atan[k] = p.atanCoef[k];
+  else a.atan[0] = static_cast<float>(1.0 / p.detInc);
+  a.xs = p.d_xy.as<float>();
+  a.ys = p.d_xy.as<float>() + p.xs.size();
+  a.vc = d_vc;
+  a.coef = ctx->scratch2.ptr;
+  a.image = d_image;
+  const size_t lds = static_cast<size_t>(a.chunk) * a.W * elem;
+  if (lds > kMixedLdsMax) return fail(CTSIM_ERR_UNSUPPORTED, "backprojector: detector row too long for LDS");
+  const size_t npix = static_cast<size_t>(p.nx) * p.ny;
+  if (a.split > 1) {
+    CTSIM_CHECK(ctx->scratch.reserve(static_cast<size_t>(a.split) * npix * sizeof(float)));
+    a.partial = ctx->scratch.as<float>();
+  }
+  const int grid = a.tilesX * a.tilesY * a.split;
+  ctx->prof_begin();
+  if (cubic) launch_mixed<INTERP_CUBIC>(gk, deg, P, a, grid, lds, s);
+  else launch_mixed<INTERP_LINEAR>(gk, deg, P, a, grid, lds, s);
+  ctx->prof_end();
+  if (a.split > 1) {
+    int g = static_cast<int>((npix + 255) / 256);
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(bp_combine_kernel, dim3(g), dim3(256), 0, s, d_image, a.partial, npix, a.split,
+                       a.endScale);
+  }
   CTSIM_HIP(hipGetLastError());
+  return CTSIM_OK;
+}
+
+// Times candidate launch shapes of the mixed kernel on the given views (into
+// a scratch image) and keeps the fastest in the plan.
+int bp_autotune(ctsim_ctx* ctx, BpPlan& p, const float* d_filtered, const double* h_angles,
+                int nView, double* bestMs) {
+  if (nView <= 0) return fail(CTSIM_ERR_INVALID, "autotune: no views");
+  const bool f64 = p.kind == KIND_DIFF || p.interp == INTERP_NEAREST;
+  if (f64) {  // these run on the FP64 index path; nothing to tune
+    if (bestMs) *bestMs = 0;
+    return CTSIM_OK;
+  }
+  const void* vc = nullptr;
+  CTSIM_CHECK(view_consts<ViewConstF32>(
+      ctx, p, 1, h_angles, nView,
+      [&](ViewConstF32* o) { bp_view_consts_f32(p, h_angles, nView, o); }, &vc));
+  DevBuf img;
+  CTSIM_CHECK(img.reserve(static_cast<size_t>(p.nx) * p.ny * sizeof(float)));
+  hipEvent_t e0, e1;
+  CTSIM_HIP(hipEventCreate(&e0));
+  CTSIM_HIP(hipEventCreate(&e1));
+  const int elem = p.interp == INTERP_CUBIC ? 16 : 8;
+  MixedShape best = p.mixed;
+  float bestT = 1e30f;
+  int rc = CTSIM_OK;
+  for (int lanesY : {8, 16, 32}) {
+    for (int P : {1, 2, 4}) {
+      if (lanesY * P > 2 * p.ny && P > 1) continue;
+      const int tx = 256 / lanesY, ty = lanesY * P;
+      const int W = bp_window_bound(p, tx, ty);
+      const int Wv = W > 0 ? W : p.nDet + 4;
+      for (int split : {1, 2, 4, 8, 16}) {
+        if (split > 1 && nView / split < 8) continue;
+        for (int chunk : {4, 8, 16, 32}) {
+          if (static_cast<size_t>(chunk) * Wv * elem > kMixedLdsMax) continue;
+          MixedShape m;
+          m.lanesY = lanesY; m.P = P; m.split = split; m.chunk = chunk; m.W = W;
+          float ms = 0, tot = 0;
+          for (int rep = 0; rep < 3 && rc == CTSIM_OK; rep++) {
+            if ((rc = bp_launch_mixed(ctx, p, m, d_filtered, static_cast<const ViewConstF32*>(vc),
+                                      nView, img.as<float>())) != CTSIM_OK) break;
+            if (rep == 0) continue;  // warm-up
+            (void)hipEventRecord(e0, ctx->stream);
+            if ((rc = bp_launch_mixed(ctx, p, m, d_filtered, static_cast<const ViewConstF32*>(vc),
+                                      nView, img.as<float>())) != CTSIM_OK) break;
+            (void)hipEventRecord(e1, ctx->stream);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            tot += ms;
+          }
+          if (rc != CTSIM_OK) break;
+          if (tot / 2 < bestT) { bestT = tot / 2; best = m; }
+        }
+        if (rc != CTSIM_OK) break;
+      }
+      if (rc != CTSIM_OK) break;
+    }
+    if (rc != CTSIM_OK) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != CTSIM_OK) return rc;
+  p.mixed = best;
+  p.mixedTuned = true;
+  if (bestMs) *bestMs = bestT;
   return CTSIM_OK;
 }
 
@@ -792,6 +980,35 @@ int ctsim_backproject_postprocess(ctsim_ctx* ctx, const ctsim_backprojector* bp,
     return fail(CTSIM_ERR_INVALID, "ctsim_backproject_postprocess: NULL argument");
   CTSIM_CHECK(ctx->activate());
   return bp_launch_post(ctx, bp->plan, d_image);
+}
+
+int ctsim_backprojector_autotune(ctsim_ctx* ctx, ctsim_backprojector* bp, const float* d_filtered,
+                                 const double* h_angles, int nView, int* shape5, double* bestMs) {
+  if (!ctx || !bp || !d_filtered || !h_angles)
+    return fail(CTSIM_ERR_INVALID, "ctsim_backprojector_autotune: NULL argument");
+  CTSIM_CHECK(ctx->activate());
+  CTSIM_CHECK(bp_autotune(ctx, bp->plan, d_filtered, h_angles, nView, bestMs));
+  if (shape5) {
+    const MixedShape& m = bp->plan.mixed;
+    shape5[0] = m.lanesY; shape5[1] = m.P; shape5[2] = m.split; shape5[3] = m.chunk; shape5[4] = m.W;
+  }
+  return CTSIM_OK;
+}
+
+int ctsim_backprojector_set_shape(ctsim_backprojector* bp, const int* shape4) {
+  if (!bp || !shape4) return fail(CTSIM_ERR_INVALID, "ctsim_backprojector_set_shape: NULL argument");
+  const int lanesY = shape4[0], P = shape4[1], split = shape4[2], chunk = shape4[3];
+  if (!(lanesY == 8 || lanesY == 16 || lanesY == 32 || lanesY == 64) || !(P == 1 || P == 2 || P == 4) ||
+      split < 1 || split > 64 || chunk < 1 || chunk > kMaxChunk)
+    return fail(CTSIM_ERR_INVALID, "ctsim_backprojector_set_shape: shape out of range");
+  MixedShape m;
+  m.lanesY = lanesY; m.P = P; m.split = split; m.chunk = chunk;
+  m.W = bp_window_bound(bp->plan, 256 / lanesY, lanesY * P);
+  const int Wv = m.W > 0 ? m.W : bp->plan.nDet + 4;
+  if (static_cast<size_t>(chunk) * Wv * (bp->plan.interp == INTERP_CUBIC ? 16 : 8) > kMixedLdsMax)
+    return fail(CTSIM_ERR_INVALID, "ctsim_backprojector_set_shape: staging exceeds 64 KB of LDS");
+  bp->plan.mixed = m;
+  return CTSIM_OK;
 }
 
 int ctsim_backprojector_info(const ctsim_backprojector* bp, double* o) {

@@ -3,6 +3,7 @@
 #include <strings.h>
 
 #include <cmath>
+#include <utility>
 #include <cstring>
 
 #include "bp_plan.h"
@@ -22,6 +23,60 @@ int lookup(const char* name, const char* const* table, int n) {
 
 // nearest<long>, include/ctsupport.h:192-194
 long nearest_long(double x) { return x > 0 ? static_cast<long>(x + 0.5) : static_cast<long>(x - 0.5); }
+
+// g(z) = atan(sqrt z) / sqrt z, so that atan(t) = t g(t^2).
+double atan_ratio(double z) {
+  if (z < 1e-12) return 1.0 - z / 3.0;
+  const double s = sqrt(z);
+  return atan(s) / s;
+}
+
+// Polynomial of degree D in z interpolating g at the D+1 Chebyshev nodes of
+// [0, zmax] (near-minimax), by Gaussian elimination on the Vandermonde system.
+void cheb_fit(double zmax, int D, double* c) {
+  double A[9][10];
+  for (int j = 0; j <= D; j++) {
+    const double z = 0.5 * zmax * (1 - cos(kPi * (j + 0.5) / (D + 1)));
+    double zp = 1;
+    for (int k = 0; k <= D; k++, zp *= z) A[j][k] = zp;
+    A[j][D + 1] = atan_ratio(z);
+  }
+  for (int col = 0; col <= D; col++) {
+    int piv = col;
+    for (int r = col + 1; r <= D; r++)
+      if (fabs(A[r][col]) > fabs(A[piv][col])) piv = r;
+    for (int k = 0; k <= D + 1; k++) std::swap(A[col][k], A[piv][k]);
+    for (int r = 0; r <= D; r++) {
+      if (r == col) continue;
+      const double f = A[r][col] / A[col][col];
+      for (int k = col; k <= D + 1; k++) A[r][k] -= f * A[col][k];
+    }
+  }
+  for (int k = 0; k <= D; k++) c[k] = A[k][D + 1] / A[k][k];
+}
+
+// Lowest degree whose relative error on [0, zmax] is below 3e-8 (a position
+// error under one FP32 ulp of a detector index); coefficients carry the
+// 1/detInc scale so the kernel forms the detector offset as t * P(t^2).
+void fit_atan(BpPlan& p, double zmax) {
+  for (int D = 4; D <= 8; D += 2) {
+    double c[9];
+    cheb_fit(zmax, D, c);
+    double err = 0;
+    for (int i = 0; i <= 4000; i++) {
+      const double z = zmax * i / 4000.0;
+      double v = c[D];
+      for (int k = D - 1; k >= 0; k--) v = v * z + c[k];
+      err = fmax(err, fabs(v / atan_ratio(z) - 1));
+    }
+    if (err <= 3e-8) {
+      p.atanDeg = D;
+      for (int k = 0; k <= D; k++) p.atanCoef[k] = static_cast<float>(c[k] / p.detInc);
+      return;
+    }
+  }
+  p.atanDeg = 0;
+}
 }  // namespace
 
 // Backprojector::initBackprojector + Backproject ctor + BackprojectTable and
@@ -114,7 +169,13 @@ int bp_plan_init(BpPlan& p, int geometry, int nDet, double detInc, int nView,
     const double rx = fmax(fabs(p.xMin), fabs(p.xMax)), ry = fmax(fabs(p.yMin), fabs(p.yMax));
     const double R = sqrt(rx * rx + ry * ry);
     p.smallTan = p.F > 0 && R * 1.4143 * 1.001 < p.F;
+    if (p.geom == GEOM_EQUIANGULAR && p.F > R) {
+      // every pixel sees the source within |gamma| <= asin(R / F)
+      const double tmax = 1.001 * R / sqrt(p.F * p.F - R * R);
+      if (tmax <= 1.0) fit_atan(p, tmax * tmax);
+    }
   }
+  p.mixed = bp_default_shape(p, 256);
   return CTSIM_OK;
 }
 
@@ -183,6 +244,26 @@ int bp_window_bound(const BpPlan& p, int tileX, int tileY) {
   double w = ceil(span) + 10;
   if (!(w < p.nDet + 8)) return 0;
   return static_cast<int>(w);
+}
+
+// Default mixed launch shape: 16 x 64 pixel tiles (4 rows per lane); eight
+// view groups (one per XCD) when there are enough views, so each XCD's L2
+// holds its views' coefficients; chunks sized for about four workgroups'
+// staging per CU in 160 KB of LDS.
+MixedShape bp_default_shape(const BpPlan& p, int numCUs) {
+  (void)numCUs;
+  MixedShape m;
+  m.lanesY = 16;
+  m.P = p.ny >= 64 ? 4 : (p.ny >= 32 ? 2 : 1);
+  const int tx = 256 / m.lanesY, ty = m.lanesY * m.P;
+  m.split = p.nView >= 8 * 32 ? 8 : 1;
+  m.W = bp_window_bound(p, tx, ty);
+  const int Wv = m.W > 0 ? m.W : p.nDet + 4;
+  const int elem = p.interp == INTERP_CUBIC ? 16 : 8;
+  int chunk = (36 * 1024) / (Wv * elem);
+  chunk = chunk > 32 ? 32 : (chunk < 1 ? 1 : chunk);
+  m.chunk = chunk;
+  return m;
 }
 
 }  // namespace ctsim

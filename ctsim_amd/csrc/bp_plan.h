@@ -30,6 +30,14 @@ struct ViewConstF32 {
   float pad0, pad1;
 };
 
+// Launch shape of the mixed kernel: a workgroup of 256 lanes covers
+// (256 / lanesY) columns x (lanesY * P) rows; views are split into `split`
+// groups (separate workgroups whose partial sums are combined in a fixed
+// order) and staged `chunk` views at a time.
+struct MixedShape {
+  int lanesY = 16, P = 4, split = 1, chunk = 16, W = 0;
+};
+
 struct BpPlan {
   int geom = 0, family = 0, kind = 0, interp = 0, interpFactor = 1;
   int nDet = 0, nView = 0, nx = 0, ny = 0, iDetCenter = 0;
@@ -45,6 +53,12 @@ struct BpPlan {
   int tileX = 16, tileY = 16;       // pixels per block along ix, iy
   int windowF64 = 0, windowF32 = 0; // detector window per (tile, view); 0 = full row
   bool smallTan = false;            // fan beam: |tan gamma| <= 1 for every pixel
+  // Mixed equiangular path: atan(t) * posScale = t * P(t^2), P of degree
+  // atanDeg (0: range-reduced generic polynomial) fitted on [0, tan^2 gmax].
+  int atanDeg = 0;
+  float atanCoef[9] = {};
+  MixedShape mixed;                 // current launch shape (tuned or default)
+  bool mixedTuned = false;
   // Per-view constants of the last angle set, kept on the device so repeated
   // reconstructions with the same views upload nothing.
   mutable std::vector<double> cachedAngles;
@@ -59,12 +73,15 @@ int bp_plan_init(BpPlan& p, int geometry, int nDet, double detInc, int nView,
 void bp_view_consts_f64(const BpPlan& p, const double* angles, int nView, ViewConstF64* out);
 void bp_view_consts_f32(const BpPlan& p, const double* angles, int nView, ViewConstF32* out);
 int bp_window_bound(const BpPlan& p, int tileX, int tileY);
+MixedShape bp_default_shape(const BpPlan& p, int numCUs);
 
 // Device side (backproject.hip).
 int bp_upload_tables(BpPlan& p);
 int bp_launch(ctsim_ctx* ctx, const BpPlan& p, int precision, const void* d_filtered,
               const double* h_angles, int nView, float* d_image);
 int bp_launch_post(ctsim_ctx* ctx, const BpPlan& p, float* d_image);
+int bp_launch_mixed(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const float* d_filtered,
+                    const ViewConstF32* d_vc, int nView, float* d_image);
 
 }  // namespace ctsim
 

@@ -25,6 +25,8 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
 
 int DevBuf::reserve(size_t n) {
   if (n <= bytes && ptr) return CTSIM_OK;
+  // kernels still queued may read the old allocation
+  if (ptr) CTSIM_HIP(hipDeviceSynchronize());
   release();
   size_t want = n < 256 ? 256 : n;
   CTSIM_HIP(hipMalloc(&ptr, want));
@@ -78,6 +80,27 @@ int ctsim_ctx::activate() const {
   return CTSIM_OK;
 }
 
+void ctsim_ctx::prof_begin() {
+  if (!profOn) return;
+  if (profEvents.size() < 2 * (profUsed + 1)) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    profEvents.push_back(a);
+    profEvents.push_back(b);
+  }
+  (void)hipEventRecord(profEvents[2 * profUsed], stream);
+}
+
+void ctsim_ctx::prof_end() {
+  if (!profOn || profEvents.size() < 2 * (profUsed + 1)) return;
+  (void)hipEventRecord(profEvents[2 * profUsed + 1], stream);
+  profUsed++;
+}
+
+ctsim_ctx::~ctsim_ctx() {
+  for (hipEvent_t e : profEvents) (void)hipEventDestroy(e);
+}
+
 using namespace ctsim;
 
 extern "C" {
@@ -120,6 +143,29 @@ int ctsim_ctx_set_stream(ctsim_ctx* ctx, void* stream) {
 }
 
 void* ctsim_ctx_stream(const ctsim_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int ctsim_ctx_profile(ctsim_ctx* ctx, int enable) {
+  if (!ctx) return fail(CTSIM_ERR_INVALID, "ctsim_ctx_profile: ctx is NULL");
+  ctx->profOn = enable != 0;
+  ctx->profUsed = 0;
+  return CTSIM_OK;
+}
+
+int ctsim_ctx_profile_read(ctsim_ctx* ctx, double* totalMs, int* count) {
+  if (!ctx || !totalMs || !count) return fail(CTSIM_ERR_INVALID, "ctsim_ctx_profile_read: NULL argument");
+  CTSIM_CHECK(ctx->activate());
+  double tot = 0;
+  for (size_t k = 0; k < ctx->profUsed; k++) {
+    CTSIM_HIP(hipEventSynchronize(ctx->profEvents[2 * k + 1]));
+    float ms = 0;
+    CTSIM_HIP(hipEventElapsedTime(&ms, ctx->profEvents[2 * k], ctx->profEvents[2 * k + 1]));
+    tot += ms;
+  }
+  *totalMs = tot;
+  *count = static_cast<int>(ctx->profUsed);
+  ctx->profUsed = 0;
+  return CTSIM_OK;
+}
 
 int ctsim_ctx_synchronize(ctsim_ctx* ctx) {
   if (!ctx) return fail(CTSIM_ERR_INVALID, "ctsim_ctx_synchronize: ctx is NULL");

@@ -80,4 +80,12 @@ struct ctsim_ctx {
   ctsim::DevBuf scratch2;
   int numCUs = 256;
   int activate() const;       // hipSetDevice(device)
+  // Kernel timing: while enabled, the main backprojection kernel of each
+  // launch is bracketed by a pair of events (see ctsim_ctx_profile).
+  bool profOn = false;
+  std::vector<hipEvent_t> profEvents;  // start, stop pairs
+  size_t profUsed = 0;                 // pairs recorded since the last read
+  void prof_begin();
+  void prof_end();
+  ~ctsim_ctx();
 };

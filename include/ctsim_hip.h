@@ -61,6 +61,13 @@ void ctsim_ctx_destroy(ctsim_ctx* ctx);
 int ctsim_ctx_set_stream(ctsim_ctx* ctx, void* stream);
 void* ctsim_ctx_stream(const ctsim_ctx* ctx);
 int ctsim_ctx_synchronize(ctsim_ctx* ctx);
+/* No counterpart in the reference (measurement).  While enabled, the main
+ * backprojection kernel of every launch on this context is bracketed by HIP
+ * events on the context's stream; ctsim_ctx_profile_read waits for them and
+ * returns the summed kernel time and the number of launches since the last
+ * read (or since enabling). */
+int ctsim_ctx_profile(ctsim_ctx* ctx, int enable);
+int ctsim_ctx_profile_read(ctsim_ctx* ctx, double* totalMs, int* count);
 /* Message of the last failure on this thread (reference: fail()/failMessage(),
  * e.g. libctsim/scanner.cpp:93-106). */
 const char* ctsim_last_error(void);
@@ -104,6 +111,16 @@ int ctsim_backproject_postprocess(ctsim_ctx* ctx, const ctsim_backprojector* bp,
 /* Geometry the backprojector derived: xMin, xMax, yMin, yMax, xInc, yInc,
  * rotScale, iDetCenter (8 doubles). */
 int ctsim_backprojector_info(const ctsim_backprojector* bp, double* out8);
+/* No counterpart in the reference (launch tuning of the MIXED kernel).
+ * Times candidate shapes (lanes along y, rows per lane, view groups, views
+ * per LDS chunk) on nView float rows of d_filtered into a scratch image and
+ * keeps the fastest for later MIXED launches of this plan; shape5 (optional)
+ * receives lanesY, P, split, chunk, window and bestMs (optional) its time. */
+int ctsim_backprojector_autotune(ctsim_ctx* ctx, ctsim_backprojector* bp, const float* d_filtered,
+                                 const double* h_angles, int nView, int* shape5, double* bestMs);
+/* Sets the MIXED launch shape explicitly: lanesY (8, 16, 32, 64), P (1, 2,
+ * 4), split (1..64), chunk (1..64). */
+int ctsim_backprojector_set_shape(ctsim_backprojector* bp, const int* shape4);
 
 /* ------------------------------------------------------------ filtering */
 
@@ -168,6 +185,35 @@ int ctsim_scanner_info(const ctsim_scanner* sc, double* out12);
 int ctsim_collect_projections(ctsim_ctx* ctx, const ctsim_scanner* sc, ctsim_phantom* phm,
                               int precision, int iStartView, int iNumViews, int iOffsetView,
                               int iStorageOffset, float* d_sino, double* h_angles);
+
+/* ---------------------------------------------------------------- helical */
+
+/* Replaces Projections::Helical180LI (projections.cpp:200-358) on the device:
+ * d_in float [nView][nDet] of a fan-beam scan (rotInc, detInc, fan angle as
+ * in the Projections header) -> d_out float [L+1][nDet], L = int((pi + fan) /
+ * rotInc), *nViewOut = L+1, h_anglesOut (optional, L+1 doubles) the new view
+ * angles.  interpView < 0 picks the reference's default view.  geometry 1
+ * (equiangular) follows the reference bit for bit; geometry 2 (equilinear,
+ * which the reference rejects at projections.cpp:214-219) uses the same
+ * rules with gamma = atan(t / sourceDetectorLength).  Samples for which the
+ * reference would index past its arrays are dropped. */
+int ctsim_helical180li(ctsim_ctx* ctx, int geometry, int nDet, int nView, double rotInc,
+                       double detInc, double fanBeamAngle, double sourceDetectorLength,
+                       int interpView, const float* d_in, float* d_out, int outCapacity,
+                       int* nViewOut, double* h_anglesOut);
+/* Replaces Projections::HalfScanFeather (projections.cpp:369-439), reading
+ * d_in and writing d_out (separate buffers), both float [nView][nDet]. */
+int ctsim_halfscan_feather(ctsim_ctx* ctx, int geometry, int nDet, int nView, double rotInc,
+                           double detInc, double fanBeamAngle, double sourceDetectorLength,
+                           const float* d_in, float* d_out);
+
+/* ------------------------------------------------------------- statistics */
+
+/* Replaces ImageFile::comparativeStatistics (imagefile.cpp:185-240) on
+ * device images: d_ref is the reference, both float [nx][ny] column-major.
+ * Writes d, r, e to h_dre[3] and synchronises the context's stream. */
+int ctsim_image_stats(ctsim_ctx* ctx, const float* d_ref, const float* d_cmp, int nx, int ny,
+                      double* h_dre);
 
 #ifdef __cplusplus
 }

@@ -360,3 +360,55 @@ def ref_read_if(path: str, cap: int) -> np.ndarray:
         raise RuntimeError(f"ref_read_if failed: {L.ref_last_error().decode()}")
     nx, ny = int(nxny[0]), int(nxny[1])
     return buf[: nx * ny].reshape(nx, ny)
+
+
+def helical180li(geom: int, data: np.ndarray, rotInc: float, detInc: float, fanAngle: float,
+                 sdl: float, interpView: int = -1, feather: bool = False):
+    """Oracle Helical180LI (+ HalfScanFeather): returns (views, angles) or
+    raises with the oracle's message.  geom 1 equiangular, 2 equilinear."""
+    L = lib("oracle")
+    f = L.orc_helical180li
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                  C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    data = np.ascontiguousarray(data, np.float32)
+    nView, nDet = data.shape
+    cap = int((np.pi + fanAngle) / rotInc) + 2
+    out = np.zeros((cap, nDet), np.float32)
+    ang = np.zeros(cap, np.float64)
+    n = f(geom, nDet, nView, rotInc, detInc, fanAngle, sdl, interpView, _ptr(data), _ptr(out),
+          _ptr(ang))
+    if n < 0:
+        raise RuntimeError(f"orc_helical180li: {L.orc_last_error().decode()}")
+    out, ang = out[:n].copy(), ang[:n].copy()
+    if feather:
+        g = L.orc_halfscan_feather
+        g.restype = C.c_int
+        g.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                      C.c_void_p]
+        rc = g(geom, nDet, n, rotInc, detInc, fanAngle, sdl, _ptr(out))
+        if rc:
+            raise RuntimeError(f"orc_halfscan_feather: {L.orc_last_error().decode()}")
+    return out, ang
+
+
+def ref_helical(builtin: str, geometry: str, nDet: int, nView: int, nSample: int, rotLen: float,
+                focalRatio: float, centerRatio: float, data: np.ndarray, interpView: int = -1,
+                feather: bool = False):
+    """The reference's Helical180LI (+ HalfScanFeather) on ``data``; returns
+    (views, angles, (rotInc, detInc, fanAngle, sdl))."""
+    L = lib("ref")
+    f = L.ref_helical
+    c, i, d, vp = C.c_char_p, C.c_int, C.c_double, C.c_void_p
+    f.restype = i
+    f.argtypes = [c, c, i, i, i, d, d, d, vp, i, i, vp, vp, i, vp]
+    data = np.ascontiguousarray(data, np.float32)
+    cap = nView
+    out = np.zeros((cap, nDet), np.float32)
+    ang = np.zeros(cap, np.float64)
+    info = np.zeros(4, np.float64)
+    n = f(_b(builtin), _b(geometry), nDet, nView, nSample, rotLen, focalRatio, centerRatio,
+          _ptr(data), interpView, int(feather), _ptr(out), _ptr(ang), cap, _ptr(info))
+    if n < 0:
+        raise RuntimeError(f"ref_helical failed ({n}): {L.ref_last_error().decode()}")
+    return out[:n].copy(), ang[:n].copy(), tuple(info)

@@ -1627,3 +1627,130 @@ int orc_stats(int nx, int ny, const float* a, const float* b, double* dre) {
   dre[2] = emax;
   return 0;
 }
+
+/* ---------------------------------------------------------------- helical */
+
+/* Fan angle of detector iDet for the helical routines.  Equiangular:
+ * (iDet - (nDet-1)/2) * detInc with the reference's integer centre
+ * (projections.cpp:279, 386).  Equilinear (no reference implementation; the
+ * reference stops at projections.cpp:214-219): the same integer centre on the
+ * flat detector at source distance sdl, gamma = atan(t / sdl). */
+static double helix_gamma(int geom, int iDet, int nDet, double detInc, double sdl) {
+  if (geom == 2) return atan((iDet - (nDet - 1) / 2) * detInc / sdl);
+  return (iDet - (nDet - 1) / 2) * detInc;
+}
+
+/* Projections::Helical180LI_Equiangular, projections.cpp:231-358, as a
+ * scatter in the reference's loop order into a zeroed float array.
+ * in: [nView][nDet] float (views from 0), out: [(L+1)][nDet] with
+ * L = int((pi + fan) / rotInc); anglesOut receives (i + offsetView) * rotInc.
+ * geom: 1 equiangular (the reference), 2 equilinear (derived).  Returns L+1,
+ * or -1 (not 2(pi + fan) of data) / -2 (interpView without pi + fan either side). */
+int orc_helical180li(int geom, int nDet, int nView, double rotInc, double detInc,
+                     double fanAngle, double sdl, int interpView, const float* in, float* out,
+                     double* anglesOut) {
+  const double dbeta = rotInc;
+  int offsetView = 0;
+  if (nView < (int)((2 * (PI + fanAngle)) / dbeta) - 1) {
+    snprintf(g_err, sizeof g_err, "Data set does not include 360 +2*FanBeamAngle views");
+    return -1;
+  }
+  if (interpView < 0) {
+    interpView = (int)((PI + fanAngle) / dbeta);
+  } else {
+    if (interpView * dbeta < PI + fanAngle || interpView * dbeta + PI + fanAngle > nView * dbeta) {
+      snprintf(g_err, sizeof g_err, "no pi+fanAngle of data on either side of the interpolation view");
+      return -2;
+    }
+    offsetView = interpView - (int)((PI + fanAngle) / dbeta);
+  }
+  const int L = (int)((PI + fanAngle) / dbeta);
+  for (int i = 0; i <= L; i++) {
+    anglesOut[i] = (i + offsetView) * dbeta;
+    for (int j = 0; j < nDet; j++) out[(size_t)i * nDet + j] = 0.f;
+  }
+  const int lastAcq = 2 * L;
+  for (int iView = 0; iView <= lastAcq; iView++) {
+    const double beta = iView * dbeta;
+    for (int iDet = 0; iDet < nDet; iDet++) {
+      const double gamma = helix_gamma(geom, iDet, nDet, detInc, sdl);
+      int newiView, newiDet;
+      if (beta < PI + fanAngle) {
+        newiDet = iDet;
+        newiView = iView;
+      } else {
+        newiDet = -iDet + (nDet - 1);
+        newiView = nearest_int(((iView * dbeta) + 2 * gamma - PI) / dbeta);
+      }
+      /* the reference can index one view past its data at the far edge of
+       * the last acquired view (projections.cpp:303); such samples count as 0 */
+      if (iView + offsetView >= nView) continue;
+      if (beta > fanAngle - 2 * gamma && beta < 2 * PI + fanAngle - 2 * gamma) {
+        /* the mirrored view rounds to L + 1 when (pi + fan) / rotInc has a
+         * fractional part >= 1/2; the reference then writes past its array
+         * (projections.cpp:281, 304); such samples are dropped */
+        if (newiView > L) continue;
+        const float src = in[(size_t)(iView + offsetView) * nDet + iDet];
+        float* dst = &out[(size_t)newiView * nDet + newiDet];
+        if (beta > fanAngle - 2 * gamma && beta <= 2 * fanAngle)
+          *dst += (beta + 2 * gamma - fanAngle) / (PI + 2 * gamma) * src;
+        else if (beta > 2 * fanAngle && beta <= PI - 2 * gamma)
+          *dst += (beta + 2 * gamma - fanAngle) / (PI + 2 * gamma) * src;
+        else if (beta > PI - 2 * gamma && beta <= PI + fanAngle)
+          *dst += (beta + 2 * gamma - fanAngle) / (PI + 2 * gamma) * src;
+        else if (beta > PI + fanAngle && beta <= PI + 2 * fanAngle - 2 * gamma)
+          *dst += (2 * PI - beta - 2 * gamma + fanAngle) / (PI - 2 * gamma) * src;
+        else if (beta > PI + 2 * fanAngle - 2 * gamma && beta <= 2 * PI)
+          *dst += (2 * PI - beta - 2 * gamma + fanAngle) / (PI - 2 * gamma) * src;
+        else if (beta > 2 * PI && beta <= 2 * PI + fanAngle - 2 * gamma)
+          *dst += (2 * PI - beta - 2 * gamma + fanAngle) / (PI - 2 * gamma) * src;
+      }
+    }
+  }
+  return L + 1;
+}
+
+/* Projections::HalfScanFeather, projections.cpp:369-439, in place on
+ * data [nView][nDet] (nView must be int((pi + fan) / rotInc) + 1).  Returns
+ * 0, or -1 (not a half-scan data set) / -4 (outside every weighting region). */
+int orc_halfscan_feather(int geom, int nDet, int nView, double rotInc, double detInc,
+                         double fanAngle, double sdl, float* data) {
+  const double dbeta = rotInc;
+  if (nView != (int)((PI + fanAngle) / dbeta) + 1) {
+    snprintf(g_err, sizeof g_err, "not a half-scan data set");
+    return -1;
+  }
+  for (int iView2 = 0; iView2 < nView; iView2++) {
+    const double beta2 = iView2 * dbeta;
+    for (int iDet2 = 0; iDet2 < nDet; iDet2++) {
+      const double gamma2 = helix_gamma(geom, iDet2, nDet, detInc, sdl);
+      if (beta2 >= PI - 2 * gamma2) {
+        const int iDet1 = (nDet - 1) - iDet2;
+        const int iView1 = nearest_int(((iView2 * dbeta) + 2 * gamma2 - PI) / dbeta);
+        float* d2 = &data[(size_t)iView2 * nDet + iDet2];
+        float* d1 = &data[(size_t)iView1 * nDet + iDet1];
+        *d1 = *d2;
+        const double beta1 = iView1 * dbeta;
+        const double gamma1 = -gamma2;
+        double x;
+        if (beta1 <= (fanAngle - 2 * gamma1))
+          x = beta1 / (fanAngle - 2 * gamma1);
+        else if ((fanAngle - 2 * gamma1 <= beta1) && beta1 <= PI - 2 * gamma1)
+          x = 1;
+        else if ((PI - 2 * gamma1 <= beta1) && (beta1 <= PI + fanAngle))
+          x = (PI + fanAngle - beta1) / (fanAngle + 2 * gamma1);
+        else {
+          snprintf(g_err, sizeof g_err, "Shouldn't be here!");
+          return -4;
+        }
+        const double w1 = (3 * x - 2 * x * x) * x;
+        const double w2 = 1 - w1;
+        *d1 *= w1;
+        *d2 *= w2;
+      }
+    }
+  }
+  const double scalefactor = nView * rotInc / PI;
+  for (size_t k = 0; k < (size_t)nView * nDet; k++) data[k] *= scalefactor;
+  return 0;
+}

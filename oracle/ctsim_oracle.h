@@ -77,6 +77,15 @@ double orc_raysum_line(const int* types, const double* params, int nElem, const 
 int orc_phantom_extent(const int* types, const double* params, int nElem, const char* builtin,
                        double* ext);
 
+/* Helical 180LI interpolation and half-scan feathering
+ * (Projections::Helical180LI_Equiangular / HalfScanFeather,
+ * libctsim/projections.cpp:231-439); geom 1 equiangular, 2 equilinear. */
+int orc_helical180li(int geom, int nDet, int nView, double rotInc, double detInc,
+                     double fanAngle, double sdl, int interpView, const float* in, float* out,
+                     double* anglesOut);
+int orc_halfscan_feather(int geom, int nDet, int nView, double rotInc, double detInc,
+                         double fanAngle, double sdl, float* data);
+
 #ifdef __cplusplus
 }
 #endif

@@ -301,3 +301,35 @@ int ref_read_if(const char* path, int* nxny, float* image, int cap) {
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Projections::Helical180LI then (if feather) HalfScanFeather on a sinogram
+// whose geometry comes from a Scanner on a built-in phantom.  out receives
+// at most cap views; returns the number of output views, or < 0.
+int ref_helical(const char* builtin, const char* geometry, int nDet, int nView, int nSample,
+                double rotLen, double focalRatio, double centerRatio, const float* data,
+                int interpView, int feather, float* out, double* anglesOut, int cap,
+                double* info4) {
+  Phantom phm;
+  buildPhantom(phm, nullptr, nullptr, 0, builtin);
+  Scanner scanner(phm, geometry, nDet, nView, 0, nSample, rotLen, focalRatio, centerRatio, 1.0, 1.0);
+  if (scanner.fail()) { g_err = scanner.failMessage(); return -10; }
+  Projections pj(scanner);
+  for (int v = 0; v < nView; v++)
+    for (int i = 0; i < nDet; i++) pj.getDetectorArray(v).detValues()[i] = data[(size_t)v * nDet + i];
+  info4[0] = pj.rotInc();
+  info4[1] = pj.detInc();
+  info4[2] = scanner.fanBeamAngle();
+  info4[3] = pj.sourceDetectorLength();
+  if (pj.Helical180LI(interpView) != 0) { g_err = "Helical180LI failed"; return -1; }
+  if (feather && pj.HalfScanFeather() != 0) { g_err = "HalfScanFeather failed"; return -2; }
+  const int n = pj.nView();
+  for (int v = 0; v < n && v < cap; v++) {
+    anglesOut[v] = pj.getDetectorArray(v).viewAngle();
+    for (int i = 0; i < nDet; i++) out[(size_t)v * nDet + i] = pj.getDetectorArray(v).detValues()[i];
+  }
+  return n;
+}
+
+}  // extern "C"

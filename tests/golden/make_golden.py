@@ -119,11 +119,29 @@ def stats():
     save("stats", a=a, b=b, dre=np.array(O.stats("ref", a, b)))
 
 
+def helical():
+    """Projections::Helical180LI (+ HalfScanFeather) of seeded equiangular
+    sinograms: the default interpolation view and an explicit one.  The
+    parameters keep (pi + fan) / rotInc below a half fraction, where the
+    reference stays inside its arrays."""
+    rng = np.random.Generator(np.random.PCG64(11))
+    arrays = {}
+    for tag, nDet, nView, turns, iv, feather in [("a", 32, 200, 1.7, -1, True),
+                                                 ("b", 31, 200, 1.7, -1, False),
+                                                 ("c", 32, 200, 1.7, 100, True)]:
+        data = rng.standard_normal((nView, nDet)).astype(np.float32)
+        out, ang, info = O.ref_helical("herman", "equiangular", nDet, nView, 1, 2 * np.pi * turns,
+                                       2.0, 2.0, data, iv, feather)
+        arrays.update({f"{tag}_in": data, f"{tag}_out": out, f"{tag}_angles": ang,
+                       f"{tag}_info": np.array(info),
+                       f"{tag}_args": np.array([nDet, nView, iv, int(feather)], np.int64)})
+    save("helical", **arrays)
+
+
 if __name__ == "__main__":
     if not O.have_ref():
         O.build(ref=True)
-    config1()
-    tiny_grids()
-    filters()
-    raysums()
-    stats()
+    jobs = {"config1": config1, "tiny_grids": tiny_grids, "filters": filters,
+            "raysums": raysums, "stats": stats, "helical": helical}
+    for name in (sys.argv[1:] or list(jobs)):
+        jobs[name]()

@@ -1,0 +1,93 @@
+"""The mixed backprojector under every launch shape the autotuner may pick
+(lanes along y, rows per lane, view groups incl. the one-group-per-XCD
+mapping, views per LDS chunk, windowed and full-row staging): each shape gives
+the same image (to FP32 summation-order noise), within d <= 1e-4 of the FP64
+path; and the autotuner's pick is one of them."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(8, 1, 1, 4), (16, 4, 8, 16), (32, 2, 2, 32), (16, 2, 3, 5), (64, 1, 16, 1),
+          (16, 4, 1, 8)]
+
+
+def _slice(ctx, geometry, nDet, nView, nx, interp, backproj, rot, filt="abs_hanning",
+           method="fft"):
+    import torch
+    import ctsim_amd
+    from ctsim_amd import workloads as W
+    elems = W.seeded_elements(24, 0xC7511 + 77)
+    phm = ctsim_amd.Phantom(None, elems)
+    sc = ctsim_amd.Scanner(phm, geometry, nDet, nView, 0, 1, rot, 2.0, 2.0)
+    info = sc.info
+    sino = torch.empty((nView, nDet), dtype=torch.float32, device="cuda")
+    ang = sc.collect(ctx, sino, ctsim_amd.PREC_F64)
+    ps = ctsim_amd.ProcessSignal(ctx, filt, method, 1 / info["detInc"], info["detInc"], nDet,
+                                 -1.0, "spatial", "direct", 1, 1, geometry, info["focalLength"],
+                                 info["sourceDetectorLength"])
+    bp = ctsim_amd.Backprojector(ctx, geometry, nDet, info["detInc"], nView, info["viewDiameter"],
+                                 info["focalLength"], info["sourceDetectorLength"], backproj,
+                                 interp, nx, nx)
+    f64 = torch.empty((nView, nDet), dtype=torch.float64, device="cuda")
+    f32 = torch.empty((nView, nDet), dtype=torch.float32, device="cuda")
+    ps.filter(sino, f64, ctsim_amd.PREC_F64)
+    ps.filter(sino, f32, ctsim_amd.PREC_MIXED)
+    return bp, f64, f32, ang
+
+
+def _bp(ctx, bp, f, ang, nx, prec):
+    import torch
+    img = torch.zeros((nx, nx), dtype=torch.float32, device="cuda")
+    bp.backproject(f, ang, img, prec)
+    bp.postprocess(img)
+    return img
+
+
+CASES = [("equiangular", 512, 360, 256, "cubic", "idiff", 2 * np.pi),
+         ("equilinear", 383, 300, 192, "linear", "idiff", 2 * np.pi),
+         ("parallel", 367, 320, 256, "linear", "idiff", np.pi),
+         ("parallel", 300, 256, 128, "cubic", "trig", np.pi),
+         ("equiangular", 96, 64, 48, "linear", "table", 2 * np.pi)]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{c[4]}-{c[5]}-{c[3]}" for c in CASES])
+def test_shapes_agree(gpu_ctx, case):
+    import ctsim_amd
+    geometry, nDet, nView, nx, interp, backproj, rot = case
+    bp, f64, f32, ang = _slice(gpu_ctx, geometry, nDet, nView, nx, interp, backproj, rot)
+    ref = _bp(gpu_ctx, bp, f64, ang, nx, ctsim_amd.PREC_F64)
+    scale = float(ref.abs().max())
+    images = []
+    for s in SHAPES:
+        print(f"case {case[:6]} shape {s}", flush=True)
+        try:
+            bp.set_shape(*s)
+        except ctsim_amd.CtsimError as exc:  # staging too large for this geometry
+            assert "LDS" in str(exc)
+            continue
+        img = _bp(gpu_ctx, bp, f32, ang, nx, ctsim_amd.PREC_MIXED)
+        d, r, e = ctsim_amd.image_stats(gpu_ctx, ref, img)
+        print(f"  d={d:.3e} r={r:.3e} e={e:.3e}")
+        assert d <= 1e-4
+        images.append(img)
+    assert len(images) >= 3
+    for img in images[1:]:
+        assert float((img - images[0]).abs().max()) <= 2e-5 * scale
+    shape = bp.autotune(f32, ang)
+    print(f"  autotuned {shape}")
+    img = _bp(gpu_ctx, bp, f32, ang, nx, ctsim_amd.PREC_MIXED)
+    assert float((img - images[0]).abs().max()) <= 2e-5 * scale
+
+
+def test_bad_shape_rejected(gpu_ctx):
+    import ctsim_amd
+    g = load_golden("tiny_parallel_65")
+    info = g["info"]
+    b = ctsim_amd.Backprojector(gpu_ctx, "parallel", 65, float(info[0]), 60, float(info[3]),
+                                float(info[4]), float(info[5]), "idiff", "linear", 48, 48)
+    for s in [(12, 1, 1, 4), (16, 3, 1, 4), (16, 1, 0, 4), (16, 1, 1, 65)]:
+        with pytest.raises(ctsim_amd.CtsimError):
+            b.set_shape(*s)
