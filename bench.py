@@ -114,6 +114,11 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", default="config2",
+                    choices=["config2", "config3", "config4", "config5"],
+                    help="config2 (default): the headline single-slice measurement; the others "
+                         "are the large-size, batched and helical configurations")
+    ap.add_argument("--slices", type=int, default=256, help="config4: slices in the batch")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the benchmark, run three device copies of this many MiB "
                          "(a kernel of known traffic for scaling HBM counters)")
@@ -128,6 +133,14 @@ def main():
 
     import ctsim_amd
     from ctsim_amd import workloads as W
+
+    if args.workload != "config2":
+        assert torch.cuda.is_available(), "bench.py needs a GPU"
+        ctx = ctsim_amd.Context(0)
+        fn = {"config3": bench_config3, "config4": bench_config4, "config5": bench_config5}
+        out = fn[args.workload](ctx, args)
+        print(json.dumps(out), flush=True)
+        return
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     spec = W.config2()
@@ -268,6 +281,170 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log(f"[bench] calibration copies: 3 x {args.copy_calibration_mib} MiB read + write")
+
+
+def _line(metric, value, unit, args, ms, dtype, data, config, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": data,
+           "config": config}
+    out.update(extra)
+    return out
+
+
+def _time(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def bench_config3(ctx, args):
+    """Config 3: parallel 2048 det x 2880 views, 'shepp' convolution, IntDiff
+    linear into 2048^2: FP64 path vs mixed with the MFMA Toeplitz filter vs
+    mixed with an FFT convolution (a one-slice batch)."""
+    import torch
+    import ctsim_amd
+    from ctsim_amd import fbp, workloads as W
+    spec = W.config3()
+    log(f"[bench] config 3: {spec.geometry} {spec.nDet} det x {spec.nView} views, {spec.nx}^2, "
+        f"{spec.filter}/{spec.method}, {len(spec.elems)} elements")
+    sino, ang, info = fbp.simulate(ctx, spec, ctsim_amd.PREC_MIXED)
+    img64 = torch.zeros((spec.nx, spec.ny), dtype=torch.float32, device="cuda")
+    img32 = torch.zeros_like(img64)
+    imgb = torch.zeros_like(img64)
+    t64 = _time(lambda: fbp.reconstruct(ctx, spec, sino, ang, info, ctsim_amd.PREC_F64, img64), 1, 1)
+    log(f"[bench] FP64 path: {t64 * 1e3:.1f} ms/slice")
+    ps = ctsim_amd.ProcessSignal(ctx, spec.filter, spec.method, 1 / info["detInc"], info["detInc"],
+                                 spec.nDet, -1.0, "spatial", "direct", 1, 1, spec.geometry,
+                                 info["focalLength"], info["sourceDetectorLength"])
+    f32 = torch.empty((spec.nView, spec.nDet), dtype=torch.float32, device="cuda")
+    t_toe = _time(lambda: ps.filter(sino, f32, ctsim_amd.PREC_MIXED), args.steps, args.warmup)
+    t_mix = _time(lambda: fbp.reconstruct(ctx, spec, sino, ang, info, ctsim_amd.PREC_MIXED, img32),
+                  args.steps, args.warmup)
+    b = ctsim_amd.Batch(ctx, [fbp.slice_desc(spec, info, ang)])
+    b.autotune([sino], [imgb])
+    t_fft = _time(lambda: b.run([sino], [imgb]), args.steps, args.warmup)
+    b.status()
+    d_t = ctsim_amd.image_stats(ctx, img64, img32)
+    d_f = ctsim_amd.image_stats(ctx, img64, imgb)
+    log(f"[bench] mixed (Toeplitz MFMA filter {t_toe * 1e3:.3f} ms): {t_mix * 1e3:.2f} ms/slice, "
+        f"d={d_t[0]:.2e}; FFT-convolution batch: {t_fft * 1e3:.2f} ms/slice, d={d_f[0]:.2e}")
+    macs = spec.nView * spec.nDet * spec.nDet
+    return _line("FBP slices/s at 2048^2 parallel (config 3): mixed path", 1 / t_mix, "slices/s",
+                 args, t_mix * 1e3, "f32", "synthetic (256 ellipses + 32 rectangles)",
+                 {"workload": "config 3: parallel 2048 det x 2880 views over pi, shepp by "
+                              "convolution, IntDiff linear, 2048x2048"},
+                 fp64_gpu={"slices_per_s": 1 / t64, "ms_per_slice": t64 * 1e3},
+                 toeplitz_filter={"ms": t_toe * 1e3, "macs": macs,
+                                  "bf16x3_tflops": 6 * macs / t_toe / 1e12},
+                 fft_convolution_batch={"ms_per_slice": t_fft * 1e3, "d": d_f[0]},
+                 accuracy_vs_fp64={"d": d_t[0], "r": d_t[1], "e": d_t[2]})
+
+
+def bench_config4(ctx, args):
+    """Config 4: a batch of uneven slices in one persistent graph launch."""
+    import torch
+    import ctsim_amd
+    from ctsim_amd import fbp, workloads as W
+    specs = W.config4(args.slices)
+    log(f"[bench] config 4: {len(specs)} slices")
+    t0 = time.perf_counter()
+    sims = [fbp.simulate(ctx, s, ctsim_amd.PREC_MIXED) for s in specs]
+    torch.cuda.synchronize()
+    log(f"[bench] sinograms simulated in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    b = ctsim_amd.Batch(ctx, [fbp.slice_desc(s, info, ang) for s, (_, ang, info) in zip(specs, sims)])
+    log(f"[bench] batch plan in {time.perf_counter() - t0:.1f} s: {b.info()}")
+    sinos = [x[0] for x in sims]
+    images = [torch.zeros((s.nx, s.ny), dtype=torch.float32, device="cuda") for s in specs]
+    shape = b.autotune(sinos, images)
+    log(f"[bench] autotuned batch shape {shape}")
+    t = _time(lambda: b.run(sinos, images), args.steps, args.warmup)
+    b.status()
+    pvu = b.info()["pvu"]
+    # accuracy on a sample of slices against the FP64 single-slice path
+    ds = []
+    for k in range(0, len(specs), max(1, len(specs) // 4)):
+        ref = fbp.reconstruct(ctx, specs[k], sims[k][0], sims[k][1], sims[k][2], ctsim_amd.PREC_F64)
+        ds.append(ctsim_amd.image_stats(ctx, ref, images[k])[0])
+    log(f"[bench] batch: {t * 1e3:.2f} ms, {len(specs) / t:.1f} slices/s, {pvu / t / 1e12:.3f} T PVU/s; "
+        f"sampled d vs FP64: max {max(ds):.2e}")
+    return _line("FBP slices/s, uneven batch of 256 slices (config 4), mixed", len(specs) / t,
+                 "slices/s", args, t * 1e3, "f32", "synthetic (seeded phantoms per slice)",
+                 {"workload": f"config 4: {len(specs)} seeded uneven slices, abs_bandlimit, linear",
+                  "slices_per_step": len(specs)},
+                 pvu_per_s=pvu / t, launch_shape=shape, accuracy_vs_fp64={"d_max_sampled": max(ds)})
+
+
+def bench_config5(ctx, args):
+    """Config 5: time-varying phantom scanned over several rotations, FP64
+    180LI interpolation to 64 planar half-scan sinograms, batched FBP."""
+    import numpy as np
+    import torch
+    import ctsim_amd
+    from ctsim_amd import fbp, workloads as W
+    h = W.config5()
+    base = h.base()
+    phm0 = ctsim_amd.Phantom(None, h.elements(0, base))
+    # geometry from the view-0 phantom, as tools/phm2helix.cpp:287-296
+    probe = ctsim_amd.Scanner(phm0, h.geometry, h.nDet, h.viewsPerRotation, 0, 1, 2 * np.pi,
+                              h.focalRatio, h.centerRatio)
+    fan, rotInc = probe.info["fanBeamAngle"], 2 * np.pi / h.viewsPerRotation
+    L = int((np.pi + fan) / rotInc)
+    nTotal = 2 * L + 1 + (h.positions - 1) * h.stride() + 2
+    sc = ctsim_amd.Scanner(phm0, h.geometry, h.nDet, nTotal, 0, 1, rotInc * nTotal, h.focalRatio,
+                           h.centerRatio)
+    info = sc.info
+    log(f"[bench] config 5: {nTotal} views ({nTotal / h.viewsPerRotation:.2f} rotations), "
+        f"{h.positions} positions, L={L}")
+    sino = torch.empty((nTotal, h.nDet), dtype=torch.float32, device="cuda")
+    t0 = time.perf_counter()
+    for v in range(nTotal):  # one phantom per view, as phm2helix.cpp:299-317
+        phm = ctsim_amd.Phantom(None, h.elements(v, base))
+        sc.collect(ctx, sino, ctsim_amd.PREC_F64, iStartView=v, iNumViews=1, phantom=phm)
+    torch.cuda.synchronize()
+    log(f"[bench] time-varying scan simulated in {time.perf_counter() - t0:.1f} s")
+    spec = W.SliceSpec(geometry=h.geometry, nDet=h.nDet, nView=L + 1, filter=h.filter,
+                       method=h.method, interp=h.interp, backproj=h.backproj, nx=h.nx, ny=h.ny)
+    ivs = [L + 1 + p * h.stride() for p in range(h.positions)]
+
+    def interpolate():
+        outs = []
+        for iv in ivs:
+            o, a = ctsim_amd.helical180li(ctx, h.geometry, sino, info["rotInc"], info["detInc"],
+                                          fan, info["sourceDetectorLength"], iv)
+            outs.append((ctsim_amd.halfscan_feather(ctx, h.geometry, o, info["rotInc"],
+                                                    info["detInc"], fan,
+                                                    info["sourceDetectorLength"]), a))
+        return outs
+
+    planar = interpolate()
+    b = ctsim_amd.Batch(ctx, [fbp.slice_desc(spec, info, a) for _, a in planar])
+    sinos = [p for p, _ in planar]
+    images = [torch.zeros((h.nx, h.ny), dtype=torch.float32, device="cuda") for _ in ivs]
+    shape = b.autotune(sinos, images)
+    t_i = _time(interpolate, max(1, args.steps // 4), 1)
+    t_b = _time(lambda: b.run(sinos, images), args.steps, args.warmup)
+    b.status()
+    ref = fbp.reconstruct(ctx, spec, sinos[0], planar[0][1], info, ctsim_amd.PREC_F64)
+    d = ctsim_amd.image_stats(ctx, ref, images[0])
+    total = t_i + t_b
+    log(f"[bench] helical interpolation {t_i * 1e3:.2f} ms, batched FBP {t_b * 1e3:.2f} ms "
+        f"({h.positions / total:.1f} slices/s); d vs FP64 (position 0) {d[0]:.2e}")
+    return _line("Helical slices/s (config 5): FP64 180LI + mixed batched FBP at 512^2",
+                 h.positions / total, "slices/s", args, total * 1e3, "f32",
+                 "synthetic (32 drifting ellipses, one phantom per view)",
+                 {"workload": f"config 5: equilinear {h.nDet} det, {h.viewsPerRotation} views per "
+                              f"rotation, {h.positions} positions, 512x512",
+                  "slices_per_step": h.positions},
+                 interpolation_ms=t_i * 1e3, batch_fbp_ms=t_b * 1e3, launch_shape=shape,
+                 accuracy_vs_fp64={"d": d[0], "r": d[1], "e": d[2]})
 
 
 if __name__ == "__main__":

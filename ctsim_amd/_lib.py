@@ -46,6 +46,17 @@ SIGNATURES = {
     "ctsim_scanner_info": (_i, [_vp, _vp]),
     "ctsim_collect_projections": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp]),
     "ctsim_image_stats": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
+    "ctsim_rebin_create": (_i, [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _d, _d, _vp]),
+    "ctsim_rebin_destroy": (None, [_vp]),
+    "ctsim_rebin_info": (_i, [_vp, _vp, _vp]),
+    "ctsim_rebin_run": (_i, [_vp, _vp, _vp, _vp]),
+    "ctsim_batch_create": (_i, [_vp, _vp, _vp, _i]),
+    "ctsim_batch_destroy": (None, [_vp]),
+    "ctsim_batch_run": (_i, [_vp, _vp, _vp, _vp]),
+    "ctsim_batch_status": (_i, [_vp, _vp]),
+    "ctsim_batch_set_shape": (_i, [_vp, _vp, _i, _i, _i, _i]),
+    "ctsim_batch_autotune": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "ctsim_batch_info": (_i, [_vp, _vp]),
     "ctsim_helical180li": (_i, [_vp, _i, _i, _i, _d, _d, _d, _d, _i, _vp, _vp, _i, _vp, _vp]),
     "ctsim_halfscan_feather": (_i, [_vp, _i, _i, _i, _d, _d, _d, _d, _vp, _vp]),
     "ctsim_ctx_profile": (_i, [_vp, _i]),

@@ -124,6 +124,131 @@ def halfscan_feather(ctx: Context, geometry, views, rotInc: float, detInc: float
     return out
 
 
+class Rebin:
+    """Fan-beam to parallel rebinning plan (Projections::interpolateToParallel,
+    libctsim/projections.cpp:1250-1328)."""
+
+    def __init__(self, ctx: Context, geometry, nDet: int, nView: int, detStart: float,
+                 detInc: float, focalLength: float, sourceDetectorLength: float,
+                 viewDiameter: float, rotStart: float, angles: np.ndarray):
+        self.ctx = ctx
+        g = GEOMETRY_IDS[geometry] if isinstance(geometry, str) else int(geometry)
+        ang = np.ascontiguousarray(angles, dtype=np.float64)
+        if ang.shape[0] < nView:
+            raise ValueError("not enough view angles")
+        h = C.c_void_p()
+        check(lib.ctsim_rebin_create(ctx.handle, C.byref(h), g, nDet, nView, detStart, detInc,
+                                     focalLength, sourceDetectorLength, viewDiameter, rotStart,
+                                     _hptr(ang)), "ctsim_rebin_create")
+        self.handle = h
+        self.nDet, self.nView = nDet, nView
+        o = np.zeros(4, np.float64)
+        self.angles = np.zeros(nView, np.float64)
+        check(lib.ctsim_rebin_info(h, _hptr(o), _hptr(self.angles)), "ctsim_rebin_info")
+        self.rotStart, self.rotInc, self.detStart, self.detInc = (float(x) for x in o)
+
+    def run(self, sino):
+        """Parallel projections (new device array) of a float32 [nView, nDet] sinogram."""
+        import torch
+        if sino.dtype != torch.float32 or tuple(sino.shape) != (self.nView, self.nDet):
+            raise TypeError("sino must be float32 [nView, nDet]")
+        src = sino.contiguous()
+        out = torch.empty_like(src)
+        check(lib.ctsim_rebin_run(self.ctx.handle, self.handle, C.c_void_p(_dptr(src)),
+                                  C.c_void_p(_dptr(out))), "ctsim_rebin_run")
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.ctsim_rebin_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+class SliceDesc(C.Structure):
+    """ctsim_slice_desc of include/ctsim_hip.h."""
+    _fields_ = [("geometry", C.c_int), ("nDet", C.c_int), ("nView", C.c_int),
+                ("detInc", C.c_double), ("viewDiameter", C.c_double),
+                ("focalLength", C.c_double), ("sourceDetectorLength", C.c_double),
+                ("filter", C.c_char_p), ("method", C.c_char_p), ("filterParam", C.c_double),
+                ("generation", C.c_char_p), ("zeropad", C.c_int), ("backproj", C.c_char_p),
+                ("interp", C.c_char_p), ("nx", C.c_int), ("ny", C.c_int),
+                ("angles", C.c_void_p)]
+
+
+class Batch:
+    """Batched mixed-precision FBP of many slices (one persistent launch in a
+    HIP graph).  ``slices``: dicts with the SliceDesc fields (``angles`` a
+    numpy array)."""
+
+    def __init__(self, ctx: Context, slices):
+        self.ctx = ctx
+        self._keep = []
+        arr = (SliceDesc * len(slices))()
+        self.shapes = []
+        for k, d in enumerate(slices):
+            ang = np.ascontiguousarray(d["angles"], dtype=np.float64)
+            self._keep.append(ang)
+            g = GEOMETRY_IDS[d["geometry"]] if isinstance(d["geometry"], str) else int(d["geometry"])
+            arr[k] = SliceDesc(g, d["nDet"], d["nView"], d["detInc"], d["viewDiameter"],
+                               d["focalLength"], d["sourceDetectorLength"], d["filter"].encode(),
+                               d["method"].encode(), d.get("filterParam", -1.0),
+                               d.get("generation", "direct").encode(), d.get("zeropad", 1),
+                               d.get("backproj", "idiff").encode(), d["interp"].encode(),
+                               d["nx"], d["ny"], ang.ctypes.data)
+            self.shapes.append((d["nView"], d["nDet"], d["nx"], d["ny"]))
+        h = C.c_void_p()
+        check(lib.ctsim_batch_create(ctx.handle, C.byref(h), arr, len(slices)), "ctsim_batch_create")
+        self.handle = h
+
+    def _ptrs(self, sinos, images):
+        import torch
+        if len(sinos) != len(self.shapes) or len(images) != len(self.shapes):
+            raise ValueError("one sinogram and one image per slice")
+        for (nv, nd, nx, ny), s, im in zip(self.shapes, sinos, images):
+            if s.dtype != torch.float32 or tuple(s.shape) != (nv, nd):
+                raise TypeError("sinogram must be float32 [nView, nDet]")
+            if im.dtype != torch.float32 or tuple(im.shape) != (nx, ny):
+                raise TypeError("image must be float32 [nx, ny]")
+        sp = (C.c_void_p * len(sinos))(*[_dptr(s) for s in sinos])
+        ip = (C.c_void_p * len(images))(*[_dptr(i) for i in images])
+        return sp, ip
+
+    def run(self, sinos, images):
+        sp, ip = self._ptrs(sinos, images)
+        check(lib.ctsim_batch_run(self.ctx.handle, self.handle, sp, ip), "ctsim_batch_run")
+
+    def status(self):
+        check(lib.ctsim_batch_status(self.ctx.handle, self.handle), "ctsim_batch_status")
+
+    def set_shape(self, P: int, lanesY: int, chunkBytes: int, fused: bool):
+        check(lib.ctsim_batch_set_shape(self.ctx.handle, self.handle, P, lanesY, chunkBytes,
+                                        int(fused)), "ctsim_batch_set_shape")
+
+    def autotune(self, sinos, images):
+        sp, ip = self._ptrs(sinos, images)
+        shape = np.zeros(4, np.int32)
+        ms = np.zeros(1, np.float64)
+        check(lib.ctsim_batch_autotune(self.ctx.handle, self.handle, sp, ip, _hptr(shape),
+                                       _hptr(ms)), "ctsim_batch_autotune")
+        return (*map(int, shape), float(ms[0]))
+
+    def info(self):
+        o = np.zeros(4, np.float64)
+        check(lib.ctsim_batch_info(self.handle, _hptr(o)), "ctsim_batch_info")
+        return {"slices": int(o[0]), "pvu": o[1], "fused": bool(o[2]), "grid": int(o[3])}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.ctsim_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
 class Backprojector:
     """Backprojector plan (libctsim/backprojectors.cpp:100-319)."""
 
@@ -313,9 +438,10 @@ class Scanner:
 
     def collect(self, ctx: Context, sino, precision: int = PREC_F64, iStartView: int = 0,
                 iNumViews: int | None = None, iOffsetView: int | None = None,
-                iStorageOffset: int | None = None):
+                iStorageOffset: int | None = None, phantom: Phantom | None = None):
         """Ray sums into ``sino`` (device float32 [nView, nDet]); returns the
-        view angles (host float64)."""
+        view angles (host float64).  ``phantom`` (default: the scanner's)
+        is the object scanned, as collectProjections' phm argument."""
         import torch
         if sino.dtype != torch.float32 or sino.shape[-1] != self.nDet:
             raise TypeError("sino must be float32 [nView, nDet]")
@@ -325,7 +451,8 @@ class Scanner:
         if store + n > sino.shape[0]:
             raise ValueError("sinogram has too few rows")
         ang = np.zeros(sino.shape[0], np.float64)
-        check(lib.ctsim_collect_projections(ctx.handle, self.handle, self.phantom.handle,
+        phm = self.phantom if phantom is None else phantom
+        check(lib.ctsim_collect_projections(ctx.handle, self.handle, phm.handle,
                                             precision, iStartView, n, off, store,
                                             C.c_void_p(_dptr(sino)), _hptr(ang)),
               "ctsim_collect_projections")

@@ -1,0 +1,636 @@
+// Batched mixed-precision FBP of many slices of uneven size, detector count,
+// view count and geometry in one persistent kernel, replayed from a HIP graph.
+//
+// The reference reconstructs one slice at a time (tools/pjrec.cpp:342-347,
+// Reconstructor::reconstructView, libctsim/reconstruct.cpp:156-239).  Here a
+// batch plan holds, per slice, the reference's filter (ProcessSignal::init)
+// and backprojector geometry (Backproject ctor), and a work list:
+//   FILTER items  two views of one slice: FFT filtering in LDS (convolution
+//                 filters as an FFT convolution with the kernel's transform),
+//                 then the interval coefficients of the slice's
+//                 interpolation, written for its backprojection;
+//   BP items      one tile of one slice over all its views (the mixed
+//                 backprojection body), image = rotScale * sum over views.
+// Slices are dealt to the 8 XCDs by longest-processing-time first; each XCD
+// has its own queue (its filter items, then its BP items, largest slices
+// first), so a slice's coefficients are produced and consumed under one L2.
+// Workgroups pull items from their XCD's queue with an atomic counter.  A BP
+// item waits (bounded spin, error flag on expiry) until its slice's filter
+// items are done; every item it can wait on sits earlier in the same queue
+// and is held by a running workgroup, so the wait always ends.  Fused mode
+// runs both item kinds in one launch; two-phase mode launches the filter
+// items and then the BP items (no waiting, smaller LDS for the BP launch).
+// The autotuner picks the mode, the tile shape and the views per LDS chunk.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+#include "bp_plan.h"
+#include "fft_device.h"
+#include "filter_plan.h"
+#include "mixed_device.h"
+
+namespace ctsim {
+namespace {
+
+using dev::CoefT;
+using dev::Cplx;
+using dev::MixedArgs;
+
+constexpr int kFilterItem = 0, kBpItem = 1;
+constexpr int kSpinLimit = 1 << 22;  // about 0.2 s of polling
+
+struct SliceDev {
+  // filtering
+  int n, N, logN, R, nView, nFilterItems;
+  const float* sino;
+  const double* w;
+  const float* hre;
+  const float* him;
+  const Cplx<float>* tw;
+  // backprojection
+  int gk, interp, lastIncl, deg;
+  MixedArgs bp;  // bp.coef, bp.image, bp.vc set; overwrite = 1, split = 1
+};
+
+struct Item {
+  int kind, slice, a, pad;
+};
+
+// Zero the queue heads and per-slice counters (a kernel, not a memset, so the
+// graph holds no memset node).
+__global__ void batch_reset_kernel(int* counters, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) counters[i] = 0;
+}
+
+template <int P>
+__device__ void bp_item(const MixedArgs& a, int gk, int interp, int deg, int tile, void* smem,
+                        float4* s_v, int* s_lo) {
+  using dev::bp_mixed_tile;
+  if (interp == INTERP_CUBIC) {
+    float4* w = static_cast<float4*>(smem);
+    if (gk == 0) bp_mixed_tile<0, INTERP_CUBIC, P, 0>(a, tile, 0, w, s_v, s_lo);
+    else if (gk == 2) bp_mixed_tile<2, INTERP_CUBIC, P, 0>(a, tile, 0, w, s_v, s_lo);
+    else if (deg == 4) bp_mixed_tile<1, INTERP_CUBIC, P, 4>(a, tile, 0, w, s_v, s_lo);
+    else if (deg == 6) bp_mixed_tile<1, INTERP_CUBIC, P, 6>(a, tile, 0, w, s_v, s_lo);
+    else if (deg == 8) bp_mixed_tile<1, INTERP_CUBIC, P, 8>(a, tile, 0, w, s_v, s_lo);
+    else bp_mixed_tile<1, INTERP_CUBIC, P, 0>(a, tile, 0, w, s_v, s_lo);
+  } else {
+    float2* w = static_cast<float2*>(smem);
+    if (gk == 0) bp_mixed_tile<0, INTERP_LINEAR, P, 0>(a, tile, 0, w, s_v, s_lo);
+    else if (gk == 2) bp_mixed_tile<2, INTERP_LINEAR, P, 0>(a, tile, 0, w, s_v, s_lo);
+    else if (deg == 4) bp_mixed_tile<1, INTERP_LINEAR, P, 4>(a, tile, 0, w, s_v, s_lo);
+    else if (deg == 6) bp_mixed_tile<1, INTERP_LINEAR, P, 6>(a, tile, 0, w, s_v, s_lo);
+    else if (deg == 8) bp_mixed_tile<1, INTERP_LINEAR, P, 8>(a, tile, 0, w, s_v, s_lo);
+    else bp_mixed_tile<1, INTERP_LINEAR, P, 0>(a, tile, 0, w, s_v, s_lo);
+  }
+}
+
+template <int R>
+__device__ void filter_item(const SliceDev& s, int pair, Cplx<float>* z, Cplx<float>* tw) {
+  const int va = 2 * pair, vb = va + 1 < s.nView ? va + 1 : -1;
+  dev::fft_filter_pair<float, R>(s.sino, s.w, s.hre, s.him, s.tw, va, vb, s.n, s.N, s.logN, z, tw);
+  // interval coefficients of both rows straight from LDS
+  const int n = s.n;
+  auto ra = [z](int j) { return z[j].x; };
+  auto rb = [z](int j) { return z[j].y; };
+  if (s.interp == INTERP_CUBIC) {
+    float4* c = static_cast<float4*>(const_cast<void*>(s.bp.coef));
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      c[static_cast<size_t>(va) * n + j] = dev::interval_coef<INTERP_CUBIC, false>(ra, j, n);
+      if (vb >= 0) c[static_cast<size_t>(vb) * n + j] = dev::interval_coef<INTERP_CUBIC, false>(rb, j, n);
+    }
+  } else {
+    float2* c = static_cast<float2*>(const_cast<void*>(s.bp.coef));
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      if (s.lastIncl) {
+        c[static_cast<size_t>(va) * n + j] = dev::interval_coef<INTERP_LINEAR, true>(ra, j, n);
+        if (vb >= 0) c[static_cast<size_t>(vb) * n + j] = dev::interval_coef<INTERP_LINEAR, true>(rb, j, n);
+      } else {
+        c[static_cast<size_t>(va) * n + j] = dev::interval_coef<INTERP_LINEAR, false>(ra, j, n);
+        if (vb >= 0) c[static_cast<size_t>(vb) * n + j] = dev::interval_coef<INTERP_LINEAR, false>(rb, j, n);
+      }
+    }
+  }
+}
+
+// counters: [0, 8) queue heads, [8, 8 + nSlices) filter items done per slice.
+template <int P>
+__global__ __launch_bounds__(256) void fbp_batch_kernel(const SliceDev* __restrict__ slices,
+                                                        const Item* __restrict__ items,
+                                                        const int* __restrict__ queueStart,
+                                                        int* counters, int* error) {
+  extern __shared__ float4 smem[];
+  __shared__ float4 s_v[dev::kMaxChunk];
+  __shared__ int s_lo[dev::kMaxChunk];
+  __shared__ int s_item;
+  const int q = blockIdx.x & 7;
+  const int qBegin = queueStart[q], qEnd = queueStart[q + 1];
+  int* done = counters + 8;
+  while (true) {
+    if (threadIdx.x == 0) s_item = qBegin + atomicAdd(&counters[q], 1);
+    __syncthreads();
+    const int i = s_item;
+    __syncthreads();
+    if (i >= qEnd) break;  // every workgroup ends when its queue is drained
+    const Item it = items[i];
+    const SliceDev& s = slices[it.slice];
+    if (it.kind == kFilterItem) {
+      Cplx<float>* z = reinterpret_cast<Cplx<float>*>(smem);
+      Cplx<float>* tw = z + s.N;
+      switch (s.R) {
+        case 1: filter_item<1>(s, it.a, z, tw); break;
+        case 2: filter_item<2>(s, it.a, z, tw); break;
+        case 4: filter_item<4>(s, it.a, z, tw); break;
+        default: filter_item<8>(s, it.a, z, tw); break;
+      }
+      __threadfence();  // every lane releases its coefficient stores device-wide
+      __syncthreads();
+      if (threadIdx.x == 0) atomicAdd(&done[it.slice], 1);
+    } else {
+      if (threadIdx.x == 0) {
+        int spins = 0;
+        while (__hip_atomic_load(&done[it.slice], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+               s.nFilterItems) {
+          __builtin_amdgcn_s_sleep(2);
+          if (++spins > kSpinLimit) {
+            atomicOr(error, 1);
+            break;
+          }
+        }
+        __threadfence();  // acquire: the coefficients are visible below
+      }
+      __syncthreads();
+      bp_item<P>(s.bp, s.gk, s.interp, s.deg, it.a, smem, s_v, s_lo);
+    }
+    __syncthreads();  // LDS is reused by the next item
+  }
+}
+
+// ------------------------------------------------------------------ host
+
+void fft_host(std::vector<std::complex<double>>& a) {
+  const int n = static_cast<int>(a.size());
+  for (int i = 1, j = 0; i < n; i++) {
+    int bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    for (int i = 0; i < n; i += len)
+      for (int k = 0; k < len / 2; k++) {
+        const double ang = -2 * kPi * k / len;
+        const std::complex<double> w(cos(ang), sin(ang));
+        const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+  }
+}
+
+}  // namespace
+}  // namespace ctsim
+
+using namespace ctsim;
+
+struct ctsim_batch {
+  struct Slice {
+    FilterPlan fp;
+    BpPlan bp;
+    std::vector<double> angles;
+    int N = 0, logN = 0;
+    DevBuf mul, tw;  // FFT-convolution multipliers and twiddles (convolution filters)
+    DevBuf vc, coef;
+    double cost = 0;
+  };
+  std::vector<Slice> s;
+  int P = 2, lanesY = 16, chunkBudget = 32 * 1024;
+  bool fused = true;
+  int ldsFilter = 0;
+  std::vector<SliceDev> host;
+  DevBuf d_slices, d_itemsF, d_itemsB, d_itemsAll, d_qF, d_qB, d_qAll, d_counters, d_error;
+  int gridF = 0, gridB = 0, gridAll = 0;
+  size_t ldsB = 0, ldsAll = 0;
+  std::vector<const float*> sinos;
+  std::vector<float*> images;
+  hipStream_t stream = nullptr;
+  hipEvent_t evIn = nullptr, evOut = nullptr;
+  hipGraphExec_t graph = nullptr;
+  bool shapeDirty = true;
+  double pvu = 0;
+  ~ctsim_batch() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (evIn) (void)hipEventDestroy(evIn);
+    if (evOut) (void)hipEventDestroy(evOut);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace ctsim {
+namespace {
+
+int occupancy_grid(const void* kernel, size_t lds, int numCUs) {
+  int perCU = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, 256, lds) != hipSuccess || perCU < 1)
+    perCU = 1;
+  int g = perCU * numCUs;
+  g -= g % 8;
+  return g < 8 ? 8 : g;
+}
+
+// Build the per-slice device descriptors and the item queues for the
+// current shape (P, lanesY, chunk budget).
+int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
+  const int S = static_cast<int>(b.s.size());
+  b.host.assign(S, SliceDev{});
+  size_t maxBpLds = 0;
+  for (int k = 0; k < S; k++) {
+    auto& sl = b.s[k];
+    const BpPlan& p = sl.bp;
+    SliceDev& d = b.host[k];
+    d.n = p.nDet;
+    d.N = sl.N;
+    d.logN = sl.logN;
+    d.R = (sl.N / 2 + 255) / 256;
+    d.nView = p.nView;
+    d.nFilterItems = (p.nView + 1) / 2;
+    d.sino = b.sinos.empty() ? nullptr : b.sinos[k];
+    d.w = sl.fp.preweight.empty() ? nullptr : sl.fp.d_weight.as<double>();
+    if (sl.fp.method == METHOD_CONVOLUTION) {
+      d.hre = sl.mul.as<float>();
+      d.him = sl.mul.as<float>();
+      d.tw = sl.tw.as<Cplx<float>>();
+    } else {
+      d.hre = sl.fp.d_hmul32.as<float>();
+      d.him = sl.fp.d_hmul32.as<float>() + (sl.N / 2 + 1);
+      d.tw = sl.fp.d_tw32.as<Cplx<float>>();
+    }
+    d.gk = p.geom == GEOM_PARALLEL ? 0 : (p.geom == GEOM_EQUIANGULAR ? 1 : 2);
+    d.interp = p.interp;
+    d.lastIncl = p.kind == KIND_IDIFF;
+    d.deg = d.gk == 1 ? p.atanDeg : 0;
+    // backprojection shape (P is the batch kernel's template parameter)
+    MixedShape m;
+    m.lanesY = b.lanesY;
+    m.P = b.P;
+    m.split = 1;
+    m.W = bp_window_bound(p, 256 / m.lanesY, m.lanesY * m.P);
+    const int Wv = m.W > 0 ? m.W : p.nDet + 4;
+    const size_t elem = p.interp == INTERP_CUBIC ? 16 : 8;
+    int chunk = static_cast<int>(b.chunkBudget / (Wv * elem));
+    chunk = chunk < 1 ? 1 : (chunk > dev::kMaxChunk ? dev::kMaxChunk : chunk);
+    m.chunk = chunk;
+    if (static_cast<size_t>(chunk) * Wv * elem > 62 * 1024)
+      return fail(CTSIM_ERR_UNSUPPORTED, "batch: a slice's detector window does not fit in LDS");
+    maxBpLds = std::max(maxBpLds, static_cast<size_t>(chunk) * Wv * elem);
+    MixedArgs& a = d.bp;
+    a = MixedArgs{};
+    a.nx = p.nx; a.ny = p.ny; a.nDet = p.nDet; a.nView = p.nView;
+    a.lanesY = m.lanesY;
+    const int tx = 256 / m.lanesY, ty = m.lanesY * m.P;
+    a.tilesX = (p.nx + tx - 1) / tx;
+    a.tilesY = (p.ny + ty - 1) / ty;
+    a.W = Wv;
+    a.chunk = m.chunk;
+    a.split = 1;
+    a.viewsPerGroup = p.nView;
+    a.center = static_cast<float>(p.iDetCenter);
+    a.F = static_cast<float>(p.F);
+    a.twoF = static_cast<float>(2 * p.F);
+    a.endScale = static_cast<float>(p.rotScale);
+    a.posScale = d.gk == 2 ? static_cast<float>(p.sdl / p.detInc) : static_cast<float>(1.0 / p.detInc);
+    if (d.deg > 0) for (int j = 0; j <= d.deg; j++) a.atan[j] = p.atanCoef[j];
+    else a.atan[0] = static_cast<float>(1.0 / p.detInc);
+    a.xs = p.d_xy.as<float>();
+    a.ys = p.d_xy.as<float>() + p.xs.size();
+    a.vc = sl.vc.as<ViewConstF32>();
+    a.coef = sl.coef.ptr;
+    a.image = b.images.empty() ? nullptr : b.images[k];
+    a.overwrite = 1;
+  }
+  // slices to XCDs, longest first
+  std::vector<int> order(S);
+  for (int k = 0; k < S; k++) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b.s[x].cost > b.s[y].cost; });
+  std::vector<std::vector<int>> xcdSlices(8);
+  std::vector<double> load(8, 0.0);
+  for (int k : order) {
+    const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+    xcdSlices[x].push_back(k);
+    load[x] += b.s[k].cost;
+  }
+  std::vector<Item> itemsF, itemsB, itemsAll;
+  std::vector<int> qF(9), qB(9), qAll(9);
+  for (int x = 0; x < 8; x++) {
+    qF[x] = static_cast<int>(itemsF.size());
+    qB[x] = static_cast<int>(itemsB.size());
+    qAll[x] = static_cast<int>(itemsAll.size());
+    std::vector<Item> f, bp;
+    for (int k : xcdSlices[x]) {
+      for (int pr = 0; pr < b.host[k].nFilterItems; pr++) f.push_back({kFilterItem, k, pr, 0});
+      const int tiles = b.host[k].bp.tilesX * b.host[k].bp.tilesY;
+      for (int t = 0; t < tiles; t++) bp.push_back({kBpItem, k, t, 0});
+    }
+    itemsF.insert(itemsF.end(), f.begin(), f.end());
+    itemsB.insert(itemsB.end(), bp.begin(), bp.end());
+    itemsAll.insert(itemsAll.end(), f.begin(), f.end());
+    itemsAll.insert(itemsAll.end(), bp.begin(), bp.end());
+  }
+  qF[8] = static_cast<int>(itemsF.size());
+  qB[8] = static_cast<int>(itemsB.size());
+  qAll[8] = static_cast<int>(itemsAll.size());
+  auto up = [](DevBuf& d, const void* src, size_t n) -> int {
+    CTSIM_CHECK(d.reserve(n));
+    CTSIM_HIP(hipMemcpy(d.ptr, src, n, hipMemcpyHostToDevice));
+    return CTSIM_OK;
+  };
+  CTSIM_CHECK(up(b.d_slices, b.host.data(), b.host.size() * sizeof(SliceDev)));
+  CTSIM_CHECK(up(b.d_itemsF, itemsF.data(), itemsF.size() * sizeof(Item)));
+  CTSIM_CHECK(up(b.d_itemsB, itemsB.data(), itemsB.size() * sizeof(Item)));
+  CTSIM_CHECK(up(b.d_itemsAll, itemsAll.data(), itemsAll.size() * sizeof(Item)));
+  CTSIM_CHECK(up(b.d_qF, qF.data(), qF.size() * sizeof(int)));
+  CTSIM_CHECK(up(b.d_qB, qB.data(), qB.size() * sizeof(int)));
+  CTSIM_CHECK(up(b.d_qAll, qAll.data(), qAll.size() * sizeof(int)));
+  CTSIM_CHECK(b.d_counters.reserve((8 + S) * sizeof(int)));
+  CTSIM_CHECK(b.d_error.reserve(sizeof(int)));
+  CTSIM_HIP(hipMemset(b.d_error.ptr, 0, sizeof(int)));
+  b.ldsB = maxBpLds;
+  b.ldsAll = std::max(maxBpLds, static_cast<size_t>(b.ldsFilter));
+  const void* kern = b.P == 4 ? reinterpret_cast<const void*>(fbp_batch_kernel<4>)
+                              : (b.P == 1 ? reinterpret_cast<const void*>(fbp_batch_kernel<1>)
+                                          : reinterpret_cast<const void*>(fbp_batch_kernel<2>));
+  b.gridF = occupancy_grid(kern, b.ldsFilter, ctx->numCUs);
+  b.gridB = occupancy_grid(kern, b.ldsB, ctx->numCUs);
+  b.gridAll = occupancy_grid(kern, b.ldsAll, ctx->numCUs);
+  b.shapeDirty = false;
+  if (b.graph) {
+    (void)hipGraphExecDestroy(b.graph);
+    b.graph = nullptr;
+  }
+  return CTSIM_OK;
+}
+
+template <int P>
+void launch_batch(ctsim_batch& b, hipStream_t st) {
+  const int S = static_cast<int>(b.s.size());
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8 + S);
+  if (b.fused) {
+    hipLaunchKernelGGL((fbp_batch_kernel<P>), dim3(b.gridAll), dim3(256), b.ldsAll, st,
+                       b.d_slices.as<SliceDev>(), b.d_itemsAll.as<Item>(), b.d_qAll.as<int>(),
+                       b.d_counters.as<int>(), b.d_error.as<int>());
+  } else {
+    hipLaunchKernelGGL((fbp_batch_kernel<P>), dim3(b.gridF), dim3(256), b.ldsFilter, st,
+                       b.d_slices.as<SliceDev>(), b.d_itemsF.as<Item>(), b.d_qF.as<int>(),
+                       b.d_counters.as<int>(), b.d_error.as<int>());
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8);
+    hipLaunchKernelGGL((fbp_batch_kernel<P>), dim3(b.gridB), dim3(256), b.ldsB, st,
+                       b.d_slices.as<SliceDev>(), b.d_itemsB.as<Item>(), b.d_qB.as<int>(),
+                       b.d_counters.as<int>(), b.d_error.as<int>());
+  }
+}
+
+int batch_enqueue(ctsim_ctx* ctx, ctsim_batch& b) {
+  // capture once per layout, replay afterwards
+  if (!b.graph) {
+    hipGraph_t g = nullptr;
+    CTSIM_HIP(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+    if (b.P == 4) launch_batch<4>(b, b.stream);
+    else if (b.P == 1) launch_batch<1>(b, b.stream);
+    else launch_batch<2>(b, b.stream);
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(b.stream, &g);
+    if (le != hipSuccess) return hip_fail(le, "batch launch", __FILE__, __LINE__);
+    if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture", __FILE__, __LINE__);
+    const hipError_t ie = hipGraphInstantiate(&b.graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+      b.graph = nullptr;
+      return hip_fail(ie, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
+  }
+  CTSIM_HIP(hipEventRecord(b.evIn, ctx->stream));
+  CTSIM_HIP(hipStreamWaitEvent(b.stream, b.evIn, 0));
+  ctx->prof_begin_on(b.stream);
+  CTSIM_HIP(hipGraphLaunch(b.graph, b.stream));
+  ctx->prof_end_on(b.stream);
+  CTSIM_HIP(hipEventRecord(b.evOut, b.stream));
+  CTSIM_HIP(hipStreamWaitEvent(ctx->stream, b.evOut, 0));
+  return CTSIM_OK;
+}
+
+}  // namespace
+}  // namespace ctsim
+
+extern "C" {
+
+int ctsim_batch_create(ctsim_ctx* ctx, ctsim_batch** out, const ctsim_slice_desc* d, int nSlices) {
+  if (!ctx || !out || (!d && nSlices > 0)) return fail(CTSIM_ERR_INVALID, "ctsim_batch_create: NULL argument");
+  if (nSlices < 1) return fail(CTSIM_ERR_INVALID, "ctsim_batch_create: no slices");
+  *out = nullptr;
+  CTSIM_CHECK(ctx->activate());
+  auto* b = new ctsim_batch();
+  b->s.resize(nSlices);
+  int rc = CTSIM_OK;
+  int maxN = 0;
+  for (int k = 0; k < nSlices && rc == CTSIM_OK; k++) {
+    const ctsim_slice_desc& sd = d[k];
+    auto& sl = b->s[k];
+    if (!sd.angles) { rc = fail(CTSIM_ERR_INVALID, "ctsim_batch_create: slice without view angles"); break; }
+    rc = filter_plan_init(sl.fp, sd.filter, sd.method, 1.0 / sd.detInc, sd.detInc, sd.nDet, sd.filterParam,
+                          "spatial", sd.generation, sd.zeropad, 1, sd.geometry, sd.focalLength,
+                          sd.sourceDetectorLength);
+    if (rc == CTSIM_OK) rc = filter_upload(sl.fp);
+    if (rc == CTSIM_OK)
+      rc = bp_plan_init(sl.bp, sd.geometry, sd.nDet, sd.detInc, sd.nView, sd.viewDiameter,
+                        sd.focalLength, sd.sourceDetectorLength, sd.backproj, sd.interp, 1, sd.nx,
+                        sd.ny, nullptr);
+    if (rc == CTSIM_OK) rc = bp_upload_tables(sl.bp);
+    if (rc != CTSIM_OK) break;
+    if (sl.bp.interp == INTERP_NEAREST || sl.bp.kind == KIND_DIFF) {
+      rc = fail(CTSIM_ERR_UNSUPPORTED, "batch: nearest interpolation and Diff take the per-slice FP64 index path");
+      break;
+    }
+    const int n = sd.nDet;
+    if (sl.fp.method == METHOD_CONVOLUTION) {
+      // FFT convolution: circular length >= 2n-1 holds every lag of the
+      // reference's direct sum (procsignal.cpp:659-673) without wrap-around;
+      // the multipliers are the real part of the kernel's transform (the
+      // kernel is even up to the rounding of its sample positions)
+      int N = 1;
+      while (N < 2 * n - 1) N <<= 1;
+      sl.N = N;
+      std::vector<std::complex<double>> g(N, 0.0);
+      for (int lag = -(n - 1); lag <= n - 1; lag++)
+        g[(lag + N) % N] = sl.fp.inc * sl.fp.filter[lag + n - 1];
+      fft_host(g);
+      std::vector<float> mul(N / 2 + 1);
+      for (int kk = 0; kk <= N / 2; kk++) mul[kk] = static_cast<float>(g[kk].real() / N);
+      std::vector<float> tw(N);
+      for (int kk = 0; kk < N / 2; kk++) {
+        const double a = -2.0 * kPi * kk / N;
+        tw[2 * kk] = static_cast<float>(cos(a));
+        tw[2 * kk + 1] = static_cast<float>(sin(a));
+      }
+      if ((rc = sl.mul.reserve(mul.size() * sizeof(float))) != CTSIM_OK) break;
+      if ((rc = sl.tw.reserve(tw.size() * sizeof(float))) != CTSIM_OK) break;
+      if (hipMemcpy(sl.mul.ptr, mul.data(), mul.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(sl.tw.ptr, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fail(CTSIM_ERR_HIP, "batch: table upload failed");
+        break;
+      }
+    } else {
+      sl.N = sl.fp.nfp;
+      if ((sl.N & (sl.N - 1)) != 0) {
+        rc = fail(CTSIM_ERR_UNSUPPORTED, "batch: frequency filtering needs a power-of-two transform (zeropad >= 1)");
+        break;
+      }
+    }
+    while ((1 << sl.logN) < sl.N) sl.logN++;
+    if (sl.N > 4096) { rc = fail(CTSIM_ERR_UNSUPPORTED, "batch: transform longer than 4096"); break; }
+    maxN = std::max(maxN, sl.N);
+    sl.angles.assign(sd.angles, sd.angles + sd.nView);
+    std::vector<ViewConstF32> vc(sd.nView);
+    bp_view_consts_f32(sl.bp, sd.angles, sd.nView, vc.data());
+    if ((rc = sl.vc.reserve(vc.size() * sizeof(ViewConstF32))) != CTSIM_OK) break;
+    if (hipMemcpy(sl.vc.ptr, vc.data(), vc.size() * sizeof(ViewConstF32), hipMemcpyHostToDevice) != hipSuccess) {
+      rc = fail(CTSIM_ERR_HIP, "batch: view constant upload failed");
+      break;
+    }
+    const size_t elem = sl.bp.interp == INTERP_CUBIC ? 16 : 8;
+    if ((rc = sl.coef.reserve(static_cast<size_t>(sd.nView) * sd.nDet * elem)) != CTSIM_OK) break;
+    const double geomCost = sl.bp.geom == GEOM_PARALLEL ? 1.0 : (sl.bp.geom == GEOM_EQUIANGULAR ? 3.0 : 2.0);
+    const double pvu = static_cast<double>(sd.nx) * sd.ny * sd.nView;
+    sl.cost = pvu * geomCost * (sl.bp.interp == INTERP_CUBIC ? 1.5 : 1.0) +
+              40.0 * sd.nView * sl.N * sl.logN;
+    b->pvu += pvu;
+  }
+  if (rc == CTSIM_OK) {
+    b->ldsFilter = static_cast<int>(1.5 * maxN * sizeof(Cplx<float>));
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->evIn, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->evOut, hipEventDisableTiming) != hipSuccess)
+      rc = fail(CTSIM_ERR_HIP, "batch: stream or event creation failed");
+  }
+  if (rc == CTSIM_OK) rc = batch_layout(ctx, *b);
+  if (rc != CTSIM_OK) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return CTSIM_OK;
+}
+
+void ctsim_batch_destroy(ctsim_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  delete b;
+}
+
+int ctsim_batch_set_shape(ctsim_ctx* ctx, ctsim_batch* b, int P, int lanesY, int chunkBytes, int fused) {
+  if (!ctx || !b) return fail(CTSIM_ERR_INVALID, "ctsim_batch_set_shape: NULL argument");
+  if (!(P == 1 || P == 2 || P == 4) || !(lanesY == 8 || lanesY == 16 || lanesY == 32) ||
+      chunkBytes < 1024 || chunkBytes > 62 * 1024)
+    return fail(CTSIM_ERR_INVALID, "ctsim_batch_set_shape: shape out of range");
+  CTSIM_CHECK(ctx->activate());
+  CTSIM_HIP(hipStreamSynchronize(b->stream));
+  b->P = P;
+  b->lanesY = lanesY;
+  b->chunkBudget = chunkBytes;
+  b->fused = fused != 0;
+  return batch_layout(ctx, *b);
+}
+
+int ctsim_batch_run(ctsim_ctx* ctx, ctsim_batch* b, const float* const* d_sinos, float* const* d_images) {
+  if (!ctx || !b || !d_sinos || !d_images) return fail(CTSIM_ERR_INVALID, "ctsim_batch_run: NULL argument");
+  const int S = static_cast<int>(b->s.size());
+  for (int k = 0; k < S; k++)
+    if (!d_sinos[k] || !d_images[k]) return fail(CTSIM_ERR_INVALID, "ctsim_batch_run: NULL slice pointer");
+  CTSIM_CHECK(ctx->activate());
+  const bool same = static_cast<int>(b->sinos.size()) == S &&
+                    std::equal(b->sinos.begin(), b->sinos.end(), d_sinos) &&
+                    std::equal(b->images.begin(), b->images.end(), d_images);
+  if (!same) {
+    CTSIM_HIP(hipStreamSynchronize(b->stream));
+    b->sinos.assign(d_sinos, d_sinos + S);
+    b->images.assign(d_images, d_images + S);
+    for (int k = 0; k < S; k++) {
+      b->host[k].sino = d_sinos[k];
+      b->host[k].bp.image = d_images[k];
+    }
+    CTSIM_HIP(hipMemcpy(b->d_slices.ptr, b->host.data(), S * sizeof(SliceDev), hipMemcpyHostToDevice));
+  }
+  return batch_enqueue(ctx, *b);
+}
+
+int ctsim_batch_status(ctsim_ctx* ctx, ctsim_batch* b) {
+  if (!ctx || !b) return fail(CTSIM_ERR_INVALID, "ctsim_batch_status: NULL argument");
+  CTSIM_CHECK(ctx->activate());
+  CTSIM_HIP(hipStreamSynchronize(b->stream));
+  int flag = 0;
+  CTSIM_HIP(hipMemcpy(&flag, b->d_error.ptr, sizeof(int), hipMemcpyDeviceToHost));
+  if (flag) return fail(CTSIM_ERR_HIP, "batch: a backprojection item timed out waiting for its filter items");
+  return CTSIM_OK;
+}
+
+int ctsim_batch_autotune(ctsim_ctx* ctx, ctsim_batch* b, const float* const* d_sinos,
+                         float* const* d_images, int* shape4, double* bestMs) {
+  if (!ctx || !b) return fail(CTSIM_ERR_INVALID, "ctsim_batch_autotune: NULL argument");
+  CTSIM_CHECK(ctx->activate());
+  hipEvent_t e0, e1;
+  CTSIM_HIP(hipEventCreate(&e0));
+  CTSIM_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bP = b->P, bL = b->lanesY, bC = b->chunkBudget, bF = b->fused;
+  int rc = CTSIM_OK;
+  for (int P : {2, 4})
+    for (int lanesY : {8, 16, 32})
+      for (int chunk : {16 * 1024, 32 * 1024})
+        for (int fused : {1, 0}) {
+          if (rc != CTSIM_OK) break;
+          if ((rc = ctsim_batch_set_shape(ctx, b, P, lanesY, chunk, fused)) != CTSIM_OK) {
+            if (rc == CTSIM_ERR_UNSUPPORTED) { rc = CTSIM_OK; continue; }
+            break;
+          }
+          float tot = 0;
+          for (int rep = 0; rep < 3 && rc == CTSIM_OK; rep++) {
+            if ((rc = ctsim_batch_run(ctx, b, d_sinos, d_images)) != CTSIM_OK) break;
+            if (rep == 0) {
+              if ((rc = ctsim_batch_status(ctx, b)) != CTSIM_OK) break;
+              continue;
+            }
+            (void)hipEventRecord(e0, ctx->stream);
+            if ((rc = ctsim_batch_run(ctx, b, d_sinos, d_images)) != CTSIM_OK) break;
+            (void)hipEventRecord(e1, ctx->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            tot += ms;
+          }
+          if (rc == CTSIM_OK && tot / 2 < best) {
+            best = tot / 2;
+            bP = P; bL = lanesY; bC = chunk; bF = fused;
+          }
+        }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != CTSIM_OK) return rc;
+  CTSIM_CHECK(ctsim_batch_set_shape(ctx, b, bP, bL, bC, bF));
+  if (shape4) { shape4[0] = bP; shape4[1] = bL; shape4[2] = bC; shape4[3] = bF; }
+  if (bestMs) *bestMs = best;
+  return CTSIM_OK;
+}
+
+int ctsim_batch_info(const ctsim_batch* b, double* out4) {
+  if (!b || !out4) return fail(CTSIM_ERR_INVALID, "ctsim_batch_info: NULL argument");
+  out4[0] = static_cast<double>(b->s.size());
+  out4[1] = b->pvu;
+  out4[2] = b->fused ? 1 : 0;
+  out4[3] = b->gridAll;
+  return CTSIM_OK;
+}
+
+}  // extern "C"

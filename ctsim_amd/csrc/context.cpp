@@ -80,7 +80,7 @@ int ctsim_ctx::activate() const {
   return CTSIM_OK;
 }
 
-void ctsim_ctx::prof_begin() {
+void ctsim_ctx::prof_begin_on(hipStream_t s) {
   if (!profOn) return;
   if (profEvents.size() < 2 * (profUsed + 1)) {
     hipEvent_t a = nullptr, b = nullptr;
@@ -88,12 +88,12 @@ void ctsim_ctx::prof_begin() {
     profEvents.push_back(a);
     profEvents.push_back(b);
   }
-  (void)hipEventRecord(profEvents[2 * profUsed], stream);
+  (void)hipEventRecord(profEvents[2 * profUsed], s);
 }
 
-void ctsim_ctx::prof_end() {
+void ctsim_ctx::prof_end_on(hipStream_t s) {
   if (!profOn || profEvents.size() < 2 * (profUsed + 1)) return;
-  (void)hipEventRecord(profEvents[2 * profUsed + 1], stream);
+  (void)hipEventRecord(profEvents[2 * profUsed + 1], s);
   profUsed++;
 }
 

@@ -85,7 +85,9 @@ struct ctsim_ctx {
   bool profOn = false;
   std::vector<hipEvent_t> profEvents;  // start, stop pairs
   size_t profUsed = 0;                 // pairs recorded since the last read
-  void prof_begin();
-  void prof_end();
+  void prof_begin() { prof_begin_on(stream); }
+  void prof_end() { prof_end_on(stream); }
+  void prof_begin_on(hipStream_t s);
+  void prof_end_on(hipStream_t s);
   ~ctsim_ctx();
 };

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include "filter_plan.h"
+#include "fft_device.h"
 
 #pragma clang fp contract(off)
 
@@ -177,53 +178,7 @@ __global__ __launch_bounds__(kLanes * kWaves) void dft_inv_kernel(
 }
 
 // ------------------------------------------------------------ LDS FFT
-template <class T>
-struct Cplx {
-  T x, y;
-};
-
-template <class T>
-__device__ __forceinline__ Cplx<T> cmul(Cplx<T> a, Cplx<T> b) {
-  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
-}
-
-// In-place radix-2 Stockham pass over z[0..N) held in LDS.  tw[k] =
-// exp(-2 pi i k / N) for k < N/2; dir = -1 forward, +1 inverse (conjugate
-// twiddles).  Every lane first reads its butterflies, then the block
-// synchronises, then writes: the autosort permutation makes the passes
-// write-after-read hazards otherwise.
-template <class T, int R>
-__device__ void stockham(Cplx<T>* z, const Cplx<T>* tw, int N, int logN, int dir) {
-  const int half = N >> 1;
-  for (int s = 0, Ls = 1; s < logN; s++, Ls <<= 1) {
-    Cplx<T> ya[R], yb[R];
-    int idx[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-      const int t = threadIdx.x + r * blockDim.x;
-      idx[r] = -1;
-      if (t < half) {
-        const int k = t & (Ls - 1);
-        Cplx<T> a = z[t], b = z[t + half];
-        Cplx<T> w = tw[k * (half / Ls)];
-        if (dir > 0) w.y = -w.y;
-        const Cplx<T> wb = cmul(w, b);
-        ya[r] = {a.x + wb.x, a.y + wb.y};
-        yb[r] = {a.x - wb.x, a.y - wb.y};
-        idx[r] = ((t - k) << 1) + k;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-      if (idx[r] >= 0) {
-        z[idx[r]] = ya[r];
-        z[idx[r] + Ls] = yb[r];
-      }
-    }
-    __syncthreads();
-  }
-}
+using dev::Cplx;
 
 template <class T, class TO, int R>
 __global__ __launch_bounds__(256) void fft_filter_kernel(
@@ -233,47 +188,12 @@ __global__ __launch_bounds__(256) void fft_filter_kernel(
   extern __shared__ unsigned char smem_raw[];
   Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw);
   Cplx<T>* tw = z + N;
-  const int va = 2 * blockIdx.x, vb = va + 1;
-  const bool hasB = vb < nView;
-  for (int k = threadIdx.x; k < N / 2; k += blockDim.x) tw[k] = twg[k];
-  for (int j = threadIdx.x; j < N; j += blockDim.x) {
-    T a = 0, b = 0;
-    if (j < n) {
-      double xa = static_cast<double>(in[static_cast<size_t>(va) * n + j]);
-      double xb = hasB ? static_cast<double>(in[static_cast<size_t>(vb) * n + j]) : 0.0;
-      if (w) {
-        xa *= w[j];
-        xb *= w[j];
-      }
-      a = static_cast<T>(xa);
-      b = static_cast<T>(xb);
-    }
-    z[j] = {a, b};
-  }
-  __syncthreads();
-  stockham<T, R>(z, tw, N, logN, -1);
-  // Split the two real spectra, apply the multipliers, recombine.
-  for (int k = threadIdx.x; k <= N / 2; k += blockDim.x) {
-    const int kn = (N - k) & (N - 1);
-    const Cplx<T> zk = z[k], zn = z[kn];
-    // X_a = (Z_k + conj Z_{N-k}) / 2, X_b = (Z_k - conj Z_{N-k}) / (2i)
-    const T ar = (zk.x + zn.x) * T(0.5), ai = (zk.y - zn.y) * T(0.5);
-    const T br = (zk.y + zn.y) * T(0.5), bi = (zn.x - zk.x) * T(0.5);
-    const T hr = hre[k], hi = him[k];
-    const T yar = hr * ar, yai = hi * ai, ybr = hr * br, ybi = hi * bi;
-    // W_k = Y_a + i Y_b, W_{N-k} = conj(Y_a) + i conj(Y_b)
-    const Cplx<T> wk = {yar - ybi, yai + ybr};
-    const Cplx<T> wn = {yar + ybi, ybr - yai};
-    // pairs (k, N-k) for k <= N/2 are disjoint across lanes: no barrier needed
-    z[k] = wk;
-    if (kn != k) z[kn] = wn;
-  }
-  __syncthreads();
-  stockham<T, R>(z, tw, N, logN, +1);
+  const int va = 2 * blockIdx.x, vb = va + 1 < nView ? va + 1 : -1;
+  dev::fft_filter_pair<T, R>(in, w, hre, him, twg, va, vb, n, N, logN, z, tw);
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
     const Cplx<T> r = z[j];
     out[static_cast<size_t>(va) * outStride + j] = static_cast<TO>(r.x);
-    if (hasB) out[static_cast<size_t>(vb) * outStride + j] = static_cast<TO>(r.y);
+    if (vb >= 0) out[static_cast<size_t>(vb) * outStride + j] = static_cast<TO>(r.y);
   }
 }
 

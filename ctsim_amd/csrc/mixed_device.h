@@ -1,0 +1,265 @@
+// Device code of the mixed-precision path shared by the single-slice
+// kernels (backproject.hip) and the batched persistent kernel (batch.hip):
+// interval coefficients of the reference's interpolants, the fan-beam
+// position and weight in FP32, and the tiled backprojection body.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "bp_plan.h"
+
+namespace ctsim {
+namespace dev {
+
+constexpr int kMaxChunk = 64;
+
+// Remap block ids so that neighbouring tiles run on the same XCD.  The
+// dispatcher deals workgroups round-robin over the 8 XCDs, so XCD k receives
+// block ids k, k+8, ...; give it the contiguous tile range
+// [k*q + min(k, r), ...) where n = 8q + r (a bijection on [0, n)).
+__device__ __forceinline__ int xcd_tile(int bid, int n) {
+  const int q = n >> 3, r = n & 7;
+  const int k = bid & 7, local = bid >> 3;
+  return k * q + min(k, r) + local;
+}
+
+#pragma clang fp contract(fast)
+
+// The mixed path interpolates from per-interval coefficients: on detector
+// interval i (position p in [i, i+1), u = p - i) the reference's interpolant
+// is the polynomial a0 + u (a1 + u (a2 + u a3)), zero outside the detector,
+// with its end rules folded in (CubicPolyInterpolator::interpolate,
+// interpolator.cpp:36-75; linear with the IntDiff last-cell rule,
+// backprojectors.cpp:635-638).  coef_kernel writes them once per view, in
+// FP64 rounded to FP32, so the backprojection inner loop is a window lookup
+// and three FMAs.
+template <int INTERP>
+using CoefT = typename std::conditional<INTERP == INTERP_CUBIC, float4, float2>::type;
+
+// Coefficients of detector interval i of one filtered row (n cells), read
+// through row(j) (any index the rules below use is in [0, n)).
+template <int INTERP, bool LAST_INCL, class Row>
+__device__ __forceinline__ CoefT<INTERP> interval_coef(const Row& row, int i, int n) {
+  if constexpr (INTERP == INTERP_CUBIC) {
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    if (i == 0) {
+      if (n >= 2) { a0 = row(0); a1 = static_cast<double>(row(1)) - row(0); }
+    } else if (i + 2 > n) {
+      // beyond the last full interval: zero
+    } else if (i + 2 == n) {
+      a0 = row(n - 2);
+      a1 = static_cast<double>(row(n - 1)) - row(n - 2);
+    } else {
+      const double y0 = row(i - 1), y1 = row(i), y2 = row(i + 1), y3 = row(i + 2);
+      a0 = y1;
+      a1 = -y0 / 3 - y1 / 2 + y2 - y3 / 6;
+      a2 = (y0 + y2) / 2 - y1;
+      a3 = (y3 - y0) / 6 + (y1 - y2) / 2;
+    }
+    return make_float4(static_cast<float>(a0), static_cast<float>(a1), static_cast<float>(a2),
+                       static_cast<float>(a3));
+  } else {
+    const int hiOk = LAST_INCL ? n - 1 : n - 2;
+    float a0 = 0.f, a1 = 0.f;
+    if (i <= hiOk) {
+      a0 = row(i);
+      a1 = i < n - 1 ? static_cast<float>(static_cast<double>(row(i + 1)) - row(i)) : 0.f;
+    }
+    return make_float2(a0, a1);
+  }
+}
+
+struct MixedArgs {
+  int nx, ny, nDet, nView, lanesY, tilesX, tilesY, W, chunk, split, viewsPerGroup;
+  float center, F, twoF, posScale, endScale;
+  float atan[9];
+  const float* xs;
+  const float* ys;
+  const ViewConstF32* vc;
+  const void* coef;
+  float* image;     // split == 1: image += endScale * sum (overwrite: image = endScale * sum)
+  float* partial;   // split > 1: partial[group][pixel] = sum
+  int overwrite;
+};
+
+// atan on the whole line for FP32: odd polynomial t*P(t^2) on [0,1]
+// (weighted least-squares fit of atan(t)/t in t^2, max abs error 1.3e-7),
+// with atan(t) = pi/2 - atan(1/t) beyond 1.  Used when the fan is wider than
+// the fitted range.
+__device__ __forceinline__ float atan_f32(float t) {
+  const float at = fabsf(t);
+  const bool inv = at > 1.f;
+  const float u = inv ? __builtin_amdgcn_rcpf(at) : at;
+  const float z = u * u;
+  float p = 0.0028499013278633356f;
+  p = p * z - 0.01606869138777256f;
+  p = p * z + 0.042691655457019806f;
+  p = p * z - 0.07504309713840485f;
+  p = p * z + 0.10640943795442581f;
+  p = p * z - 0.14203648269176483f;
+  p = p * z + 0.1999261975288391f;
+  p = p * z - 0.3333307206630707f;
+  p = p * z + 1.0f;
+  float r = u * p;
+  r = inv ? 1.5707963267948966f - r : r;
+  return copysignf(r, t);
+}
+
+// Detector offset (before the centre) of the equiangular geometry:
+// atan(t) / detInc with t = tan(gamma).
+template <int DEG>
+__device__ __forceinline__ float fan_angle(const float* c, float t) {
+  if constexpr (DEG > 0) {
+    const float z = t * t;
+    float p = c[DEG];
+#pragma unroll
+    for (int k = DEG - 1; k >= 0; k--) p = fmaf(p, z, c[k]);
+    return t * p;
+  } else {
+    return atan_f32(t) * c[0];
+  }
+}
+
+// Position of pixel (x, y) in one view (window-relative when cshift is the
+// centre minus the window start) and its weight.  Parallel: ca, sa carry
+// 1/detInc.  Fan: ca, sa = cos, sin of the view angle.
+template <int GK, int DEG>
+__device__ __forceinline__ void pos_mixed(const MixedArgs& a, float ca, float sa, float cshift,
+                                          float x, float y, float r2mF2, float& pos, float& wgt) {
+  if constexpr (GK == 0) {
+    pos = fmaf(y, sa, fmaf(x, ca, cshift));
+    wgt = 1.f;
+  } else {
+    const float rc = fmaf(y, sa, x * ca);             // r cos(beta - phi)
+    const float fl = fmaf(x, sa, fmaf(-y, ca, a.F));  // F + r sin(beta - phi)
+    if constexpr (GK == 1) {
+      // one reciprocal for both 1/fl and 1/L^2, L^2 = fl^2 + rc^2 = 2 F fl + r^2 - F^2
+      const float L2 = fmaf(a.twoF, fl, r2mF2);
+      const float rr = __builtin_amdgcn_rcpf(fl * L2);
+      pos = fan_angle<DEG>(a.atan, rc * (L2 * rr)) + cshift;
+      wgt = fl * rr;
+    } else {
+      const float q = __builtin_amdgcn_rcpf(fl);
+      pos = fmaf(rc * q, a.posScale, cshift);
+      const float u = a.F * q;
+      wgt = u * u;
+    }
+  }
+}
+
+// One workgroup's work item: a tile of (256 / lanesY) columns x
+// (lanesY * P) rows and one group of views.  Lane (cx, ly) owns column ix0 + cx and rows
+// iy0 + P*ly .. + P-1 (contiguous, so the final read-modify-write is a
+// vector access).  Per chunk of views: lanes < nv compute each view's window
+// start from the tile corners, the workgroup stages the window's interval
+// coefficients in LDS, then every lane walks the chunk's views for its P
+// pixels, summing the chunk in FP32 and folding the chunk sum into a
+// Kahan-compensated total.
+template <int GK, int INTERP, int P, int DEG>
+__device__ __forceinline__ void bp_mixed_tile(const MixedArgs& a, int tile, int grp,
+                                              CoefT<INTERP>* s_w, float4* s_v, int* s_lo) {
+  using CT = CoefT<INTERP>;
+  const int t = threadIdx.x;
+  const int tx = 256 / a.lanesY, ty = a.lanesY * P;
+  const int ix0 = (tile / a.tilesY) * tx, iy0 = (tile % a.tilesY) * ty;
+  const int ix = ix0 + t / a.lanesY, iyb = iy0 + (t % a.lanesY) * P;
+  const int cx1 = min(ix0 + tx - 1, a.nx - 1), cy1 = min(iy0 + ty - 1, a.ny - 1);
+  const int W = a.W;
+  const float x = a.xs[min(ix, a.nx - 1)];
+  float y[P], r2mF2[P];
+#pragma unroll
+  for (int k = 0; k < P; k++) {
+    y[k] = a.ys[min(iyb + k, a.ny - 1)];
+    r2mF2[k] = fmaf(x, x, fmaf(y[k], y[k], -a.F * a.F));
+  }
+  const int vBegin = grp * a.viewsPerGroup;
+  const int vEnd = min(a.nView, vBegin + a.viewsPerGroup);
+  float sum[P], comp[P];
+#pragma unroll
+  for (int k = 0; k < P; k++) sum[k] = comp[k] = 0.f;
+  const CT* coef = static_cast<const CT*>(a.coef);
+
+  for (int v0 = vBegin; v0 < vEnd; v0 += a.chunk) {
+    const int nv = min(a.chunk, vEnd - v0);
+    if (t < nv) {
+      const ViewConstF32 c = a.vc[v0 + t];
+      int lo = -2;
+      if (a.W < a.nDet + 4) {
+        float p0, p1, p2, p3, wd;
+        const float xa = a.xs[ix0], xb = a.xs[cx1], ya = a.ys[iy0], yb = a.ys[cy1];
+        pos_mixed<GK, DEG>(a, c.ca, c.sa, a.center, xa, ya, fmaf(xa, xa, fmaf(ya, ya, -a.F * a.F)), p0, wd);
+        pos_mixed<GK, DEG>(a, c.ca, c.sa, a.center, xb, ya, fmaf(xb, xb, fmaf(ya, ya, -a.F * a.F)), p1, wd);
+        pos_mixed<GK, DEG>(a, c.ca, c.sa, a.center, xa, yb, fmaf(xa, xa, fmaf(yb, yb, -a.F * a.F)), p2, wd);
+        pos_mixed<GK, DEG>(a, c.ca, c.sa, a.center, xb, yb, fmaf(xb, xb, fmaf(yb, yb, -a.F * a.F)), p3, wd);
+        float pmin = fminf(fminf(p0, p1), fminf(p2, p3));
+        if (!(pmin > -1e9f && pmin < 1e9f)) pmin = 0.f;
+        lo = static_cast<int>(floorf(pmin)) - 2;
+      }
+      s_lo[t] = lo;
+      s_v[t] = make_float4(c.ca, c.sa, a.center - static_cast<float>(lo), 0.f);
+    }
+    __syncthreads();
+    for (int vv = 0; vv < nv; vv++) {
+      const int lo = s_lo[vv];
+      const CT* crow = coef + static_cast<size_t>(v0 + vv) * a.nDet;
+      for (int j = t; j < W; j += 256) {
+        const int d = lo + j;
+        CT val;
+        if (d >= 0 && d < a.nDet) val = crow[d];
+        else if constexpr (INTERP == INTERP_CUBIC) val = make_float4(0.f, 0.f, 0.f, 0.f);
+        else val = make_float2(0.f, 0.f);
+        s_w[vv * W + j] = val;
+      }
+    }
+    __syncthreads();
+    float part[P];
+#pragma unroll
+    for (int k = 0; k < P; k++) part[k] = 0.f;
+    for (int vv = 0; vv < nv; vv++) {
+      const float4 c = s_v[vv];
+      const CT* w = s_w + vv * W;
+#pragma unroll
+      for (int k = 0; k < P; k++) {
+        float pos, wgt;
+        pos_mixed<GK, DEG>(a, c.x, c.y, c.z, x, y[k], r2mF2[k], pos, wgt);
+        const float pf = floorf(pos);
+        const float u = pos - pf;
+        const int j = min(max(static_cast<int>(pf), 0), W - 1);
+        const CT q = w[j];
+        float val;
+        if constexpr (INTERP == INTERP_CUBIC) val = fmaf(u, fmaf(u, fmaf(u, q.w, q.z), q.y), q.x);
+        else val = fmaf(u, q.y, q.x);
+        part[k] = fmaf(val, wgt, part[k]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < P; k++) {  // Kahan step with the chunk's sum
+      const float yk = part[k] - comp[k];
+      const float tk = sum[k] + yk;
+      comp[k] = (tk - sum[k]) - yk;
+      sum[k] = tk;
+    }
+    __syncthreads();
+  }
+  if (ix >= a.nx) return;
+  const size_t base = static_cast<size_t>(ix) * a.ny + iyb;
+  if (a.split == 1) {
+#pragma unroll
+    for (int k = 0; k < P; k++)
+      if (iyb + k < a.ny)
+        a.image[base + k] = a.overwrite ? a.endScale * (sum[k] - comp[k])
+                                        : fmaf(a.endScale, sum[k] - comp[k], a.image[base + k]);
+  } else {
+    float* out = a.partial + static_cast<size_t>(grp) * a.nx * a.ny;
+#pragma unroll
+    for (int k = 0; k < P; k++)
+      if (iyb + k < a.ny) out[base + k] = sum[k] - comp[k];
+  }
+}
+
+#pragma clang fp contract(off)
+
+}  // namespace dev
+}  // namespace ctsim

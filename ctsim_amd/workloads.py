@@ -122,3 +122,53 @@ def config4(n_slices: int = 256):
                              filter="abs_bandlimit", method="convolution", interp="linear",
                              backproj="idiff", nx=nx, ny=nx))
     return out
+
+
+@dataclass
+class HelicalSpec:
+    """Config 5: a time-varying phantom scanned over several rotations by an
+    equilinear fan beam, then one planar half-scan sinogram per slice position
+    by 180LI helical interpolation (the pjHinterp chain, tools/pjHinterp.cpp:
+    131-146) and a batched FBP of all positions."""
+    n_ellipses: int = 32
+    drift: float = 0.05          # amplitude of the centre and axis drift
+    nDet: int = 888
+    viewsPerRotation: int = 1152
+    positions: int = 64          # slice positions, one every viewsPerRotation/positions views
+    geometry: str = "equilinear"
+    focalRatio: float = 2.0
+    centerRatio: float = 2.0
+    filter: str = "abs_bandlimit"
+    method: str = "convolution"
+    interp: str = "linear"
+    backproj: str = "idiff"
+    nx: int = 512
+    ny: int = 512
+    seed: int = field(default_factory=lambda: config_seed(5))
+
+    def base(self):
+        """Elements at time 0 plus per-element drift phases."""
+        elems = seeded_elements(self.n_ellipses, self.seed)
+        rng = np.random.Generator(np.random.PCG64(self.seed + 1))
+        phases = rng.uniform(0, 2 * np.pi, size=(len(elems), 4))
+        return elems, phases
+
+    def elements(self, view: int, base=None):
+        """The phantom at acquisition view ``view`` (one rotation per period):
+        centre and semi-axes drift sinusoidally by ``drift``."""
+        elems, phases = base if base is not None else self.base()
+        t = 2 * np.pi * view / self.viewsPerRotation
+        out = []
+        for (kind, cx, cy, u, v, rot, att), ph in zip(elems, phases):
+            a = self.drift
+            out.append((kind, cx + a * np.sin(t + ph[0]), cy + a * np.sin(t + ph[1]),
+                        max(0.01, u + a * np.sin(t + ph[2])), max(0.01, v + a * np.sin(t + ph[3])),
+                        rot, att))
+        return out
+
+    def stride(self) -> int:
+        return self.viewsPerRotation // self.positions
+
+
+def config5() -> HelicalSpec:
+    return HelicalSpec()

@@ -49,6 +49,8 @@ typedef struct ctsim_phantom ctsim_phantom;
 typedef struct ctsim_scanner ctsim_scanner;
 typedef struct ctsim_procsignal ctsim_procsignal;
 typedef struct ctsim_backprojector ctsim_backprojector;
+typedef struct ctsim_batch ctsim_batch;
+typedef struct ctsim_rebin ctsim_rebin;
 
 /* ------------------------------------------------------------- context */
 
@@ -186,6 +188,61 @@ int ctsim_collect_projections(ctsim_ctx* ctx, const ctsim_scanner* sc, ctsim_pha
                               int precision, int iStartView, int iNumViews, int iOffsetView,
                               int iStorageOffset, float* d_sino, double* h_angles);
 
+/* ------------------------------------------------------ batched slices */
+
+/* One slice of a batch: the Projections header fields a Reconstructor reads
+ * (reconstruct.cpp:50-103), the reconstruction options (pjrec's, with
+ * interpolation factor 1) and the view angles (host, nView doubles; read at
+ * ctsim_batch_create). */
+typedef struct ctsim_slice_desc {
+  int geometry; /* 0 parallel, 1 equiangular, 2 equilinear */
+  int nDet;
+  int nView;
+  double detInc;
+  double viewDiameter;
+  double focalLength;
+  double sourceDetectorLength;
+  const char* filter;
+  const char* method;
+  double filterParam;
+  const char* generation;
+  int zeropad;
+  const char* backproj;
+  const char* interp; /* "linear" or "cubic" */
+  int nx;
+  int ny;
+  const double* angles;
+} ctsim_slice_desc;
+
+/* No single counterpart in the reference (it reconstructs one slice per
+ * pjrec run, tools/pjrec.cpp:342-347): a batch plan for the MIXED path of
+ * nSlices slices of any geometry and size.  Filtering and backprojection of
+ * every slice run in one persistent kernel (or a filter launch and a
+ * backprojection launch) captured in a HIP graph.  Convolution filters run as
+ * FFT convolutions in the batch. */
+int ctsim_batch_create(ctsim_ctx* ctx, ctsim_batch** out, const ctsim_slice_desc* slices,
+                       int nSlices);
+void ctsim_batch_destroy(ctsim_batch* b);
+/* Reconstructs every slice: d_sinos[k] float [nView][nDet] (device) into
+ * d_images[k] float [nx][ny] (device, overwritten, post-scaled as after
+ * PostProcessing).  Host arrays of device pointers; asynchronous on the
+ * context's stream. */
+int ctsim_batch_run(ctsim_ctx* ctx, ctsim_batch* b, const float* const* d_sinos,
+                    float* const* d_images);
+/* Waits for the batch and reports a backprojection item that gave up
+ * waiting for its slice's filtering (never expected). */
+int ctsim_batch_status(ctsim_ctx* ctx, ctsim_batch* b);
+/* Launch shape: P rows per lane (1, 2, 4), lanesY (8, 16, 32), LDS bytes per
+ * view chunk, fused (1) or two-phase (0). */
+int ctsim_batch_set_shape(ctsim_ctx* ctx, ctsim_batch* b, int P, int lanesY, int chunkBytes,
+                          int fused);
+/* Times candidate shapes on these inputs and keeps the fastest; shape4 gets
+ * P, lanesY, chunkBytes, fused. */
+int ctsim_batch_autotune(ctsim_ctx* ctx, ctsim_batch* b, const float* const* d_sinos,
+                         float* const* d_images, int* shape4, double* bestMs);
+/* out4: slices, pixel-view updates per run, fused, persistent grid size. */
+int ctsim_batch_info(const ctsim_batch* b, double* out4);
+
 /* ---------------------------------------------------------------- helical */
 
 /* Replaces Projections::Helical180LI (projections.cpp:200-358) on the device:
@@ -206,6 +263,24 @@ int ctsim_helical180li(ctsim_ctx* ctx, int geometry, int nDet, int nView, double
 int ctsim_halfscan_feather(ctsim_ctx* ctx, int geometry, int nDet, int nView, double rotInc,
                            double detInc, double fanBeamAngle, double sourceDetectorLength,
                            const float* d_in, float* d_out);
+
+/* -------------------------------------------------------------- rebinning */
+
+/* Replaces Projections::interpolateToParallel (projections.cpp:1250-1328,
+ * ParallelRaysums 1339-1469) for a fan-beam sinogram (geometry 1 or 2) with
+ * the Projections header fields given and view angles h_angles (host): the
+ * plan sorts the (theta, t) coordinates of every sample once; runs rebin
+ * float [nView][nDet] rows on the device.  Exact (t, theta) ties, which the
+ * reference's unstable sort leaves unordered, keep view-major order. */
+int ctsim_rebin_create(ctsim_ctx* ctx, ctsim_rebin** out, int geometry, int nDet, int nView,
+                       double detStart, double detInc, double focalLength,
+                       double sourceDetectorLength, double viewDiameter, double rotStart,
+                       const double* h_angles);
+void ctsim_rebin_destroy(ctsim_rebin* r);
+/* out4: rotStart (0), rotInc, detStart, detInc of the parallel projections;
+ * h_newAngles (optional, nView doubles): their view angles. */
+int ctsim_rebin_info(const ctsim_rebin* r, double* out4, double* h_newAngles);
+int ctsim_rebin_run(ctsim_ctx* ctx, ctsim_rebin* r, const float* d_in, float* d_out);
 
 /* ------------------------------------------------------------- statistics */
 

@@ -412,3 +412,44 @@ def ref_helical(builtin: str, geometry: str, nDet: int, nView: int, nSample: int
     if n < 0:
         raise RuntimeError(f"ref_helical failed ({n}): {L.ref_last_error().decode()}")
     return out[:n].copy(), ang[:n].copy(), tuple(info)
+
+
+def interpolate_to_parallel(geom: int, data: np.ndarray, detStart: float, detInc: float,
+                            focalLength: float, sdl: float, viewDiameter: float,
+                            rotStart: float, angles: np.ndarray):
+    """Oracle Projections::interpolateToParallel; returns (views, angles, info6)."""
+    L = lib("oracle")
+    f = L.orc_interpolate_to_parallel
+    d, i, vp = C.c_double, C.c_int, C.c_void_p
+    f.restype = i
+    f.argtypes = [i, i, i, d, d, d, d, d, d, vp, vp, vp, vp, vp]
+    data = np.ascontiguousarray(data, np.float32)
+    nView, nDet = data.shape
+    angles = np.ascontiguousarray(angles, np.float64)
+    out = np.zeros_like(data)
+    ang = np.zeros(nView, np.float64)
+    info = np.zeros(6, np.float64)
+    f(geom, nDet, nView, detStart, detInc, focalLength, sdl, viewDiameter, rotStart,
+      _ptr(angles), _ptr(data), _ptr(out), _ptr(ang), _ptr(info))
+    return out, ang, info
+
+
+def ref_rebin(builtin: str, geometry: str, nDet: int, nView: int, rotLen: float,
+              focalRatio: float, centerRatio: float, data: np.ndarray):
+    """The reference's interpolateToParallel; returns (views, angles_in,
+    angles_out, info8)."""
+    L = lib("ref")
+    f = L.ref_rebin
+    c, i, d, vp = C.c_char_p, C.c_int, C.c_double, C.c_void_p
+    f.restype = i
+    f.argtypes = [c, c, i, i, d, d, d, vp, vp, vp, vp, vp]
+    data = np.ascontiguousarray(data, np.float32)
+    out = np.zeros_like(data)
+    ain = np.zeros(nView, np.float64)
+    aout = np.zeros(nView, np.float64)
+    info = np.zeros(8, np.float64)
+    rc = f(_b(builtin), _b(geometry), nDet, nView, rotLen, focalRatio, centerRatio, _ptr(data),
+           _ptr(out), _ptr(ain), _ptr(aout), _ptr(info))
+    if rc:
+        raise RuntimeError(f"ref_rebin failed: {L.ref_last_error().decode()}")
+    return out, ain, aout, info

@@ -1754,3 +1754,112 @@ int orc_halfscan_feather(int geom, int nDet, int nView, double rotInc, double de
   for (size_t k = 0; k < (size_t)nView * nDet; k++) data[k] *= scalefactor;
   return 0;
 }
+
+/* -------------------------------------------------------------- rebinning */
+
+typedef struct {
+  double t, theta, raysum;
+  int index;
+} RebinCoord;
+
+/* ParallelRaysumCoordinate::compareByT (include/projections.h:50-52), with
+ * the original (view-major) position breaking exact (t, theta) ties */
+static int cmp_by_t(const void* pa, const void* pb) {
+  const RebinCoord* a = (const RebinCoord*)pa;
+  const RebinCoord* b = (const RebinCoord*)pb;
+  if (a->t != b->t) return a->t < b->t ? -1 : 1;
+  if (a->theta != b->theta) return a->theta < b->theta ? -1 : 1;
+  return a->index - b->index;
+}
+
+/* LinearInterpolator<double>::interpolate with abscissae, zero outside off
+ * (include/interpolator.h:203-262); the search hint does not change the
+ * result for increasing abscissae and is not kept. */
+static double lin_interp_x(const double* X, const double* Y, int n, double dX) {
+  int iLower = -1, iUpper = n;
+  while (iUpper - iLower > 1) {
+    const int iMiddle = (iUpper + iLower) >> 1;
+    if (dX >= X[iMiddle]) iLower = iMiddle;
+    else iUpper = iMiddle;
+  }
+  if (dX == X[0]) return Y[0];
+  if (dX < X[0]) return Y[0];
+  if (dX == X[n - 1]) return Y[n - 1];
+  if (dX > X[n - 1]) return Y[n - 1];
+  return Y[iLower] + (Y[iUpper] - Y[iLower]) * ((dX - X[iLower]) / (X[iUpper] - X[iLower]));
+}
+
+/* Projections::interpolateToParallel, libctsim/projections.cpp:1250-1328,
+ * with ParallelRaysums (1339-1469, theta normalised to [0, 2 pi)).
+ * in/out [nView][nDet] float; info6 receives the new rotStart(0), rotInc,
+ * detStart, detInc and the first and last new view angles. */
+int orc_interpolate_to_parallel(int geom, int nDet, int nView, double detStart, double detInc,
+                                double focalLength, double sdl, double viewDiameter,
+                                double rotStart, const double* angles, const float* in,
+                                float* out, double* anglesOut, double* info6) {
+  if (geom == 0) {
+    memcpy(out, in, sizeof(float) * (size_t)nDet * nView);
+    memcpy(anglesOut, angles, sizeof(double) * nView);
+    return 0;
+  }
+  const double newRotInc = TWOPI / nView;
+  const double newDetStart = -viewDiameter / 2;
+  double newDetInc = viewDiameter / nDet;
+  if (nDet % 2 == 0) newDetInc = viewDiameter / (nDet - 1);
+  const size_t nc = (size_t)nView * nDet;
+  RebinCoord* c = (RebinCoord*)malloc(nc * sizeof(RebinCoord));
+  size_t k = 0;
+  for (int iV = 0; iV < nView; iV++) {
+    const double dViewAngle = angles[iV];
+    double dDetPos = detStart;
+    for (int iD = 0; iD < nDet; iD++, k++) {
+      if (geom == 2) {
+        const double dFanAngle = atan(dDetPos / sdl);
+        c[k].theta = dViewAngle + dFanAngle;
+        c[k].t = focalLength * sin(dFanAngle);
+      } else {
+        c[k].theta = dViewAngle + dDetPos;
+        c[k].t = focalLength * sin(dDetPos);
+      }
+      c[k].theta = normalize_angle(c[k].theta);
+      c[k].raysum = in[k];
+      c[k].index = (int)k;
+      dDetPos += detInc;
+    }
+  }
+  qsort(c, nc, sizeof(RebinCoord), cmp_by_t);
+  double* X = (double*)malloc(sizeof(double) * (nView > nDet ? nView : nDet));
+  double* Y = (double*)malloc(sizeof(double) * (nView > nDet ? nView : nDet));
+  /* interpolate to evenly spaced theta (views) */
+  for (int iD = 0; iD < nDet; iD++) {
+    for (int i = 0; i < nView; i++) {
+      X[i] = c[(size_t)iD * nView + i].theta;
+      Y[i] = c[(size_t)iD * nView + i].raysum;
+    }
+    double dViewAngle = rotStart;
+    for (int iV = 0; iV < nView; iV++, dViewAngle += newRotInc)
+      out[(size_t)iV * nDet + iD] = (float)lin_interp_x(X, Y, nView, dViewAngle);
+  }
+  /* interpolate to evenly spaced t (detectors) */
+  double* T = (double*)malloc(sizeof(double) * nDet);
+  for (int i = 0; i < nDet; i++) T[i] = c[(size_t)i * nView].t;
+  double dViewAngle = rotStart;
+  for (int iV = 0; iV < nView; iV++, dViewAngle += newRotInc) {
+    anglesOut[iV] = dViewAngle;
+    for (int i = 0; i < nDet; i++) Y[i] = out[(size_t)iV * nDet + i];
+    double dDetPos = newDetStart;
+    for (int iD = 0; iD < nDet; iD++, dDetPos += newDetInc)
+      out[(size_t)iV * nDet + iD] = (float)lin_interp_x(T, Y, nDet, dDetPos);
+  }
+  info6[0] = 0;
+  info6[1] = newRotInc;
+  info6[2] = newDetStart;
+  info6[3] = newDetInc;
+  info6[4] = anglesOut[0];
+  info6[5] = anglesOut[nView - 1];
+  free(T);
+  free(X);
+  free(Y);
+  free(c);
+  return 0;
+}

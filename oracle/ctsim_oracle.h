@@ -86,6 +86,13 @@ int orc_helical180li(int geom, int nDet, int nView, double rotInc, double detInc
 int orc_halfscan_feather(int geom, int nDet, int nView, double rotInc, double detInc,
                          double fanAngle, double sdl, float* data);
 
+/* Fan-beam to parallel rebinning (Projections::interpolateToParallel,
+ * libctsim/projections.cpp:1250-1328). */
+int orc_interpolate_to_parallel(int geom, int nDet, int nView, double detStart, double detInc,
+                                double focalLength, double sdl, double viewDiameter,
+                                double rotStart, const double* angles, const float* in,
+                                float* out, double* anglesOut, double* info6);
+
 #ifdef __cplusplus
 }
 #endif

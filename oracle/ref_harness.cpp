@@ -333,3 +333,41 @@ int ref_helical(const char* builtin, const char* geometry, int nDet, int nView, 
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Projections::interpolateToParallel of a sinogram whose geometry comes from
+// a Scanner on a built-in phantom.  info8: detStart, detInc, focalLength,
+// sdl, viewDiameter, rotStart of the input, new rotInc, new detInc.
+int ref_rebin(const char* builtin, const char* geometry, int nDet, int nView, double rotLen,
+              double focalRatio, double centerRatio, const float* data, float* out,
+              double* anglesIn, double* anglesOut, double* info8) {
+  Phantom phm;
+  buildPhantom(phm, nullptr, nullptr, 0, builtin);
+  Scanner scanner(phm, geometry, nDet, nView, 0, 1, rotLen, focalRatio, centerRatio, 1.0, 1.0);
+  if (scanner.fail()) { g_err = scanner.failMessage(); return 2; }
+  Projections pj(scanner);
+  double a = 0;
+  for (int v = 0; v < nView; v++, a += pj.rotInc()) {  // collectProjections' angles (scanner.cpp:330)
+    pj.getDetectorArray(v).setViewAngle(a);
+    anglesIn[v] = a;
+    for (int i = 0; i < nDet; i++) pj.getDetectorArray(v).detValues()[i] = data[(size_t)v * nDet + i];
+  }
+  Projections* p = pj.interpolateToParallel();
+  info8[0] = pj.detStart();
+  info8[1] = pj.detInc();
+  info8[2] = pj.focalLength();
+  info8[3] = pj.sourceDetectorLength();
+  info8[4] = pj.viewDiameter();
+  info8[5] = pj.rotStart();
+  info8[6] = p->rotInc();
+  info8[7] = p->detInc();
+  for (int v = 0; v < nView; v++) {
+    anglesOut[v] = p->getDetectorArray(v).viewAngle();
+    for (int i = 0; i < nDet; i++) out[(size_t)v * nDet + i] = p->getDetectorArray(v).detValues()[i];
+  }
+  if (p != &pj) delete p;
+  return 0;
+}
+
+}  // extern "C"

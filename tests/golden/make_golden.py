@@ -138,10 +138,23 @@ def helical():
     save("helical", **arrays)
 
 
+def rebin():
+    """Projections::interpolateToParallel of seeded fan-beam sinograms."""
+    rng = np.random.Generator(np.random.PCG64(12))
+    arrays = {}
+    for tag, geom, nDet, nView in [("a", "equiangular", 65, 90), ("b", "equilinear", 64, 72)]:
+        data = rng.standard_normal((nView, nDet)).astype(np.float32)
+        out, ain, aout, info = O.ref_rebin("herman", geom, nDet, nView, 2 * np.pi, 2.0, 2.0, data)
+        arrays.update({f"{tag}_in": data, f"{tag}_out": out, f"{tag}_angles_in": ain,
+                       f"{tag}_angles_out": aout, f"{tag}_info": info,
+                       f"{tag}_geom": np.array([1 if geom == "equiangular" else 2], np.int64)})
+    save("rebin", **arrays)
+
+
 if __name__ == "__main__":
     if not O.have_ref():
         O.build(ref=True)
     jobs = {"config1": config1, "tiny_grids": tiny_grids, "filters": filters,
-            "raysums": raysums, "stats": stats, "helical": helical}
+            "raysums": raysums, "stats": stats, "helical": helical, "rebin": rebin}
     for name in (sys.argv[1:] or list(jobs)):
         jobs[name]()

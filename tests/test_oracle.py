@@ -462,3 +462,40 @@ def test_helical_static_invariant(oracle_lib):
         assert err <= rotInc / (np.pi - fan) * row.max()
         # outside both regions nothing is written
         assert np.all(out[beta[:, 0] < fan - 2 * gam.max() - rotInc] == 0)
+
+
+# ---------------------------------------------------------------- rebinning
+def test_rebin_golden(oracle_lib):
+    """Fan-beam to parallel rebinning of the oracle equals the reference's
+    (projections.cpp:1250-1469) bit for bit."""
+    g = load_golden("rebin")
+    for tag in "ab":
+        info = g[f"{tag}_info"]
+        out, ang, inf6 = O.interpolate_to_parallel(int(g[f"{tag}_geom"][0]), g[f"{tag}_in"],
+                                                   *info[:6], g[f"{tag}_angles_in"])
+        assert bits_equal(out, g[f"{tag}_out"]) and bits_equal(ang, g[f"{tag}_angles_out"])
+        assert inf6[1] == info[6] and inf6[3] == info[7]
+
+
+def test_rebin_closed_form(oracle_lib):
+    """A fan-beam sinogram of a centred disc (radius R, attenuation 1) is
+    2 sqrt(R^2 - t^2) on every ray at distance t from the centre, so rebinning
+    it must give that profile on the parallel grid (to the interpolation
+    error of the fan samples)."""
+    for geom in (1, 2):
+        nDet, nView, F, sdl, R = 257, 720, 2.0, 4.0, 0.6
+        fan = 2 * np.arcsin(1.0 / F)
+        detInc = fan / nDet if geom == 1 else 2 * sdl * np.tan(fan / 2) / nDet
+        detStart = -(nDet - 1) / 2 * detInc
+        pos = detStart + np.arange(nDet) * detInc
+        gam = pos if geom == 1 else np.arctan(pos / sdl)
+        t = F * np.sin(gam)
+        row = 2 * np.sqrt(np.clip(R * R - t * t, 0, None))
+        data = np.tile(row, (nView, 1)).astype(np.float32)
+        angles = np.arange(nView) * 2 * np.pi / nView
+        out, _, info = O.interpolate_to_parallel(geom, data, detStart, detInc, F, sdl, 2.0, 0.0,
+                                                 angles)
+        tp = info[2] + np.arange(nDet) * info[3]
+        want = 2 * np.sqrt(np.clip(R * R - tp * tp, 0, None))
+        inner = np.abs(tp) < 0.55
+        assert np.abs(out[:, inner] - want[inner]).max() < 2e-3
