@@ -46,6 +46,7 @@ SIGNATURES = {
     "ctsim_scanner_info": (_i, [_vp, _vp]),
     "ctsim_collect_projections": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp]),
     "ctsim_image_stats": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
+    "ctsim_phantom_rasterize": (_i, [_vp, _vp, _i, _i, _d, _i, _vp, _vp]),
     "ctsim_rebin_create": (_i, [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _d, _d, _vp]),
     "ctsim_rebin_destroy": (None, [_vp]),
     "ctsim_rebin_info": (_i, [_vp, _vp, _vp]),

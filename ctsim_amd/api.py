@@ -397,6 +397,17 @@ class Phantom:
         check(lib.ctsim_phantom_add_element(self.handle, type.encode(), cx, cy, u, v, rot_deg,
                                             atten), "ctsim_phantom_add_element")
 
+    def rasterize(self, ctx: Context, nx: int, ny: int, viewRatio: float = 1.0, nsample: int = 1):
+        """Phantom::convertToImagefile (phantom.cpp:463-538): a new device
+        image float32 [nx, ny] and the axis extent (xmin, xmax, ymin, ymax)."""
+        import torch
+        img = torch.empty((nx, ny), dtype=torch.float32, device=f"cuda:{ctx.device}")
+        ext = np.zeros(4, np.float64)
+        check(lib.ctsim_phantom_rasterize(ctx.handle, self.handle, nx, ny, viewRatio, nsample,
+                                          C.c_void_p(_dptr(img)), _hptr(ext)),
+              "ctsim_phantom_rasterize")
+        return img, tuple(ext)
+
     def extent(self):
         e = np.zeros(4, np.float64)
         lib.ctsim_phantom_extent(self.handle, _hptr(e))

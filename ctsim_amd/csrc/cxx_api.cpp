@@ -142,6 +142,26 @@ double Phantom::maxAxisLength() const {
 }
 double Phantom::getDiameterBoundaryCircle() const { return kSqrt2 * maxAxisLength(); }
 
+void Phantom::convertToImagefile(ImageFile& im, double dViewRatio, int nsample, int) const {
+  const int nx = im.nx(), ny = im.ny();
+  if (nx < 2 || ny < 2) return;  // phantom.cpp:469-470
+  ctsim_ctx* ctx = ctsim_amd::context();
+  float* d = nullptr;
+  if (hipMalloc(&d, sizeof(float) * nx * ny) != hipSuccess) {
+    warn("convertToImagefile: device allocation failed");
+    return;
+  }
+  double ext[4];
+  if (ctsim_phantom_rasterize(ctx, m_hip, nx, ny, dViewRatio, nsample, d, ext) == CTSIM_OK) {
+    (void)hipMemcpy(im.data(), d, sizeof(float) * nx * ny, hipMemcpyDeviceToHost);
+    im.setAxisExtent(ext[0], ext[1], ext[2], ext[3]);
+    im.setAxisIncrement((ext[1] - ext[0]) / nx, (ext[3] - ext[2]) / ny);
+  } else {
+    warn(ctsim_last_error());
+  }
+  (void)hipFree(d);
+}
+
 // -------------------------------------------------------------- Scanner
 int Scanner::convertGeometryNameToID(const char* name) {
   static const char* const names[] = {"parallel", "equiangular", "equilinear", "linogram"};

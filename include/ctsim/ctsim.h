@@ -69,6 +69,9 @@ class Phantom {
   double maxAxisLength() const;
   double getDiameterBoundaryCircle() const;
   ctsim_phantom* handle() const { return m_hip; }
+  // Rasterise on the device (phantom.cpp:448-538, phm2if).
+  void convertToImagefile(class ImageFile& im, double dViewRatio, int nsample,
+                          int trace = Trace::TRACE_NONE) const;
 
  private:
   ctsim_phantom* m_hip = nullptr;

@@ -282,6 +282,15 @@ void ctsim_rebin_destroy(ctsim_rebin* r);
 int ctsim_rebin_info(const ctsim_rebin* r, double* out4, double* h_newAngles);
 int ctsim_rebin_run(ctsim_ctx* ctx, ctsim_rebin* r, const float* d_in, float* d_out);
 
+/* ------------------------------------------------------------ rasteriser */
+
+/* Replaces Phantom::convertToImagefile (phantom.cpp:463-538, the phm2if
+ * tool): d_image float [nx][ny] column-major (nx, ny >= 2), nsample x
+ * nsample samples per pixel; ext4 (optional) receives the axis extent
+ * xmin, xmax, ymin, ymax.  Synchronises the context's stream. */
+int ctsim_phantom_rasterize(ctsim_ctx* ctx, ctsim_phantom* phm, int nx, int ny, double viewRatio,
+                            int nsample, float* d_image, double* ext4);
+
 /* ------------------------------------------------------------- statistics */
 
 /* Replaces ImageFile::comparativeStatistics (imagefile.cpp:185-240) on

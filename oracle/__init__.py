@@ -453,3 +453,23 @@ def ref_rebin(builtin: str, geometry: str, nDet: int, nView: int, rotLen: float,
     if rc:
         raise RuntimeError(f"ref_rebin failed: {L.ref_last_error().decode()}")
     return out, ain, aout, info
+
+
+def rasterize(kind: str, phm: Phantom, nx: int, ny: int, viewRatio: float = 1.0,
+              nsample: int = 1) -> np.ndarray:
+    """Phantom::convertToImagefile (phm2if): returns the [nx][ny] image."""
+    L = lib(kind)
+    pre = "orc" if kind == "oracle" else "ref"
+    f = getattr(L, f"{pre}_rasterize")
+    c, i, d, vp = C.c_char_p, C.c_int, C.c_double, C.c_void_p
+    f.restype = i
+    t, p, n = phm.arrays()
+    img = np.zeros((nx, ny), np.float32)
+    if kind == "oracle":
+        f.argtypes = [vp, vp, i, c, i, i, d, i, vp, vp]
+        ext = np.zeros(4, np.float64)
+        f(_ptr(t), _ptr(p), n, _b(phm.builtin), nx, ny, viewRatio, nsample, _ptr(img), _ptr(ext))
+    else:
+        f.argtypes = [vp, vp, i, c, i, i, d, i, vp]
+        f(_ptr(t), _ptr(p), n, _b(phm.builtin), nx, ny, viewRatio, nsample, _ptr(img))
+    return img

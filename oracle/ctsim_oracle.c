@@ -1863,3 +1863,77 @@ int orc_interpolate_to_parallel(int geom, int nDet, int nView, double detStart, 
   free(c);
   return 0;
 }
+
+/* ------------------------------------------------------------ rasteriser */
+
+/* PhantomElement::isPointInside with PHM_COORD, phantom.cpp:913-975 */
+static int point_inside(const elem_t* e, double x, double y) {
+  xform_m(e->phmToObj, &x, &y);
+  switch (e->type) {
+    case PE_RECT:
+      return !(x > 1. || x < -1. || y > 1. || y < -1.);
+    case PE_TRI:
+      return !(y < 0. || y > 1. - x || y > 1. + x);
+    case PE_ELLIPSE:
+      if (x > 1. || x < -1. || y > 1. || y < -1.) return 0;
+      return !(x * x + y * y > 1.);
+    case PE_SEGMENT:
+      if (x > 1. || x < -1. || y > 0.) return 0;
+      x *= e->u;
+      y *= e->v;
+      return !(x * x + (y - e->v) * (y - e->v) > e->u * e->u + e->v * e->v);
+    case PE_SECTOR:
+      if (x > 1. || x < -1. || y > 1.) return 0;
+      if (y > 1. - x || y > 1. + x) return 0;
+      x *= e->u;
+      y *= e->v;
+      return !(x * x + (y - e->v) * (y - e->v) > e->u * e->u + e->v * e->v);
+  }
+  return 0;
+}
+
+/* Phantom::convertToImagefile, phantom.cpp:463-538, whole image: img is
+ * [nx][ny] column-major float (pixel (ix, iy) at ix*ny+iy); ext4 receives the
+ * axis extent xmin, xmax, ymin, ymax.  Returns 0, or 1 when nx or ny < 2
+ * (the reference leaves the image untouched). */
+int orc_rasterize(const int* types, const double* params, int nElem, const char* builtin,
+                  int nx, int ny, double viewRatio, int nsample, float* img, double* ext4) {
+  if (nx < 2 || ny < 2) return 1;
+  phantom_t p;
+  if (phm_build(&p, types, params, nElem, builtin)) return 2;
+  if (nsample < 1) nsample = 1;
+  const double dx = p.xmax - p.xmin, dy = p.ymax - p.ymin;
+  const double xcent = p.xmin + dx / 2, ycent = p.ymin + dy / 2;
+  const double maxAxis = dx > dy ? dx : dy;
+  const double dHalflen = viewRatio * ((SQRT2 * maxAxis) / SQRT2 / 2);
+  const double xmin = xcent - dHalflen, xmax = xcent + dHalflen;
+  const double ymin = ycent - dHalflen, ymax = ycent + dHalflen;
+  const double xinc = (xmax - xmin) / nx, yinc = (ymax - ymin) / ny;
+  const double kxinc = xinc / nsample, kyinc = yinc / nsample;
+  const double kxofs = kxinc / 2, kyofs = kyinc / 2;
+  ext4[0] = xmin; ext4[1] = xmax; ext4[2] = ymin; ext4[3] = ymax;
+  for (size_t k = 0; k < (size_t)nx * ny; k++) img[k] = 0;
+  for (int el = p.n - 1; el >= 0; el--) { /* the reference's list is newest first */
+    const elem_t* e = &p.e[el];
+    double x = xmin;
+    for (int ix = 0; ix < nx; ix++, x += xinc) {
+      double y = ymin;
+      for (int iy = 0; iy < ny; iy++, y += yinc) {
+        double dAtten = 0;
+        double xi = x + kxofs;
+        for (int kx = 0; kx < nsample; kx++, xi += kxinc) {
+          double yi = y + kyofs;
+          for (int ky = 0; ky < nsample; ky++, yi += kyinc)
+            if (point_inside(e, xi, yi)) dAtten += e->atten;
+        }
+        img[(size_t)ix * ny + iy] += dAtten;
+      }
+    }
+  }
+  if (nsample > 1) {
+    const double factor = 1.0 / (double)(nsample * nsample);
+    for (size_t k = 0; k < (size_t)nx * ny; k++) img[k] *= factor;
+  }
+  free(p.e);
+  return 0;
+}

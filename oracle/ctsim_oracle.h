@@ -93,6 +93,10 @@ int orc_interpolate_to_parallel(int geom, int nDet, int nView, double detStart, 
                                 double rotStart, const double* angles, const float* in,
                                 float* out, double* anglesOut, double* info6);
 
+/* Phantom::convertToImagefile (libctsim/phantom.cpp:463-538). */
+int orc_rasterize(const int* types, const double* params, int nElem, const char* builtin,
+                  int nx, int ny, double viewRatio, int nsample, float* img, double* ext4);
+
 #ifdef __cplusplus
 }
 #endif

@@ -371,3 +371,18 @@ int ref_rebin(const char* builtin, const char* geometry, int nDet, int nView, do
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Phantom::convertToImagefile (phm2if) into img [nx][ny] column-major.
+int ref_rasterize(const int* types, const double* params, int nElem, const char* builtin, int nx,
+                  int ny, double viewRatio, int nsample, float* img) {
+  Phantom phm;
+  buildPhantom(phm, types, params, nElem, builtin);
+  ImageFile im(nx, ny);
+  phm.convertToImagefile(im, viewRatio, nsample, Trace::TRACE_NONE);
+  copyImage(im, img);
+  return 0;
+}
+
+}  // extern "C"

@@ -151,10 +151,22 @@ def rebin():
     save("rebin", **arrays)
 
 
+def raster():
+    """Phantom::convertToImagefile (phm2if) of the built-in phantoms and of a
+    phantom holding every element type."""
+    arrays = {}
+    for tag, phm, nx, ny, vr, ns in [("shepp", O.Phantom("shepp-logan"), 64, 48, 1.0, 1),
+                                     ("mixed", O.Phantom(None, W.mixed_elements(3)), 40, 40, 1.2, 3),
+                                     ("herman", O.Phantom("herman"), 32, 33, 0.9, 2)]:
+        arrays[f"{tag}_image"] = O.rasterize("ref", phm, nx, ny, vr, ns)
+        arrays[f"{tag}_args"] = np.array([nx, ny, vr, ns], np.float64)
+    save("raster", **arrays)
+
+
 if __name__ == "__main__":
     if not O.have_ref():
         O.build(ref=True)
     jobs = {"config1": config1, "tiny_grids": tiny_grids, "filters": filters,
-            "raysums": raysums, "stats": stats, "helical": helical, "rebin": rebin}
+            "raysums": raysums, "stats": stats, "helical": helical, "rebin": rebin, "raster": raster}
     for name in (sys.argv[1:] or list(jobs)):
         jobs[name]()
