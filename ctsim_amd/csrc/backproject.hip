@@ -377,7 +377,7 @@ __global__ __launch_bounds__(256) void coef_kernel(const float* __restrict__ f, 
 template <int GK, int INTERP, int P, int DEG>
 __global__ __launch_bounds__(256) void bp_mixed_kernel(MixedArgs a) {
   extern __shared__ float4 s_dyn[];
-  __shared__ float4 s_v[dev::kMaxChunk];
+  __shared__ float4 s_v[dev::kMaxChunk + 2];
   __shared__ int s_lo[dev::kMaxChunk];
   const int nTiles = a.tilesX * a.tilesY;
   int tile, grp;

@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void fbp_batch_kernel(const SliceDev* __restri
                                                         const int* __restrict__ queueStart,
                                                         int* counters, int* error) {
   extern __shared__ float4 smem[];
-  __shared__ float4 s_v[dev::kMaxChunk];
+  __shared__ float4 s_v[dev::kMaxChunk + 2];
   __shared__ int s_lo[dev::kMaxChunk];
   __shared__ int s_item;
   const int q = blockIdx.x & 7;

@@ -201,15 +201,18 @@ __global__ __launch_bounds__(256) void fft_filter_kernel(
 // ------------------------------------------- Toeplitz GEMM on the MFMA units
 // Mixed-precision convolution (procsignal.cpp:659-673 as a matrix product):
 //   Y[v][i] = sum_j X[v][j] * T[j][i],  T[j][i] = dx * h[i - j + n - 1] = g[j - i + n - 1]
-// M = views, N = outputs, K = inputs.  X and g are split into bf16 hi + lo
-// (hi = bf16(x), lo = bf16(x - hi)) and the product is hi*hi + hi*lo + lo*hi
-// with FP32 accumulation, three v_mfma_f32_32x32x16_bf16 per tile and k-step.
-// A workgroup (4 waves) computes a 64-view x 128-output tile; wave (wm, wn)
-// owns rows wm*32.. and two 32-column MFMA tiles at wn*64.  The Toeplitz
-// operand is never materialised: for a k-chunk the window of g it needs is
-// staged in LDS as 8 copies shifted by one element, so the 8 consecutive
-// values a lane needs always start at an 8-element (16-byte) boundary of one
-// copy.
+// M = views, N = outputs, K = inputs.  X and g are split into three BF16
+// parts (hi = bf16(x), mid = bf16(x - hi), lo = bf16(x - hi - mid)) and the
+// product keeps every term down to 2^-16 relative, hi*hi + hi*mid + mid*hi +
+// hi*lo + lo*hi + mid*mid, with FP32 accumulation: six
+// v_mfma_f32_32x32x16_bf16 per tile and k-step, a product accurate to about
+// 2^-24 (a two-part split leaves 2^-17 per product, which the ramp filter's
+// cancellation amplifies past 1e-4 in the image).  A workgroup (4 waves)
+// computes a 64-view x 128-output tile; wave (wm, wn) owns rows wm*32.. and
+// two 32-column MFMA tiles at wn*64.  The Toeplitz operand is never
+// materialised: for a k-chunk the window of g it needs is staged in LDS as 8
+// copies shifted by one element, so the 8 consecutive values a lane needs
+// always start at an 8-element (16-byte) boundary of one copy.
 #pragma clang fp contract(fast)
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx16_t __attribute__((ext_vector_type(16)));
@@ -224,11 +227,20 @@ __device__ __forceinline__ unsigned short bf16_bits(float x) {
   return __builtin_bit_cast(unsigned short, b);
 }
 
+__device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
+  h = static_cast<__bf16>(x);
+  const float r1 = x - static_cast<float>(h);
+  m = static_cast<__bf16>(r1);
+  l = static_cast<__bf16>(r1 - static_cast<float>(m));
+}
+
 __global__ __launch_bounds__(256) void toeplitz_mfma_kernel(
     const float* __restrict__ in, const float* __restrict__ w32,
-    const unsigned short* __restrict__ gh, const unsigned short* __restrict__ gl,
-    float* __restrict__ out, int nView, int n, int outStride) {
+    const unsigned short* __restrict__ gh, const unsigned short* __restrict__ gm,
+    const unsigned short* __restrict__ gl, float* __restrict__ out, int nView, int n,
+    int outStride) {
   __shared__ __attribute__((aligned(16))) unsigned short s_gh[8][kWinPad];
+  __shared__ __attribute__((aligned(16))) unsigned short s_gm[8][kWinPad];
   __shared__ __attribute__((aligned(16))) unsigned short s_gl[8][kWinPad];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -245,6 +257,7 @@ __global__ __launch_bounds__(256) void toeplitz_mfma_kernel(
       const int q = qb + c + m;
       const bool ok = q >= 0 && q < nq;
       s_gh[c][m] = ok ? gh[q] : 0;
+      s_gm[c][m] = ok ? gm[q] : 0;
       s_gl[c][m] = ok ? gl[q] : 0;
     }
     __syncthreads();
@@ -252,7 +265,7 @@ __global__ __launch_bounds__(256) void toeplitz_mfma_kernel(
     for (int s = 0; s < kTK / 16; s++) {
       // A fragment: X[row][j0 + 16 s + 8 kg + t], t = 0..7
       const int k0 = j0 + 16 * s + 8 * kg;
-      bf16x8_t ah, al;
+      bf16x8_t ah, am, al;
 #pragma unroll
       for (int t = 0; t < 8; t++) {
         float x = 0.f;
@@ -261,9 +274,11 @@ __global__ __launch_bounds__(256) void toeplitz_mfma_kernel(
           x = in[static_cast<size_t>(row) * n + k];
           if (w32) x *= w32[k];
         }
-        const __bf16 h = static_cast<__bf16>(x);
+        __bf16 h, m, l;
+        split3(x, h, m, l);
         ah[t] = h;
-        al[t] = static_cast<__bf16>(x - static_cast<float>(h));
+        am[t] = m;
+        al[t] = l;
       }
 #pragma unroll
       for (int tile = 0; tile < 2; tile++) {
@@ -271,14 +286,17 @@ __global__ __launch_bounds__(256) void toeplitz_mfma_kernel(
         const int cbase = wn * 64 + tile * 32;
         const int p0 = (kTN - 1) + 16 * s + 8 * kg - cbase - (lane & 31);
         const int c = p0 & 7, mpos = p0 - c;
-        const u16x8_t bh_bits = *reinterpret_cast<const u16x8_t*>(&s_gh[c][mpos]);
-        const u16x8_t bl_bits = *reinterpret_cast<const u16x8_t*>(&s_gl[c][mpos]);
-        const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, bh_bits);
-        const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, bl_bits);
+        const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u16x8_t*>(&s_gh[c][mpos]));
+        const bf16x8_t bm = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u16x8_t*>(&s_gm[c][mpos]));
+        const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u16x8_t*>(&s_gl[c][mpos]));
         floatx16_t& acc = tile == 0 ? acc0 : acc1;
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+        // smallest terms first
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bm, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bm, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
       }
     }
   }
@@ -349,15 +367,19 @@ int filter_upload(FilterPlan& p) {
     CTSIM_HIP(hipMemcpy(p.d_weight32.ptr, w32.data(), w32.size() * sizeof(float), hipMemcpyHostToDevice));
   }
   if (p.method == METHOD_CONVOLUTION) {
-    // Toeplitz operand of the mixed convolution: g[q] = dx * h[2n-2-q]
+    // Toeplitz operand of the mixed convolution: g[q] = dx * h[2n-2-q],
+    // split into bf16 hi, mid, lo from its double value
     const int nq = 2 * p.n - 1;
-    std::vector<unsigned short> g16(2 * static_cast<size_t>(nq));
+    std::vector<unsigned short> g16(3 * static_cast<size_t>(nq));
     for (int q = 0; q < nq; q++) {
-      const float g = static_cast<float>(p.inc * p.filter[2 * p.n - 2 - q]);
-      const __bf16 hi = static_cast<__bf16>(g);
-      const __bf16 lo = static_cast<__bf16>(g - static_cast<float>(hi));
+      const double g = p.inc * p.filter[2 * p.n - 2 - q];
+      const __bf16 hi = static_cast<__bf16>(static_cast<float>(g));
+      const double r1 = g - static_cast<double>(static_cast<float>(hi));
+      const __bf16 mid = static_cast<__bf16>(static_cast<float>(r1));
+      const __bf16 lo = static_cast<__bf16>(static_cast<float>(r1 - static_cast<double>(static_cast<float>(mid))));
       g16[q] = __builtin_bit_cast(unsigned short, hi);
-      g16[nq + q] = __builtin_bit_cast(unsigned short, lo);
+      g16[nq + q] = __builtin_bit_cast(unsigned short, mid);
+      g16[2 * static_cast<size_t>(nq) + q] = __builtin_bit_cast(unsigned short, lo);
     }
     CTSIM_CHECK(p.d_toeplitz.reserve(g16.size() * sizeof(unsigned short)));
     CTSIM_HIP(hipMemcpy(p.d_toeplitz.ptr, g16.data(), g16.size() * sizeof(unsigned short),
@@ -424,8 +446,8 @@ int filter_launch(ctsim_ctx* ctx, FilterPlan& p, int precision, const float* d_i
     const unsigned short* g = p.d_toeplitz.as<unsigned short>();
     const float* w32 = p.preweight.empty() ? nullptr : p.d_weight32.as<float>();
     hipLaunchKernelGGL(toeplitz_mfma_kernel, dim3((p.n + kTN - 1) / kTN, (nView + kTM - 1) / kTM),
-                       dim3(256), 0, s, d_in, w32, g, g + nq, static_cast<float*>(d_out), nView,
-                       p.n, outStride);
+                       dim3(256), 0, s, d_in, w32, g, g + nq, g + 2 * static_cast<size_t>(nq),
+                       static_cast<float*>(d_out), nView, p.n, outStride);
     CTSIM_HIP(hipGetLastError());
     return CTSIM_OK;
   }

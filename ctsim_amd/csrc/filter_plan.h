@@ -32,7 +32,7 @@ struct FilterPlan {
   bool tablesBuilt = false;
 
   DevBuf d_filter, d_weight, d_weight32, d_hmul64, d_hmul32, d_tw64, d_tw32, d_dft;
-  DevBuf d_toeplitz;  // bf16 hi then lo of g[q] = dx * filter[2n-2-q], mixed convolution
+  DevBuf d_toeplitz;  // bf16 hi, mid, lo of g[q] = dx * filter[2n-2-q], mixed convolution
 };
 
 // Host side (filter_host.cpp).

@@ -150,7 +150,8 @@ __device__ __forceinline__ void pos_mixed(const MixedArgs& a, float ca, float sa
 }
 
 // One workgroup's work item: a tile of (256 / lanesY) columns x
-// (lanesY * P) rows and one group of views.  Lane (cx, ly) owns column ix0 + cx and rows
+// (lanesY * P) rows and one group of views.  s_v must hold kMaxChunk + 2
+// entries.  Lane (cx, ly) owns column ix0 + cx and rows
 // iy0 + P*ly .. + P-1 (contiguous, so the final read-modify-write is a
 // vector access).  Per chunk of views: lanes < nv compute each view's window
 // start from the tile corners, the workgroup stages the window's interval
@@ -217,20 +218,55 @@ __device__ __forceinline__ void bp_mixed_tile(const MixedArgs& a, int tile, int 
     float part[P];
 #pragma unroll
     for (int k = 0; k < P; k++) part[k] = 0.f;
-    for (int vv = 0; vv < nv; vv++) {
-      const float4 c = s_v[vv];
+    // Two views per iteration in three phases (all positions, then all
+    // window reads, then all interpolations), so 2P independent LDS reads are
+    // in flight together; the next pair's constants are read ahead (s_v holds
+    // kMaxChunk + 2 entries, so the read-ahead stays in bounds).
+    float4 c0 = s_v[0], c1 = s_v[1];
+    int vv = 0;
+    for (; vv + 1 < nv; vv += 2) {
+      const float4 n0 = s_v[vv + 2], n1 = s_v[vv + 3];
+      float u[2 * P], wg[2 * P];
+      int j[2 * P];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const float4& c = h ? c1 : c0;
+#pragma unroll
+        for (int k = 0; k < P; k++) {
+          float pos;
+          pos_mixed<GK, DEG>(a, c.x, c.y, c.z, x, y[k], r2mF2[k], pos, wg[h * P + k]);
+          const float pf = floorf(pos);
+          u[h * P + k] = pos - pf;
+          j[h * P + k] = (vv + h) * W + min(max(static_cast<int>(pf), 0), W - 1);
+        }
+      }
+      CT q[2 * P];
+#pragma unroll
+      for (int i = 0; i < 2 * P; i++) q[i] = s_w[j[i]];
+#pragma unroll
+      for (int i = 0; i < 2 * P; i++) {
+        float val;
+        if constexpr (INTERP == INTERP_CUBIC)
+          val = fmaf(u[i], fmaf(u[i], fmaf(u[i], q[i].w, q[i].z), q[i].y), q[i].x);
+        else
+          val = fmaf(u[i], q[i].y, q[i].x);
+        part[i % P] = fmaf(val, wg[i], part[i % P]);  // view vv before view vv+1
+      }
+      c0 = n0;
+      c1 = n1;
+    }
+    if (vv < nv) {
       const CT* w = s_w + vv * W;
 #pragma unroll
       for (int k = 0; k < P; k++) {
         float pos, wgt;
-        pos_mixed<GK, DEG>(a, c.x, c.y, c.z, x, y[k], r2mF2[k], pos, wgt);
+        pos_mixed<GK, DEG>(a, c0.x, c0.y, c0.z, x, y[k], r2mF2[k], pos, wgt);
         const float pf = floorf(pos);
-        const float u = pos - pf;
-        const int j = min(max(static_cast<int>(pf), 0), W - 1);
-        const CT q = w[j];
+        const float uu = pos - pf;
+        const CT qq = w[min(max(static_cast<int>(pf), 0), W - 1)];
         float val;
-        if constexpr (INTERP == INTERP_CUBIC) val = fmaf(u, fmaf(u, fmaf(u, q.w, q.z), q.y), q.x);
-        else val = fmaf(u, q.y, q.x);
+        if constexpr (INTERP == INTERP_CUBIC) val = fmaf(uu, fmaf(uu, fmaf(uu, qq.w, qq.z), qq.y), qq.x);
+        else val = fmaf(uu, qq.y, qq.x);
         part[k] = fmaf(val, wgt, part[k]);
       }
     }

@@ -1,8 +1,10 @@
 """Batched mixed FBP (one persistent launch in a HIP graph) against the
-single-slice paths.  Bars: every slice within d <= 1e-4 of the FP64
-single-slice image (Herman's d), and within 2e-5 * max|image| of the
-single-slice mixed image; every launch shape gives the same images; graph
-replays are bitwise identical; new image buffers are picked up."""
+single-slice paths.  Bars: every slice within d <= 1e-5 of the FP64
+single-slice image (Herman's d; the batch filters convolutions by FP32 FFT
+convolution, measured d ~ 1e-6) and within d <= 1e-4 of the single-slice
+mixed image (which filters convolutions by the BF16x3 Toeplitz product);
+every launch shape gives the same images to FP32 summation-order noise;
+graph replays are bitwise identical; new image buffers are picked up."""
 import numpy as np
 import pytest
 
@@ -17,7 +19,7 @@ def _specs():
             ("equiangular", 512, 360, 256, "cubic", "fft", "abs_hanning"),
             ("parallel", 401, 200, 128, "cubic", "fourier", "abs_hamming"),
             ("equiangular", 255, 181, 384, "linear", "convolution", "abs_cosine"),
-            ("equilinear", 300, 97, 128, "cubic", "fft", "abs_sinc"),
+            ("equilinear", 300, 97, 128, "cubic", "fft", "abs_cosine"),
             ("parallel", 1024, 720, 512, "linear", "convolution", "abs_bandlimit")]
     for k, (g, nd, nv, nx, interp, meth, filt) in enumerate(rows):
         out.append(W.SliceSpec(elems=W.seeded_elements(16 + 4 * k, 0xC7511 + 900 + k), geometry=g,
@@ -49,12 +51,12 @@ def batch_case(gpu_ctx):
 def _check(ctx, specs, images, ref64, refmx):
     import ctsim_amd
     for s, img, r64, rmx in zip(specs, images, ref64, refmx):
-        d, r, e = ctsim_amd.image_stats(ctx, r64, img)
-        dm = float((img - rmx).abs().max()) / float(rmx.abs().max())
+        d = ctsim_amd.image_stats(ctx, r64, img)[0]
+        dm = ctsim_amd.image_stats(ctx, rmx, img)[0]
         print(f"  {s.geometry} {s.nDet}x{s.nView} {s.nx}^2 {s.method} {s.interp}: d={d:.2e} "
-              f"vs mixed {dm:.2e}", flush=True)
-        assert d <= 1e-4
-        assert dm <= 2e-5
+              f"vs single-slice mixed d={dm:.2e}", flush=True)
+        assert d <= 1e-5
+        assert dm <= 1e-4
 
 
 def test_batch_matches_single_slice(gpu_ctx, batch_case):
@@ -77,6 +79,12 @@ def test_batch_shapes(gpu_ctx, batch_case, P, lanesY, chunk, fused):
     b.run(sinos, imgs)
     b.status()
     _check(gpu_ctx, specs, imgs, ref64, refmx)
+    b.set_shape(2, 16, 32768, True)
+    base = [torch.zeros_like(i) for i in images]
+    b.run(sinos, base)
+    b.status()
+    for a, c in zip(imgs, base):
+        assert float((a - c).abs().max()) <= 2e-5 * float(c.abs().max())
 
 
 def test_batch_graph_replays(gpu_ctx, batch_case):

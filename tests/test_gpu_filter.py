@@ -148,9 +148,10 @@ def test_fourier_hanning_cubic_end_to_end(gpu_ctx, geom, nDet):
 
 @pytest.mark.parametrize("n", [33, 130, 367, 1024])
 def test_mixed_convolution_toeplitz_mfma(gpu_ctx, oracle_lib, n):
-    """Mixed convolution runs as a split-BF16 Toeplitz product on the MFMA
-    units: its error against the FP64 oracle is bounded by the bf16 hi+lo
-    split (about 2^-16 relative per product term)."""
+    """Mixed convolution runs as a three-way split-BF16 Toeplitz product on
+    the MFMA units (terms down to 2^-16 relative kept): its error against the
+    FP64 oracle is at FP32 level, max relative 5e-6 and rms 1e-6 of the
+    output's scale (a hi+lo split measured 1.1e-5 and 6.7e-6)."""
     import ctsim_amd
     O = oracle_lib
     rng = np.random.default_rng(n)
@@ -165,4 +166,4 @@ def test_mixed_convolution_toeplitz_mfma(gpu_ctx, oracle_lib, n):
         err = np.abs(mix - ref).max() / np.abs(ref).max()
         rms = np.sqrt(np.mean((mix - ref) ** 2) / np.mean(ref ** 2))
         print(f"  n={n} geom={geom} {f}: max rel {err:.2e}, rms rel {rms:.2e}")
-        assert err <= 1e-4 and rms <= 2e-5
+        assert err <= 5e-6 and rms <= 1e-6
