@@ -119,6 +119,10 @@ def main():
                     help="config2 (default): the headline single-slice measurement; the others "
                          "are the large-size, batched and helical configurations")
     ap.add_argument("--slices", type=int, default=256, help="config4: slices in the batch")
+    ap.add_argument("--shape", default="",
+                    help="config2: mixed launch shape lanesY,P,split,chunk instead of autotuning")
+    ap.add_argument("--batch", type=int, default=8,
+                    help="config2: also time this many copies of the slice as one batch (0: skip)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the benchmark, run three device copies of this many MiB "
                          "(a kernel of known traffic for scaling HBM counters)")
@@ -201,9 +205,15 @@ def main():
     ctx.profile(False)
     log(f"[bench] FP64 path: {t_f64 * 1e3:.2f} ms/slice (backprojection kernel {k64_ms / k64_n:.3f} ms)")
 
-    # --- mixed path: launch shape from the autotuner (setup, untimed)
+    # --- mixed path: launch shape from the autotuner (setup, untimed), or
+    # the one given with --shape (profiling runs pin the timed run's shape)
     ps.filter(sino, f32, ctsim_amd.PREC_MIXED)
-    shape = bp.autotune(f32, ang)
+    if args.shape:
+        lanesY, P, split, chunk = (int(x) for x in args.shape.split(","))
+        bp.set_shape(lanesY, P, split, chunk)
+        shape = (lanesY, P, split, chunk, -1, float("nan"))
+    else:
+        shape = bp.autotune(f32, ang)
     log(f"[bench] autotuned mixed shape: lanesY={shape[0]} P={shape[1]} split={shape[2]} "
         f"chunk={shape[3]} window={shape[4]} ({shape[5]:.3f} ms)")
 
@@ -224,6 +234,35 @@ def main():
 
     d, r, e = ctsim_amd.image_stats(ctx, img64, img32)
     log(f"[bench] mixed vs FP64: d={d:.3e} r={r:.3e} e={e:.3e}")
+
+    # --- the same slice as a batch of B copies: filtering, coefficients and
+    # backprojection of all B in one persistent launch replayed from a graph
+    batch = None
+    if args.batch > 0:
+        from ctsim_amd import fbp
+        desc = fbp.slice_desc(spec, info, ang)
+        b = ctsim_amd.Batch(ctx, [desc] * args.batch)
+        sinos = [sino] * args.batch
+        bimgs = [torch.empty((spec.nx, spec.ny), dtype=torch.float32, device=dev)
+                 for _ in range(args.batch)]
+        bshape = b.autotune(sinos, bimgs)
+        for _ in range(args.warmup):
+            b.run(sinos, bimgs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.run(sinos, bimgs)
+        torch.cuda.synchronize()
+        t_batch = (time.perf_counter() - t0) / args.steps
+        b.status()
+        db = ctsim_amd.image_stats(ctx, img64, bimgs[-1])
+        log(f"[bench] batch of {args.batch}: {t_batch * 1e3:.3f} ms "
+            f"({args.batch / t_batch:.1f} slices/s), shape {bshape}, d={db[0]:.3e}")
+        batch = {"slices": args.batch, "ms_per_batch": t_batch * 1e3,
+                 "slices_per_s": args.batch / t_batch, "d": db[0],
+                 "speedup_over_fp64": t_f64 * args.batch / t_batch,
+                 "launch_shape": {"P": bshape[0], "lanesY": bshape[1], "chunk_bytes": bshape[2],
+                                  "fused": bool(bshape[3])}}
 
     pvu = spec.nx * spec.ny * spec.nView
     flops = FLOPS_PER_PVU_EQUIANGULAR_CUBIC_KAHAN * pvu
@@ -264,6 +303,7 @@ def main():
                                       "chunk": shape[3], "window": shape[4]}},
         "raysum": {"fp64_ms": t_raysum64 * 1e3, "closed_form_ms": t_raysum_m * 1e3,
                    "ray_element_pairs": n_re},
+        "batched": batch,
     }
     if not args.no_cpu:
         log(f"[bench] CPU baseline (~{args.cpu_seconds:.0f} s of single-thread work) ...")

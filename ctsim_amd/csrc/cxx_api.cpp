@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
+#include <algorithm>
 #include <fstream>
 #include <iostream>
 
@@ -338,6 +339,117 @@ bool Projections::read(const char* fname) {
   return true;
 }
 
+namespace {
+
+// Runs fn(ctx, d_in, d_out) on a device copy of the projections' data.
+template <class F>
+int on_device(const std::vector<float>& in, std::vector<float>& out, size_t outElems, F fn) {
+  ctsim_ctx* ctx = ctsim_amd::context();
+  float *d_in = nullptr, *d_out = nullptr;
+  if (hipMalloc(&d_in, sizeof(float) * std::max<size_t>(in.size(), 1)) != hipSuccess ||
+      hipMalloc(&d_out, sizeof(float) * std::max<size_t>(outElems, 1)) != hipSuccess) {
+    if (d_in) (void)hipFree(d_in);
+    warn("device allocation failed");
+    return CTSIM_ERR_HIP;
+  }
+  (void)hipMemcpy(d_in, in.data(), sizeof(float) * in.size(), hipMemcpyHostToDevice);
+  int rc = fn(ctx, d_in, d_out);
+  if (rc == CTSIM_OK) {
+    ctsim_ctx_synchronize(ctx);
+    out.resize(outElems);
+    (void)hipMemcpy(out.data(), d_out, sizeof(float) * outElems, hipMemcpyDeviceToHost);
+  } else {
+    warn(ctsim_last_error());
+  }
+  (void)hipFree(d_in);
+  (void)hipFree(d_out);
+  return rc;
+}
+
+}  // namespace
+
+// Projections::Helical180LI, projections.cpp:200-358
+int Projections::Helical180LI(int interpView) {
+  if (m_geometry == Scanner::GEOMETRY_PARALLEL) {
+    std::cerr << "Helical 180LI not yet implemented for PARALLEL geometry" << std::endl;
+    return 2;
+  }
+  const int cap = static_cast<int>((3.14159265358979323846 + m_dFanBeamAngle) / m_rotInc) + 2;
+  std::vector<float> out;
+  std::vector<double> ang(cap);
+  int n = 0;
+  const int rc = on_device(m_data, out, static_cast<size_t>(cap) * m_nDet,
+                           [&](ctsim_ctx* ctx, const float* din, float* dout) {
+                             return ctsim_helical180li(ctx, m_geometry, m_nDet, m_nView, m_rotInc,
+                                                       m_detInc, m_dFanBeamAngle,
+                                                       m_dSourceDetectorLength, interpView, din,
+                                                       dout, cap, &n, ang.data());
+                           });
+  if (rc != CTSIM_OK) return 1;
+  const int nDet = m_nDet;
+  init(n, nDet);
+  std::copy(out.begin(), out.begin() + static_cast<size_t>(n) * nDet, m_data.begin());
+  for (int v = 0; v < n; v++) m_views[v].setViewAngle(ang[v]);
+  return 0;
+}
+
+// Projections::HalfScanFeather, projections.cpp:369-439
+int Projections::HalfScanFeather() {
+  if (m_geometry == Scanner::GEOMETRY_PARALLEL) {
+    std::cerr << "HalfScanFeather not yet implemented for PARALLEL geometry" << std::endl;
+    return 2;
+  }
+  std::vector<float> out;
+  const int rc = on_device(m_data, out, m_data.size(),
+                           [&](ctsim_ctx* ctx, const float* din, float* dout) {
+                             return ctsim_halfscan_feather(ctx, m_geometry, m_nDet, m_nView,
+                                                           m_rotInc, m_detInc, m_dFanBeamAngle,
+                                                           m_dSourceDetectorLength, din, dout);
+                           });
+  if (rc != CTSIM_OK) return 1;
+  std::copy(out.begin(), out.end(), m_data.begin());
+  return 0;
+}
+
+// Projections::interpolateToParallel, projections.cpp:1250-1328
+Projections* Projections::interpolateToParallel() const {
+  if (m_geometry == Scanner::GEOMETRY_PARALLEL) return const_cast<Projections*>(this);
+  std::vector<double> angles(m_nView);
+  for (int v = 0; v < m_nView; v++) angles[v] = m_views[v].viewAngle();
+  ctsim_rebin* rb = nullptr;
+  if (ctsim_rebin_create(ctsim_amd::context(), &rb, m_geometry, m_nDet, m_nView, m_detStart,
+                         m_detInc, m_dFocalLength, m_dSourceDetectorLength, m_dViewDiameter,
+                         m_rotStart, angles.data()) != CTSIM_OK) {
+    warn(ctsim_last_error());
+    return nullptr;
+  }
+  double info[4];
+  std::vector<double> newAngles(m_nView);
+  ctsim_rebin_info(rb, info, newAngles.data());
+  std::vector<float> out;
+  const int rc = on_device(m_data, out, m_data.size(),
+                           [&](ctsim_ctx* ctx, const float* din, float* dout) {
+                             return ctsim_rebin_run(ctx, rb, din, dout);
+                           });
+  ctsim_rebin_destroy(rb);
+  if (rc != CTSIM_OK) return nullptr;
+  auto* p = new Projections(m_nView, m_nDet);
+  p->m_geometry = Scanner::GEOMETRY_PARALLEL;
+  p->m_dFocalLength = m_dFocalLength;
+  p->m_dSourceDetectorLength = m_dSourceDetectorLength;
+  p->m_dViewDiameter = m_dViewDiameter;
+  p->m_dFanBeamAngle = m_dFanBeamAngle;
+  p->m_calcTime = 0;
+  p->m_remark = m_remark + "; Interpolate to Parallel";
+  p->m_rotStart = info[0];
+  p->m_rotInc = info[1];
+  p->m_detStart = info[2];
+  p->m_detInc = info[3];
+  std::copy(out.begin(), out.end(), p->m_data.begin());
+  for (int v = 0; v < m_nView; v++) p->m_views[v].setViewAngle(newAngles[v]);
+  return p;
+}
+
 // ------------------------------------------------------------ ImageFile
 ImageFile::ImageFile() {}
 ImageFile::ImageFile(int nx, int ny) { setArraySize(nx, ny); }
@@ -647,17 +759,19 @@ Reconstructor::Reconstructor(const Projections& rProj, ImageFile& rIF, const cha
                              const char* filterGenerationName, const char* interpName,
                              int interpFactor, const char* backprojectName, int iTrace,
                              ReconstructionROI* pROI, bool bRebinToParallel, SGP*)
-    : m_proj(rProj), m_iTrace(iTrace) {
-  if (bRebinToParallel) {
+    : m_rOriginalProj(rProj), m_bRebinToParallel(bRebinToParallel), m_iTrace(iTrace) {
+  m_pProj = bRebinToParallel ? rProj.interpolateToParallel() : &rProj;
+  if (!m_pProj) {
     m_bFail = true;
-    m_strFailMessage = "Rebinning to parallel geometry is not supported by this build";
+    m_strFailMessage = "Error rebinning to parallel geometry";
     return;
   }
-  const double bw = 1. / rProj.detInc();
-  m_pProcessSignal = new ProcessSignal(filterName, filterMethodName, bw, rProj.detInc(), rProj.nDet(),
+  const Projections& proj = *m_pProj;
+  const double bw = 1. / proj.detInc();
+  m_pProcessSignal = new ProcessSignal(filterName, filterMethodName, bw, proj.detInc(), proj.nDet(),
                                        filt_param, "spatial", filterGenerationName, zeropad,
-                                       interpFactor, iTrace, rProj.geometry(), rProj.focalLength(),
-                                       rProj.sourceDetectorLength());
+                                       interpFactor, iTrace, proj.geometry(), proj.focalLength(),
+                                       proj.sourceDetectorLength());
   if (m_pProcessSignal->fail()) {
     m_bFail = true;
     m_strFailMessage = "Error creating ProcessSignal: " + m_pProcessSignal->failMessage();
@@ -665,7 +779,7 @@ Reconstructor::Reconstructor(const Projections& rProj, ImageFile& rIF, const cha
     m_pProcessSignal = nullptr;
     return;
   }
-  m_pBackprojector = new Backprojector(rProj, rIF, backprojectName, interpName, interpFactor, pROI);
+  m_pBackprojector = new Backprojector(proj, rIF, backprojectName, interpName, interpFactor, pROI);
   if (m_pBackprojector->fail()) {
     m_bFail = true;
     m_strFailMessage = "Error creating backprojector: " + m_pBackprojector->failMessage();
@@ -679,12 +793,13 @@ Reconstructor::Reconstructor(const Projections& rProj, ImageFile& rIF, const cha
 Reconstructor::~Reconstructor() {
   delete m_pBackprojector;
   delete m_pProcessSignal;
+  if (m_bRebinToParallel && m_pProj != &m_rOriginalProj) delete m_pProj;
   if (m_dSino) (void)hipFree(m_dSino);
   if (m_dFiltered) (void)hipFree(m_dFiltered);
 }
 
 void Reconstructor::reconstructAllViews() {
-  reconstructView(0, m_proj.nView());
+  reconstructView(0, m_pProj->nView());
   postProcessing();
 }
 
@@ -697,19 +812,19 @@ void Reconstructor::postProcessing() {
 void Reconstructor::reconstructView(int iStartView, int iViewCount, SGP*, bool bBackprojectView,
                                     double) {
   if (m_bFail) return;
-  if (iViewCount <= 0) iViewCount = m_proj.nView() - iStartView;
+  if (iViewCount <= 0) iViewCount = m_pProj->nView() - iStartView;
   if (iViewCount <= 0) return;
-  const int nDet = m_proj.nDet(), nView = m_proj.nView();
+  const int nDet = m_pProj->nDet(), nView = m_pProj->nView();
   const int prec = ctsim_amd::precision();
   if (!m_dSino) {
     (void)hipMalloc(reinterpret_cast<void**>(&m_dSino), sizeof(float) * static_cast<size_t>(nView) * nDet);
     (void)hipMalloc(&m_dFiltered, sizeof(double) * static_cast<size_t>(nView) * nDet);
-    (void)hipMemcpy(m_dSino, m_proj.data(), sizeof(float) * static_cast<size_t>(nView) * nDet,
+    (void)hipMemcpy(m_dSino, m_pProj->data(), sizeof(float) * static_cast<size_t>(nView) * nDet,
                     hipMemcpyHostToDevice);
   }
   std::vector<double> angles(iViewCount);
   for (int v = 0; v < iViewCount; v++) {
-    angles[v] = m_proj.getDetectorArray(iStartView + v).viewAngle();
+    angles[v] = m_pProj->getDetectorArray(iStartView + v).viewAngle();
     if (m_iTrace == Trace::TRACE_CONSOLE)
       std::cout << "Reconstructing view " << iStartView + v << " (last = " << nView - 1 << ")\n";
   }

@@ -223,6 +223,7 @@ __device__ __forceinline__ void bp_mixed_tile(const MixedArgs& a, int tile, int 
     // in flight together; the next pair's constants are read ahead (s_v holds
     // kMaxChunk + 2 entries, so the read-ahead stays in bounds).
     float4 c0 = s_v[0], c1 = s_v[1];
+    const float wmax = static_cast<float>(W - 1);
     int vv = 0;
     for (; vv + 1 < nv; vv += 2) {
       const float4 n0 = s_v[vv + 2], n1 = s_v[vv + 3];
@@ -235,9 +236,10 @@ __device__ __forceinline__ void bp_mixed_tile(const MixedArgs& a, int tile, int 
         for (int k = 0; k < P; k++) {
           float pos;
           pos_mixed<GK, DEG>(a, c.x, c.y, c.z, x, y[k], r2mF2[k], pos, wg[h * P + k]);
-          const float pf = floorf(pos);
-          u[h * P + k] = pos - pf;
-          j[h * P + k] = (vv + h) * W + min(max(static_cast<int>(pf), 0), W - 1);
+          // interval index clamped in float (one med3; the clamp keeps NaN in
+          // range too), truncated (non-negative), offset within the interval
+          u[h * P + k] = __builtin_amdgcn_fractf(pos);
+          j[h * P + k] = (vv + h) * W + static_cast<int>(__builtin_amdgcn_fmed3f(pos, 0.f, wmax));
         }
       }
       CT q[2 * P];
@@ -261,9 +263,8 @@ __device__ __forceinline__ void bp_mixed_tile(const MixedArgs& a, int tile, int 
       for (int k = 0; k < P; k++) {
         float pos, wgt;
         pos_mixed<GK, DEG>(a, c0.x, c0.y, c0.z, x, y[k], r2mF2[k], pos, wgt);
-        const float pf = floorf(pos);
-        const float uu = pos - pf;
-        const CT qq = w[min(max(static_cast<int>(pf), 0), W - 1)];
+        const float uu = __builtin_amdgcn_fractf(pos);
+        const CT qq = w[static_cast<int>(__builtin_amdgcn_fmed3f(pos, 0.f, wmax))];
         float val;
         if constexpr (INTERP == INTERP_CUBIC) val = fmaf(uu, fmaf(uu, fmaf(uu, qq.w, qq.z), qq.y), qq.x);
         else val = fmaf(uu, qq.y, qq.x);

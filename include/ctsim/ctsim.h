@@ -132,6 +132,14 @@ class Projections {
   // Contiguous [nView][nDet] float data (the DetectorArrays point into it).
   const float* data() const { return m_data.data(); }
   float* data() { return m_data.data(); }
+  // On the device (projections.cpp:200-439, 1250-1328).  Helical180LI and
+  // HalfScanFeather return 0 on success like the reference; the equilinear
+  // geometry is accepted (the reference supports equiangular only).
+  int Helical180LI(int interpView);
+  int HalfScanFeather();
+  // A new parallel-beam Projections (the caller deletes it), or this object
+  // itself for parallel data, as the reference.
+  Projections* interpolateToParallel() const;
 
  private:
   void init(int nView, int nDet);
@@ -310,7 +318,9 @@ class Reconstructor {
   void postProcessing();
 
  private:
-  const Projections& m_proj;
+  const Projections& m_rOriginalProj;
+  const Projections* m_pProj = nullptr;  // the original, or its parallel rebinning
+  bool m_bRebinToParallel = false;
   ProcessSignal* m_pProcessSignal = nullptr;
   Backprojector* m_pBackprojector = nullptr;
   int m_iTrace = 0;
