@@ -57,6 +57,92 @@ __device__ void stockham(Cplx<T>* z, const Cplx<T>* tw, int N, int logN, int dir
   }
 }
 
+// Twiddle w^m = exp(dir * 2 pi i m / N) for 0 <= m < N from the half table
+// (w^(m + N/2) = -w^m).
+template <class T>
+__device__ __forceinline__ Cplx<T> twiddle(const Cplx<T>* tw, int m, int half, int dir) {
+  Cplx<T> w = tw[m < half ? m : m - half];
+  if (m >= half) { w.x = -w.x; w.y = -w.y; }
+  if (dir > 0) w.y = -w.y;
+  return w;
+}
+
+// The same transform as stockham() with radix-4 passes (one radix-2 pass
+// first when log2 N is odd): half the LDS round trips and barriers.
+// Radix-R Stockham step for butterfly t: k = t mod Ls, inputs z[t + r N/R]
+// times w^(r k N/(R Ls)), an R-point DFT, outputs at (t - k) R + k + r Ls.
+template <class T, int R>
+__device__ void stockham4(Cplx<T>* z, const Cplx<T>* tw, int N, int logN, int dir) {
+  const int half = N >> 1, quarter = N >> 2;
+  int Ls = 1, s = 0;
+  if (logN & 1) {
+    Cplx<T> ya[R], yb[R];
+    int idx[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int t = threadIdx.x + r * blockDim.x;
+      idx[r] = -1;
+      if (t < half) {
+        const Cplx<T> a = z[t], b = z[t + half];
+        ya[r] = {a.x + b.x, a.y + b.y};
+        yb[r] = {a.x - b.x, a.y - b.y};
+        idx[r] = t << 1;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (idx[r] >= 0) {
+        z[idx[r]] = ya[r];
+        z[idx[r] + 1] = yb[r];
+      }
+    }
+    __syncthreads();
+    Ls = 2;
+    s = 1;
+  }
+  constexpr int R4 = (R + 1) / 2;  // radix-4 butterflies per lane
+  for (; s < logN; s += 2, Ls <<= 2) {
+    Cplx<T> y0[R4], y1[R4], y2[R4], y3[R4];
+    int idx[R4];
+    const int stride = quarter / Ls;  // N / (4 Ls)
+#pragma unroll
+    for (int r = 0; r < R4; r++) {
+      const int t = threadIdx.x + r * blockDim.x;
+      idx[r] = -1;
+      if (t < quarter) {
+        const int k = t & (Ls - 1);
+        Cplx<T> a0 = z[t], a1 = z[t + quarter], a2 = z[t + 2 * quarter], a3 = z[t + 3 * quarter];
+        if (Ls > 1) {
+          a1 = cmul(twiddle(tw, k * stride, half, dir), a1);
+          a2 = cmul(twiddle(tw, 2 * k * stride, half, dir), a2);
+          a3 = cmul(twiddle(tw, 3 * k * stride, half, dir), a3);
+        }
+        // 4-point DFT; the forward transform multiplies a1 - a3 by -i
+        const Cplx<T> s02 = {a0.x + a2.x, a0.y + a2.y}, d02 = {a0.x - a2.x, a0.y - a2.y};
+        const Cplx<T> s13 = {a1.x + a3.x, a1.y + a3.y}, d13 = {a1.x - a3.x, a1.y - a3.y};
+        const Cplx<T> jd = dir < 0 ? Cplx<T>{d13.y, -d13.x} : Cplx<T>{-d13.y, d13.x};
+        y0[r] = {s02.x + s13.x, s02.y + s13.y};
+        y2[r] = {s02.x - s13.x, s02.y - s13.y};
+        y1[r] = {d02.x + jd.x, d02.y + jd.y};
+        y3[r] = {d02.x - jd.x, d02.y - jd.y};
+        idx[r] = ((t - k) << 2) + k;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R4; r++) {
+      if (idx[r] >= 0) {
+        z[idx[r]] = y0[r];
+        z[idx[r] + Ls] = y1[r];
+        z[idx[r] + 2 * Ls] = y2[r];
+        z[idx[r] + 3 * Ls] = y3[r];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Filters rows va and vb (vb < 0: none) of `in` (n cells each, stride n)
 // with one complex transform z = x_a + i x_b of length N (a power of two
 // >= n): optional FP64 pre-weights w, forward FFT, the two real spectra split
@@ -85,7 +171,7 @@ __device__ void fft_filter_pair(const float* __restrict__ in, const double* __re
     z[j] = {a, b};
   }
   __syncthreads();
-  stockham<T, R>(z, tw, N, logN, -1);
+  stockham4<T, R>(z, tw, N, logN, -1);
   // Split the two real spectra, apply the multipliers, recombine.
   for (int k = threadIdx.x; k <= N / 2; k += blockDim.x) {
     const int kn = (N - k) & (N - 1);
@@ -103,7 +189,7 @@ __device__ void fft_filter_pair(const float* __restrict__ in, const double* __re
     if (kn != k) z[kn] = wn;
   }
   __syncthreads();
-  stockham<T, R>(z, tw, N, logN, +1);
+  stockham4<T, R>(z, tw, N, logN, +1);
 }
 
 }  // namespace dev

@@ -53,10 +53,11 @@ __device__ __forceinline__ CoefT<INTERP> interval_coef(const Row& row, int i, in
       a1 = static_cast<double>(row(n - 1)) - row(n - 2);
     } else {
       const double y0 = row(i - 1), y1 = row(i), y2 = row(i + 1), y3 = row(i + 2);
+      const double third = 1.0 / 3, sixth = 1.0 / 6;  // FP64, rounded once to FP32 below
       a0 = y1;
-      a1 = -y0 / 3 - y1 / 2 + y2 - y3 / 6;
-      a2 = (y0 + y2) / 2 - y1;
-      a3 = (y3 - y0) / 6 + (y1 - y2) / 2;
+      a1 = -y0 * third - y1 * 0.5 + y2 - y3 * sixth;
+      a2 = (y0 + y2) * 0.5 - y1;
+      a3 = (y3 - y0) * sixth + (y1 - y2) * 0.5;
     }
     return make_float4(static_cast<float>(a0), static_cast<float>(a1), static_cast<float>(a2),
                        static_cast<float>(a3));

@@ -53,10 +53,9 @@ def test_rebin_against_oracle(gpu_ctx, oracle_lib, geom, nDet, nView):
 def test_rebinned_reconstruction(gpu_ctx):
     """Reconstructor(bRebinToParallel) chain: a fan-beam scan of the Herman
     head rebinned to parallel and reconstructed on the parallel path matches
-    the direct fan-beam reconstruction up to a global scale (the rebinned
-    views span 2 pi while the parallel scale assumes pi, backprojectors.cpp:
-    272-273) and the extra interpolation: Herman's d of the scaled images
-    stays below 0.05."""
+    the direct fan-beam reconstruction: the best-fit scale is 1 within 5%
+    and Herman's d of the scaled images stays below 0.06 (measured 0.046; the extra
+    interpolation of the rebinning)."""
     import torch
     import ctsim_amd
     from ctsim_amd import fbp, workloads as W
@@ -79,4 +78,4 @@ def test_rebinned_reconstruction(gpu_ctx):
     scale = float((a * b).sum() / (b * b).sum())
     d = ctsim_amd.image_stats(gpu_ctx, fan, (img * scale).float().contiguous())[0]
     print(f"  rebinned vs fan-beam: scale {scale:.4f}, d={d:.3e}")
-    assert 0.4 < scale < 0.6 and d < 0.05
+    assert 0.95 < scale < 1.05 and d < 0.06
