@@ -209,22 +209,37 @@ def main():
     # the one given with --shape (profiling runs pin the timed run's shape)
     ps.filter(sino, f32, ctsim_amd.PREC_MIXED)
     if args.shape:
-        lanesY, P, split, chunk = (int(x) for x in args.shape.split(","))
-        bp.set_shape(lanesY, P, split, chunk)
-        shape = (lanesY, P, split, chunk, -1, float("nan"))
+        vals = [int(x) for x in args.shape.split(",")]
+        lanesY, P, split, chunk = vals[:4]
+        sym = vals[4] if len(vals) > 4 else 0
+        bp.set_shape(lanesY, P, split, chunk, sym)
+        shape = (lanesY, P, split, chunk, -1, sym, float("nan"))
     else:
         shape = bp.autotune(f32, ang)
     log(f"[bench] autotuned mixed shape: lanesY={shape[0]} P={shape[1]} split={shape[2]} "
-        f"chunk={shape[3]} window={shape[4]} ({shape[5]:.3f} ms)")
+        f"chunk={shape[3]} window={shape[4]} sym={shape[5]} ({shape[6]:.3f} ms)")
 
-    # --- mixed path: warmup, then exactly K timed steps
-    for _ in range(args.warmup):
+    # --- mixed path as separate steps (zero, filter, backproject, scale)
+    for _ in range(2):
         fbp(ctsim_amd.PREC_MIXED, f32, img32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        fbp(ctsim_amd.PREC_MIXED, f32, img32)
+    torch.cuda.synchronize()
+    t_sep = (time.perf_counter() - t0) / 5
+    log(f"[bench] mixed path as separate steps: {t_sep * 1e3:.3f} ms/slice")
+
+    # --- mixed path, one call per slice (ctsim_fbp_slice: the filter writes
+    # interval coefficients, the backprojection overwrites the scaled image):
+    # warmup, then exactly K timed steps
+    for _ in range(args.warmup):
+        bp.fbp(ps, sino, ang, img32, ctsim_amd.PREC_MIXED)
     torch.cuda.synchronize()
     ctx.profile(True)
     t0 = time.perf_counter()
     for k in range(args.steps):
-        fbp(ctsim_amd.PREC_MIXED, f32, img32)
+        bp.fbp(ps, sino, ang, img32, ctsim_amd.PREC_MIXED)
     torch.cuda.synchronize()
     t_mixed = (time.perf_counter() - t0) / args.steps
     kms, kn = ctx.profile_read()
@@ -262,7 +277,7 @@ def main():
                  "slices_per_s": args.batch / t_batch, "d": db[0],
                  "speedup_over_fp64": t_f64 * args.batch / t_batch,
                  "launch_shape": {"P": bshape[0], "lanesY": bshape[1], "chunk_bytes": bshape[2],
-                                  "fused": bool(bshape[3])}}
+                                  "fused": bool(bshape[3]), "symmetric": bool(bshape[4])}}
 
     pvu = spec.nx * spec.ny * spec.nView
     flops = FLOPS_PER_PVU_EQUIANGULAR_CUBIC_KAHAN * pvu
@@ -285,7 +300,8 @@ def main():
         "data": "synthetic (64 seeded ellipses, GPU-simulated sinogram)",
         "config": {"workload": "config 2: equiangular fan beam, 1024 det x 1440 views over 2pi, "
                                "abs_hanning by FFT (2048-point), cubic, 512x512",
-                   "slices_per_step": 1, "precision": "mixed"},
+                   "slices_per_step": 1, "precision": "mixed", "entry": "ctsim_fbp_slice"},
+        "mixed_separate_steps_ms": t_sep * 1e3,
         "fp64_gpu": {"slices_per_s": 1.0 / t_f64, "ms_per_slice": t_f64 * 1e3,
                      "backprojection_kernel_ms": k64_ms / k64_n,
                      "speedup_mixed_over_fp64": t_f64 / t_mixed},
@@ -300,7 +316,8 @@ def main():
                      "hbm_frac": hbm_bytes / (bp_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
                      "pvu_per_s": pvu / (bp_ms * 1e-3),
                      "launch_shape": {"lanesY": shape[0], "P": shape[1], "split": shape[2],
-                                      "chunk": shape[3], "window": shape[4]}},
+                                      "chunk": shape[3], "window": shape[4],
+                                      "symmetric": shape[5]}},
         "raysum": {"fp64_ms": t_raysum64 * 1e3, "closed_form_ms": t_raysum_m * 1e3,
                    "ray_element_pairs": n_re},
         "batched": batch,

@@ -30,6 +30,7 @@ SIGNATURES = {
     "ctsim_backprojector_destroy": (None, [_vp]),
     "ctsim_backproject_views": (_i, [_vp, _vp, _i, _vp, _vp, _i, _vp]),
     "ctsim_backproject_postprocess": (_i, [_vp, _vp, _vp]),
+    "ctsim_fbp_slice": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _i, _vp]),
     "ctsim_backprojector_info": (_i, [_vp, _vp]),
     "ctsim_procsignal_create": (_i, [_vp, _pp, _cp, _cp, _d, _d, _i, _d, _cp, _cp, _i, _i, _i,
                                      _d, _d]),
@@ -55,7 +56,7 @@ SIGNATURES = {
     "ctsim_batch_destroy": (None, [_vp]),
     "ctsim_batch_run": (_i, [_vp, _vp, _vp, _vp]),
     "ctsim_batch_status": (_i, [_vp, _vp]),
-    "ctsim_batch_set_shape": (_i, [_vp, _vp, _i, _i, _i, _i]),
+    "ctsim_batch_set_shape": (_i, [_vp, _vp, _i, _i, _i, _i, _i]),
     "ctsim_batch_autotune": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "ctsim_batch_info": (_i, [_vp, _vp]),
     "ctsim_helical180li": (_i, [_vp, _i, _i, _i, _d, _d, _d, _d, _i, _vp, _vp, _i, _vp, _vp]),

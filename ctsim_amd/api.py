@@ -223,13 +223,14 @@ class Batch:
     def status(self):
         check(lib.ctsim_batch_status(self.ctx.handle, self.handle), "ctsim_batch_status")
 
-    def set_shape(self, P: int, lanesY: int, chunkBytes: int, fused: bool):
+    def set_shape(self, P: int, lanesY: int, chunkBytes: int, fused: bool, sym: bool = True):
         check(lib.ctsim_batch_set_shape(self.ctx.handle, self.handle, P, lanesY, chunkBytes,
-                                        int(fused)), "ctsim_batch_set_shape")
+                                        int(fused), int(sym)), "ctsim_batch_set_shape")
 
     def autotune(self, sinos, images):
+        """Times the launch shapes; returns (P, lanesY, chunkBytes, fused, sym, ms)."""
         sp, ip = self._ptrs(sinos, images)
-        shape = np.zeros(4, np.int32)
+        shape = np.zeros(5, np.int32)
         ms = np.zeros(1, np.float64)
         check(lib.ctsim_batch_autotune(self.ctx.handle, self.handle, sp, ip, _hptr(shape),
                                        _hptr(ms)), "ctsim_batch_autotune")
@@ -294,7 +295,7 @@ class Backprojector:
 
     def autotune(self, filtered, angles: np.ndarray):
         """Times the MIXED kernel's launch shapes on these float32 rows and
-        keeps the fastest; returns (lanesY, P, split, chunk, window, ms)."""
+        keeps the fastest; returns (lanesY, P, split, chunk, window, sym, ms)."""
         import torch
         if filtered.dtype != torch.float32 or filtered.dim() != 2 or \
                 filtered.shape[1] != self.nDet * self.interpFactor:
@@ -302,7 +303,7 @@ class Backprojector:
         ang = np.ascontiguousarray(angles, dtype=np.float64)
         if ang.shape[0] < filtered.shape[0]:
             raise ValueError("not enough view angles")
-        shape = np.zeros(5, np.int32)
+        shape = np.zeros(6, np.int32)
         ms = np.zeros(1, np.float64)
         check(lib.ctsim_backprojector_autotune(self.ctx.handle, self.handle,
                                                C.c_void_p(_dptr(filtered)), _hptr(ang),
@@ -310,14 +311,34 @@ class Backprojector:
               "ctsim_backprojector_autotune")
         return (*map(int, shape), float(ms[0]))
 
-    def set_shape(self, lanesY: int, P: int, split: int, chunk: int):
-        s = np.array([lanesY, P, split, chunk], np.int32)
+    def set_shape(self, lanesY: int, P: int, split: int, chunk: int, sym: int = 0):
+        """Pins the MIXED launch shape; sym = 1 runs the rotationally symmetric
+        kernel when the image and the views allow it (P <= 2)."""
+        s = np.array([lanesY, P, split, chunk, sym], np.int32)
         check(lib.ctsim_backprojector_set_shape(self.handle, _hptr(s)), "ctsim_backprojector_set_shape")
 
     def postprocess(self, image):
         check(lib.ctsim_backproject_postprocess(self.ctx.handle, self.handle,
                                                 C.c_void_p(_dptr(image))),
               "ctsim_backproject_postprocess")
+
+    def fbp(self, ps: "ProcessSignal", sino, angles: np.ndarray, image,
+            precision: int = PREC_MIXED):
+        """One whole slice (ctsim_fbp_slice): ``image`` (device float32
+        [nx, ny]) is overwritten with the post-scaled reconstruction of every
+        row of ``sino`` (device float32 [nView, nDet]) filtered by ``ps``."""
+        import torch
+        if sino.dtype != torch.float32 or sino.dim() != 2 or sino.shape[1] != self.nDet:
+            raise TypeError("sino must be float32 [nView, nDet]")
+        if image.dtype != torch.float32 or tuple(image.shape) != (self.nx, self.ny):
+            raise TypeError("image must be float32 [nx, ny]")
+        ang = np.ascontiguousarray(angles, dtype=np.float64)
+        nView = sino.shape[0]
+        if ang.shape[0] < nView:
+            raise ValueError("not enough view angles")
+        check(lib.ctsim_fbp_slice(self.ctx.handle, ps.handle, self.handle, precision,
+                                  C.c_void_p(_dptr(sino)), _hptr(ang), nView,
+                                  C.c_void_p(_dptr(image))), "ctsim_fbp_slice")
 
     def close(self):
         if getattr(self, "handle", None):

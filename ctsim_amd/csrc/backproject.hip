@@ -20,6 +20,7 @@
 #include <cstring>
 
 #include "bp_plan.h"
+#include "filter_plan.h"
 #include "mixed_device.h"
 
 // FP64 parity kernels keep the reference's rounding: no contraction.
@@ -33,6 +34,7 @@ using dev::kMaxChunk;
 using dev::xcd_tile;
 // Dynamic LDS of the mixed kernel: 64 KB per workgroup less its static arrays.
 constexpr size_t kMixedLdsMax = 62 * 1024;
+constexpr size_t kSymLds = 62 * 1024;  // coefficient staging of the symmetric kernel
 
 struct BpArgs {
   int nx, ny, nDet, iDetCenter, nView, fstride;
@@ -394,16 +396,45 @@ __global__ __launch_bounds__(256) void bp_mixed_kernel(MixedArgs a) {
                                          s_lo);
 }
 
+// The rotationally symmetric variant: one workgroup per (quadrant tile, group
+// of base views), dealt to XCDs as above.
+template <int GK, int INTERP, int P, int DEG, int S, int K>
+__global__ __launch_bounds__(256) void bp_sym_kernel(MixedArgs a) {
+  extern __shared__ float4 s_dyn[];
+  __shared__ float2 s_ab[16];
+  __shared__ dev::SymWin s_win[64];
+  __shared__ int s_n;
+  const int nTiles = a.tilesX * a.tilesY;
+  int tile, grp;
+  if ((a.split & 7) == 0) {
+    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+    grp = xcd * (a.split >> 3) + local / nTiles;
+    tile = local % nTiles;
+  } else {
+    const int bid = dev::xcd_tile(blockIdx.x, gridDim.x);
+    tile = bid % nTiles;
+    grp = bid / nTiles;
+  }
+  dev::bp_sym_tile<GK, INTERP, P, DEG, S, K>(a, tile, grp, reinterpret_cast<CoefT<INTERP>*>(s_dyn),
+                                             s_ab, s_win, &s_n);
+}
+
 // image += endScale * (sum of the view groups' partial images), groups added
-// in order in FP64.
+// in order in FP64 (overwrite: image = endScale * sum).
 __global__ void bp_combine_kernel(float* __restrict__ image, const float* __restrict__ partial,
-                                  size_t n, int split, float endScale) {
+                                  size_t n, int split, double endScale, int overwrite) {
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     double s = 0;
     for (int g = 0; g < split; g++) s += partial[static_cast<size_t>(g) * n + i];
-    image[i] = static_cast<float>(static_cast<double>(image[i]) + endScale * s);
+    image[i] = static_cast<float>((overwrite ? 0.0 : static_cast<double>(image[i])) + endScale * s);
   }
+}
+
+__global__ void bp_fill_kernel(float* img, size_t n, float v) {
+  size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (; i < n; i += stride) img[i] = v;
 }
 
 #pragma clang fp contract(off)
@@ -467,6 +498,32 @@ void launch_mixed(int gk, int deg, int P, const MixedArgs& a, int grid, size_t l
     case 6: launch_mixed_p<1, INTERP, 6>(P, a, grid, lds, s); break;
     case 8: launch_mixed_p<1, INTERP, 8>(P, a, grid, lds, s); break;
     default: launch_mixed_p<1, INTERP, 0>(P, a, grid, lds, s); break;
+  }
+}
+
+template <int GK, int INTERP, int DEG, int S, int K>
+void launch_sym_p(int P, const MixedArgs& a, int grid, size_t lds, hipStream_t s) {
+  if (P == 1) hipLaunchKernelGGL((bp_sym_kernel<GK, INTERP, 1, DEG, S, K>), dim3(grid), dim3(256), lds, s, a);
+  else hipLaunchKernelGGL((bp_sym_kernel<GK, INTERP, 2, DEG, S, K>), dim3(grid), dim3(256), lds, s, a);
+}
+
+template <int GK, int INTERP, int DEG>
+void launch_sym_sk(int S, int K, int P, const MixedArgs& a, int grid, size_t lds, hipStream_t s) {
+  if (S == 4) launch_sym_p<GK, INTERP, DEG, 4, 1>(P, a, grid, lds, s);
+  else if (K == 1) launch_sym_p<GK, INTERP, DEG, 2, 1>(P, a, grid, lds, s);
+  else launch_sym_p<GK, INTERP, DEG, 2, 2>(P, a, grid, lds, s);
+}
+
+template <int INTERP>
+void launch_sym(int gk, int deg, int S, int K, int P, const MixedArgs& a, int grid, size_t lds,
+                hipStream_t s) {
+  if (gk == 0) return launch_sym_sk<0, INTERP, 0>(S, K, P, a, grid, lds, s);
+  if (gk == 2) return launch_sym_sk<2, INTERP, 0>(S, K, P, a, grid, lds, s);
+  switch (deg) {
+    case 4: launch_sym_sk<1, INTERP, 4>(S, K, P, a, grid, lds, s); break;
+    case 6: launch_sym_sk<1, INTERP, 6>(S, K, P, a, grid, lds, s); break;
+    case 8: launch_sym_sk<1, INTERP, 8>(S, K, P, a, grid, lds, s); break;
+    default: launch_sym_sk<1, INTERP, 0>(S, K, P, a, grid, lds, s); break;
   }
 }
 
@@ -538,10 +595,7 @@ int bp_launch(ctsim_ctx* ctx, const BpPlan& p, int precision, const void* d_filt
   // reads float filtered views.
   // A mixed launch whose full-row staging cannot fit one view in LDS (very
   // long rows with the source close to the image) also takes the index path.
-  const size_t mixedView = static_cast<size_t>(p.mixed.W > 0 ? p.mixed.W : p.nDet + 4) *
-                           (p.interp == INTERP_CUBIC ? sizeof(float4) : sizeof(float2));
-  const bool f64 = precision == CTSIM_PREC_F64 || p.kind == KIND_DIFF ||
-                   p.interp == INTERP_NEAREST || mixedView > kMixedLdsMax;
+  const bool f64 = precision == CTSIM_PREC_F64 || !bp_mixed_tiled(p);
   const bool inF32 = precision == CTSIM_PREC_MIXED;
   if (f64) {
     CTSIM_CHECK(view_consts<ViewConstF64>(
@@ -571,17 +625,42 @@ int bp_launch(ctsim_ctx* ctx, const BpPlan& p, int precision, const void* d_filt
     return CTSIM_OK;
   }
 
-  CTSIM_CHECK(view_consts<ViewConstF32>(
-      ctx, p, 1, h_angles, nView,
-      [&](ViewConstF32* o) { bp_view_consts_f32(p, h_angles, nView, o); }, &a.vc));
-  return bp_launch_mixed(ctx, p, p.mixed, static_cast<const float*>(d_filtered),
-                         static_cast<const ViewConstF32*>(a.vc), nView, d_image);
+  const ViewConstF32* vc32 = nullptr;
+  CTSIM_CHECK(bp_view_consts_mixed(ctx, p, h_angles, nView, &vc32));
+  return bp_launch_mixed(ctx, p, p.mixed, static_cast<const float*>(d_filtered), vc32, nView,
+                         d_image);
+}
+
+// The symmetric kernel applies to a square centred grid, views with a
+// rotational symmetry (of the angles last uploaded, which must be these
+// nView), P <= 2 and windows small enough that one base view's 4*S windows
+// fit the LDS it stages into.
+bool bp_sym_ok(const BpPlan& p, const MixedShape& m, int nView) {
+  const SymViews& sv = p.cachedSym;
+  if (!p.symImage || sv.S == 0 || m.P > 2 || m.W <= 0 || static_cast<int>(p.cachedAngles.size()) != nView ||
+      p.cachedKind != 1)
+    return false;
+  const size_t elem = p.interp == INTERP_CUBIC ? sizeof(float4) : sizeof(float2);
+  return static_cast<size_t>(4 * sv.S) * m.W * elem <= kSymLds;
+}
+
+bool bp_mixed_tiled(const BpPlan& p) {
+  const size_t mixedView = static_cast<size_t>(p.mixed.W > 0 ? p.mixed.W : p.nDet + 4) *
+                           (p.interp == INTERP_CUBIC ? sizeof(float4) : sizeof(float2));
+  return p.kind != KIND_DIFF && p.interp != INTERP_NEAREST && mixedView <= kMixedLdsMax;
 }
 
 // The mixed path: interval coefficients of every view, then the tiled
 // kernel, then (with view groups) the ordered combine.
 int bp_launch_mixed(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const float* d_filtered,
                     const ViewConstF32* d_vc, int nView, float* d_image) {
+  CTSIM_CHECK(bp_mixed_coef(ctx, p, d_filtered, nView));
+  const double endScale = p.kind == KIND_TRIG ? p.rotScale : 1.0;
+  return bp_mixed_tiles(ctx, p, m, ctx->scratch2.ptr, d_vc, nView, d_image, endScale, false);
+}
+
+// Interval coefficients of nView filtered float rows into ctx->scratch2.
+int bp_mixed_coef(ctsim_ctx* ctx, const BpPlan& p, const float* d_filtered, int nView) {
   const hipStream_t s = ctx->stream;
   const bool cubic = p.interp == INTERP_CUBIC;
   const bool lastIncl = p.kind == KIND_IDIFF;
@@ -599,7 +678,19 @@ int bp_launch_mixed(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const 
   else
     hipLaunchKernelGGL((coef_kernel<INTERP_LINEAR, false>), dim3(cgrid), dim3(256), 0, s, d_filtered,
                        fstride, p.nDet, nView, perRow, ctx->scratch2.ptr);
+  CTSIM_HIP(hipGetLastError());
+  return CTSIM_OK;
+}
 
+// The tiled kernel (and, with view groups, the ordered combine) over
+// coefficients `coef`: image += endScale * sum, or image = endScale * sum
+// when overwrite.
+int bp_mixed_tiles(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const void* coef,
+                   const ViewConstF32* d_vc, int nView, float* d_image, double endScale,
+                   bool overwrite) {
+  const hipStream_t s = ctx->stream;
+  const bool cubic = p.interp == INTERP_CUBIC;
+  const size_t elem = cubic ? sizeof(float4) : sizeof(float2);
   MixedArgs a{};
   a.nx = p.nx; a.ny = p.ny; a.nDet = p.nDet; a.nView = nView;
   a.lanesY = m.lanesY;
@@ -614,7 +705,8 @@ int bp_launch_mixed(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const 
   a.center = static_cast<float>(p.iDetCenter);
   a.F = static_cast<float>(p.F);
   a.twoF = static_cast<float>(2 * p.F);
-  a.endScale = p.kind == KIND_TRIG ? static_cast<float>(p.rotScale) : 1.f;
+  a.endScale = static_cast<float>(endScale);
+  a.overwrite = overwrite ? 1 : 0;
   const int gk = p.geom == GEOM_PARALLEL ? 0 : (p.geom == GEOM_EQUIANGULAR ? 1 : 2);
   a.posScale = gk == 2 ? static_cast<float>(p.sdl / p.detInc) : static_cast<float>(1.0 / p.detInc);
   const int deg = gk == 1 ? p.atanDeg : 0;
@@ -623,27 +715,62 @@ int bp_launch_mixed(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const 
   a.xs = p.d_xy.as<float>();
   a.ys = p.d_xy.as<float>() + p.xs.size();
   a.vc = d_vc;
-  a.coef = ctx->scratch2.ptr;
+  a.coef = coef;
   a.image = d_image;
-  const size_t lds = static_cast<size_t>(a.chunk) * a.W * elem;
-  if (lds > kMixedLdsMax) return fail(CTSIM_ERR_UNSUPPORTED, "backprojector: detector row too long for LDS");
   const size_t npix = static_cast<size_t>(p.nx) * p.ny;
+  const SymViews sv = p.cachedSym;
+  const bool sym = m.sym && bp_sym_ok(p, m, nView);
+  if (sym) {
+    // tiles of the quadrant; groups and chunks of base views
+    a.half = p.nx / 2;
+    a.Q = nView / sv.S;
+    a.dirf = sv.dir;
+    a.tilesX = (a.half + tx - 1) / tx;
+    a.tilesY = (a.half + ty - 1) / ty;
+    a.split = m.split < 1 ? 1 : (m.split > a.Q ? a.Q : m.split);
+    a.viewsPerGroup = (a.Q + a.split - 1) / a.split;
+    a.split = (a.Q + a.viewsPerGroup - 1) / a.viewsPerGroup;
+    a.capW = static_cast<int>(kSymLds / elem);
+  }
+  const size_t lds = sym ? kSymLds : static_cast<size_t>(a.chunk) * a.W * elem;
+  if (lds > kMixedLdsMax) return fail(CTSIM_ERR_UNSUPPORTED, "backprojector: detector row too long for LDS");
   if (a.split > 1) {
     CTSIM_CHECK(ctx->scratch.reserve(static_cast<size_t>(a.split) * npix * sizeof(float)));
     a.partial = ctx->scratch.as<float>();
   }
   const int grid = a.tilesX * a.tilesY * a.split;
   ctx->prof_begin();
-  if (cubic) launch_mixed<INTERP_CUBIC>(gk, deg, P, a, grid, lds, s);
-  else launch_mixed<INTERP_LINEAR>(gk, deg, P, a, grid, lds, s);
+  if (sym) {
+    if (cubic) launch_sym<INTERP_CUBIC>(gk, deg, sv.S, sv.K, P, a, grid, lds, s);
+    else launch_sym<INTERP_LINEAR>(gk, deg, sv.S, sv.K, P, a, grid, lds, s);
+  } else if (cubic) {
+    launch_mixed<INTERP_CUBIC>(gk, deg, P, a, grid, lds, s);
+  } else {
+    launch_mixed<INTERP_LINEAR>(gk, deg, P, a, grid, lds, s);
+  }
   ctx->prof_end();
   if (a.split > 1) {
     int g = static_cast<int>((npix + 255) / 256);
     if (g > 4096) g = 4096;
     hipLaunchKernelGGL(bp_combine_kernel, dim3(g), dim3(256), 0, s, d_image, a.partial, npix, a.split,
-                       a.endScale);
+                       endScale, a.overwrite);
   }
   CTSIM_HIP(hipGetLastError());
+  return CTSIM_OK;
+}
+
+// Per-view FP32 constants of a mixed launch on the device (cached by angles).
+int bp_view_consts_mixed(ctsim_ctx* ctx, const BpPlan& p, const double* h_angles, int nView,
+                         const ViewConstF32** out) {
+  const void* vc = nullptr;
+  CTSIM_CHECK(view_consts<ViewConstF32>(
+      ctx, p, 1, h_angles, nView,
+      [&](ViewConstF32* o) {
+        bp_view_consts_f32(p, h_angles, nView, o);
+        p.cachedSym = bp_sym_views(h_angles, nView);
+      },
+      &vc));
+  *out = static_cast<const ViewConstF32*>(vc);
   return CTSIM_OK;
 }
 
@@ -657,10 +784,8 @@ int bp_autotune(ctsim_ctx* ctx, BpPlan& p, const float* d_filtered, const double
     if (bestMs) *bestMs = 0;
     return CTSIM_OK;
   }
-  const void* vc = nullptr;
-  CTSIM_CHECK(view_consts<ViewConstF32>(
-      ctx, p, 1, h_angles, nView,
-      [&](ViewConstF32* o) { bp_view_consts_f32(p, h_angles, nView, o); }, &vc));
+  const ViewConstF32* vc = nullptr;
+  CTSIM_CHECK(bp_view_consts_mixed(ctx, p, h_angles, nView, &vc));
   DevBuf img;
   CTSIM_CHECK(img.reserve(static_cast<size_t>(p.nx) * p.ny * sizeof(float)));
   hipEvent_t e0, e1;
@@ -670,39 +795,46 @@ int bp_autotune(ctsim_ctx* ctx, BpPlan& p, const float* d_filtered, const double
   MixedShape best = p.mixed;
   float bestT = 1e30f;
   int rc = CTSIM_OK;
-  for (int lanesY : {8, 16, 32}) {
-    for (int P : {1, 2, 4}) {
-      if (lanesY * P > 2 * p.ny && P > 1) continue;
-      const int tx = 256 / lanesY, ty = lanesY * P;
-      const int W = bp_window_bound(p, tx, ty);
-      const int Wv = W > 0 ? W : p.nDet + 4;
-      for (int split : {1, 2, 4, 8, 16}) {
-        if (split > 1 && nView / split < 8) continue;
-        for (int chunk : {4, 8, 16, 32}) {
-          if (static_cast<size_t>(chunk) * Wv * elem > kMixedLdsMax) continue;
-          MixedShape m;
-          m.lanesY = lanesY; m.P = P; m.split = split; m.chunk = chunk; m.W = W;
-          float ms = 0, tot = 0;
-          for (int rep = 0; rep < 3 && rc == CTSIM_OK; rep++) {
-            if ((rc = bp_launch_mixed(ctx, p, m, d_filtered, static_cast<const ViewConstF32*>(vc),
-                                      nView, img.as<float>())) != CTSIM_OK) break;
-            if (rep == 0) continue;  // warm-up
-            (void)hipEventRecord(e0, ctx->stream);
-            if ((rc = bp_launch_mixed(ctx, p, m, d_filtered, static_cast<const ViewConstF32*>(vc),
-                                      nView, img.as<float>())) != CTSIM_OK) break;
-            (void)hipEventRecord(e1, ctx->stream);
-            (void)hipEventSynchronize(e1);
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            tot += ms;
+  // candidates: the plain kernel over lanesY x P x split x chunk, then the
+  // symmetric kernel (when the views and the image allow it) over
+  // lanesY x P x split (of the base views) x base views per chunk
+  for (int sym = 0; sym <= 1 && rc == CTSIM_OK; sym++) {
+    for (int lanesY : {8, 16, 32}) {
+      for (int P : {1, 2, 4}) {
+        if (lanesY * P > 2 * p.ny && P > 1) continue;
+        const int tx = 256 / lanesY, ty = lanesY * P;
+        const int W = bp_window_bound(p, tx, ty);
+        const int Wv = W > 0 ? W : p.nDet + 4;
+        MixedShape m;
+        m.lanesY = lanesY; m.P = P; m.W = W; m.sym = sym;
+        if (sym && !bp_sym_ok(p, m, nView)) continue;
+        const int views = sym ? nView / p.cachedSym.S : nView;
+        for (int split : {1, 2, 4, 8, 16}) {
+          if (split > 1 && views / split < (sym ? 4 : 8)) continue;
+          for (int chunk : {2, 4, 8, 16, 32}) {
+            if (sym ? chunk > 16 : (chunk < 4 || static_cast<size_t>(chunk) * Wv * elem > kMixedLdsMax))
+              continue;
+            m.split = split; m.chunk = chunk;
+            float ms = 0, tot = 0;
+            for (int rep = 0; rep < 3 && rc == CTSIM_OK; rep++) {
+              if ((rc = bp_launch_mixed(ctx, p, m, d_filtered, vc, nView, img.as<float>())) != CTSIM_OK) break;
+              if (rep == 0) continue;  // warm-up
+              (void)hipEventRecord(e0, ctx->stream);
+              if ((rc = bp_launch_mixed(ctx, p, m, d_filtered, vc, nView, img.as<float>())) != CTSIM_OK) break;
+              (void)hipEventRecord(e1, ctx->stream);
+              (void)hipEventSynchronize(e1);
+              (void)hipEventElapsedTime(&ms, e0, e1);
+              tot += ms;
+            }
+            if (rc != CTSIM_OK) break;
+            if (tot / 2 < bestT) { bestT = tot / 2; best = m; }
           }
           if (rc != CTSIM_OK) break;
-          if (tot / 2 < bestT) { bestT = tot / 2; best = m; }
         }
         if (rc != CTSIM_OK) break;
       }
       if (rc != CTSIM_OK) break;
     }
-    if (rc != CTSIM_OK) break;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -771,32 +903,83 @@ int ctsim_backproject_postprocess(ctsim_ctx* ctx, const ctsim_backprojector* bp,
 }
 
 int ctsim_backprojector_autotune(ctsim_ctx* ctx, ctsim_backprojector* bp, const float* d_filtered,
-                                 const double* h_angles, int nView, int* shape5, double* bestMs) {
+                                 const double* h_angles, int nView, int* shape6, double* bestMs) {
   if (!ctx || !bp || !d_filtered || !h_angles)
     return fail(CTSIM_ERR_INVALID, "ctsim_backprojector_autotune: NULL argument");
   CTSIM_CHECK(ctx->activate());
   CTSIM_CHECK(bp_autotune(ctx, bp->plan, d_filtered, h_angles, nView, bestMs));
-  if (shape5) {
+  if (shape6) {
     const MixedShape& m = bp->plan.mixed;
-    shape5[0] = m.lanesY; shape5[1] = m.P; shape5[2] = m.split; shape5[3] = m.chunk; shape5[4] = m.W;
+    shape6[0] = m.lanesY; shape6[1] = m.P; shape6[2] = m.split; shape6[3] = m.chunk; shape6[4] = m.W;
+    shape6[5] = m.sym;
   }
   return CTSIM_OK;
 }
 
-int ctsim_backprojector_set_shape(ctsim_backprojector* bp, const int* shape4) {
-  if (!bp || !shape4) return fail(CTSIM_ERR_INVALID, "ctsim_backprojector_set_shape: NULL argument");
-  const int lanesY = shape4[0], P = shape4[1], split = shape4[2], chunk = shape4[3];
+int ctsim_backprojector_set_shape(ctsim_backprojector* bp, const int* shape5) {
+  if (!bp || !shape5) return fail(CTSIM_ERR_INVALID, "ctsim_backprojector_set_shape: NULL argument");
+  const int lanesY = shape5[0], P = shape5[1], split = shape5[2], chunk = shape5[3], sym = shape5[4];
   if (!(lanesY == 8 || lanesY == 16 || lanesY == 32 || lanesY == 64) || !(P == 1 || P == 2 || P == 4) ||
-      split < 1 || split > 64 || chunk < 1 || chunk > kMaxChunk)
+      split < 1 || split > 64 || chunk < 1 || chunk > kMaxChunk || (sym != 0 && sym != 1))
     return fail(CTSIM_ERR_INVALID, "ctsim_backprojector_set_shape: shape out of range");
   MixedShape m;
-  m.lanesY = lanesY; m.P = P; m.split = split; m.chunk = chunk;
+  m.lanesY = lanesY; m.P = P; m.split = split; m.chunk = chunk; m.sym = sym;
   m.W = bp_window_bound(bp->plan, 256 / lanesY, lanesY * P);
   const int Wv = m.W > 0 ? m.W : bp->plan.nDet + 4;
   if (static_cast<size_t>(chunk) * Wv * (bp->plan.interp == INTERP_CUBIC ? 16 : 8) > kMixedLdsMax)
     return fail(CTSIM_ERR_INVALID, "ctsim_backprojector_set_shape: staging exceeds 64 KB of LDS");
   bp->plan.mixed = m;
   return CTSIM_OK;
+}
+
+int ctsim_fbp_slice(ctsim_ctx* ctx, ctsim_procsignal* ps, const ctsim_backprojector* bp,
+                    int precision, const float* d_sino, const double* h_angles, int nView,
+                    float* d_image) {
+  if (!ctx || !ps || !bp || !d_image || (nView > 0 && (!d_sino || !h_angles)))
+    return fail(CTSIM_ERR_INVALID, "ctsim_fbp_slice: NULL argument");
+  if (precision != CTSIM_PREC_F64 && precision != CTSIM_PREC_MIXED)
+    return fail(CTSIM_ERR_INVALID, "ctsim_fbp_slice: unknown precision");
+  if (nView < 0) return fail(CTSIM_ERR_INVALID, "ctsim_fbp_slice: nView < 0");
+  FilterPlan& f = ps->plan;
+  const BpPlan& p = bp->plan;
+  if (f.n != p.nDet || f.n * f.preinterp != p.nDet * p.interpFactor)
+    return fail(CTSIM_ERR_INVALID, "ctsim_fbp_slice: filter and backprojector detector counts differ");
+  CTSIM_CHECK(ctx->activate());
+  const hipStream_t s = ctx->stream;
+  const size_t npix = static_cast<size_t>(p.nx) * p.ny;
+  const bool mixed = precision == CTSIM_PREC_MIXED;
+  if (!mixed || !bp_mixed_tiled(p) || p.interpFactor != 1 || nView == 0) {
+    // The separate steps (pjrec: zeroed image, filterSignal, BackprojectView
+    // per view, PostProcessing).
+    int g = static_cast<int>((npix + 255) / 256);
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(bp_fill_kernel, dim3(g), dim3(256), 0, s, d_image, npix, 0.f);
+    CTSIM_HIP(hipGetLastError());
+    if (nView == 0) return CTSIM_OK;
+    const int stride = f.n * f.preinterp;
+    CTSIM_CHECK(ctx->scratch3.reserve(static_cast<size_t>(nView) * stride *
+                                      (mixed ? sizeof(float) : sizeof(double))));
+    CTSIM_CHECK(filter_launch(ctx, f, precision, d_sino, nView, ctx->scratch3.ptr, stride));
+    CTSIM_CHECK(bp_launch(ctx, p, precision, ctx->scratch3.ptr, h_angles, nView, d_image));
+    return bp_launch_post(ctx, p, d_image);
+  }
+  // Mixed and tiled: the filter writes interval coefficients (FFT methods
+  // straight from LDS; convolution through float rows), then the tiled
+  // kernel overwrites the image with rotScale * sum.
+  const bool cubic = p.interp == INTERP_CUBIC;
+  const size_t elem = cubic ? sizeof(float4) : sizeof(float2);
+  if (filter_can_coef(f)) {
+    CTSIM_CHECK(ctx->scratch2.reserve(static_cast<size_t>(nView) * p.nDet * elem));
+    CTSIM_CHECK(filter_launch_coef(ctx, f, d_sino, nView, ctx->scratch2.ptr, p.interp,
+                                   p.kind == KIND_IDIFF));
+  } else {
+    CTSIM_CHECK(ctx->scratch3.reserve(static_cast<size_t>(nView) * f.n * sizeof(float)));
+    CTSIM_CHECK(filter_launch(ctx, f, precision, d_sino, nView, ctx->scratch3.ptr, f.n));
+    CTSIM_CHECK(bp_mixed_coef(ctx, p, ctx->scratch3.as<float>(), nView));
+  }
+  const ViewConstF32* vc = nullptr;
+  CTSIM_CHECK(bp_view_consts_mixed(ctx, p, h_angles, nView, &vc));
+  return bp_mixed_tiles(ctx, p, p.mixed, ctx->scratch2.ptr, vc, nView, d_image, p.rotScale, true);
 }
 
 int ctsim_backprojector_info(const ctsim_backprojector* bp, double* o) {

@@ -53,6 +53,7 @@ struct SliceDev {
   const Cplx<float>* tw;
   // backprojection
   int gk, interp, lastIncl, deg;
+  int sym, symS, symK;  // rotationally symmetric body (tiles of the quadrant)
   MixedArgs bp;  // bp.coef, bp.image, bp.vc set; overwrite = 1, split = 1
 };
 
@@ -89,6 +90,27 @@ __device__ void bp_item(const MixedArgs& a, int gk, int interp, int deg, int til
   }
 }
 
+// The symmetric body for slices whose views come in rotated copies.
+template <int GK, int INTERP, int P, int DEG>
+__device__ void bp_sym_sk(const SliceDev& s, int tile, void* smem, float2* s_ab, dev::SymWin* s_win,
+                          int* s_n) {
+  auto* w = static_cast<CoefT<INTERP>*>(smem);
+  if (s.symS == 4) dev::bp_sym_tile<GK, INTERP, P, DEG, 4, 1>(s.bp, tile, 0, w, s_ab, s_win, s_n);
+  else if (s.symK == 1) dev::bp_sym_tile<GK, INTERP, P, DEG, 2, 1>(s.bp, tile, 0, w, s_ab, s_win, s_n);
+  else dev::bp_sym_tile<GK, INTERP, P, DEG, 2, 2>(s.bp, tile, 0, w, s_ab, s_win, s_n);
+}
+
+template <int P, int INTERP>
+__device__ void bp_sym_interp(const SliceDev& s, int tile, void* smem, float2* s_ab, dev::SymWin* s_win,
+                              int* s_n) {
+  if (s.gk == 0) bp_sym_sk<0, INTERP, P, 0>(s, tile, smem, s_ab, s_win, s_n);
+  else if (s.gk == 2) bp_sym_sk<2, INTERP, P, 0>(s, tile, smem, s_ab, s_win, s_n);
+  else if (s.deg == 4) bp_sym_sk<1, INTERP, P, 4>(s, tile, smem, s_ab, s_win, s_n);
+  else if (s.deg == 6) bp_sym_sk<1, INTERP, P, 6>(s, tile, smem, s_ab, s_win, s_n);
+  else if (s.deg == 8) bp_sym_sk<1, INTERP, P, 8>(s, tile, smem, s_ab, s_win, s_n);
+  else bp_sym_sk<1, INTERP, P, 0>(s, tile, smem, s_ab, s_win, s_n);
+}
+
 template <int R>
 __device__ void filter_item(const SliceDev& s, int pair, Cplx<float>* z, Cplx<float>* tw) {
   const int va = 2 * pair, vb = va + 1 < s.nView ? va + 1 : -1;
@@ -118,15 +140,26 @@ __device__ void filter_item(const SliceDev& s, int pair, Cplx<float>* z, Cplx<fl
 }
 
 // counters: [0, 8) queue heads, [8, 8 + nSlices) filter items done per slice.
-template <int P>
+// SYM instantiates the symmetric body (P <= 2); the plain kernel leaves it
+// out and keeps its own register budget.
+template <int P, bool SYM>
 __global__ __launch_bounds__(256) void fbp_batch_kernel(const SliceDev* __restrict__ slices,
                                                         const Item* __restrict__ items,
                                                         const int* __restrict__ queueStart,
                                                         int* counters, int* error) {
   extern __shared__ float4 smem[];
-  __shared__ float4 s_v[dev::kMaxChunk + 2];
+  // s_meta holds the plain body's per-view constants (kMaxChunk + 2 float4)
+  // or the symmetric body's 64 windows and 16 view constants, so the static
+  // LDS stays under 1.5 KB beside up to 62 KB of staging
+  static_assert(sizeof(dev::SymWin) == 16, "window record is one float4");
+  constexpr int kMeta = (dev::kMaxChunk + 2) > 72 ? (dev::kMaxChunk + 2) : 72;
+  __shared__ float4 s_meta[kMeta];
   __shared__ int s_lo[dev::kMaxChunk];
+  __shared__ int s_n;
   __shared__ int s_item;
+  float4* s_v = s_meta;
+  dev::SymWin* s_win = reinterpret_cast<dev::SymWin*>(s_meta);
+  float2* s_ab = reinterpret_cast<float2*>(s_meta + 64);
   const int q = blockIdx.x & 7;
   const int qBegin = queueStart[q], qEnd = queueStart[q + 1];
   int* done = counters + 8;
@@ -164,7 +197,16 @@ __global__ __launch_bounds__(256) void fbp_batch_kernel(const SliceDev* __restri
         __threadfence();  // acquire: the coefficients are visible below
       }
       __syncthreads();
-      bp_item<P>(s.bp, s.gk, s.interp, s.deg, it.a, smem, s_v, s_lo);
+      if constexpr (SYM && P <= 2) {
+        if (s.sym) {
+          if (s.interp == INTERP_CUBIC) bp_sym_interp<P, INTERP_CUBIC>(s, it.a, smem, s_ab, s_win, &s_n);
+          else bp_sym_interp<P, INTERP_LINEAR>(s, it.a, smem, s_ab, s_win, &s_n);
+        } else {
+          bp_item<P>(s.bp, s.gk, s.interp, s.deg, it.a, smem, s_v, s_lo);
+        }
+      } else {
+        bp_item<P>(s.bp, s.gk, s.interp, s.deg, it.a, smem, s_v, s_lo);
+      }
     }
     __syncthreads();  // LDS is reused by the next item
   }
@@ -209,7 +251,7 @@ struct ctsim_batch {
   };
   std::vector<Slice> s;
   int P = 2, lanesY = 16, chunkBudget = 32 * 1024;
-  bool fused = true;
+  bool fused = true, sym = true;
   int ldsFilter = 0;
   std::vector<SliceDev> host;
   DevBuf d_slices, d_itemsF, d_itemsB, d_itemsAll, d_qF, d_qB, d_qAll, d_counters, d_error;
@@ -232,6 +274,13 @@ struct ctsim_batch {
 
 namespace ctsim {
 namespace {
+
+// The symmetric instantiation runs when some slice takes the symmetric body.
+bool batch_sym_kernel(const ctsim_batch& b) {
+  for (const SliceDev& d : b.host)
+    if (d.sym) return true;
+  return false;
+}
 
 int occupancy_grid(const void* kernel, size_t lds, int numCUs) {
   int perCU = 0;
@@ -286,7 +335,15 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
     m.chunk = chunk;
     if (static_cast<size_t>(chunk) * Wv * elem > 62 * 1024)
       return fail(CTSIM_ERR_UNSUPPORTED, "batch: a slice's detector window does not fit in LDS");
-    maxBpLds = std::max(maxBpLds, static_cast<size_t>(chunk) * Wv * elem);
+    // the symmetric body when the views come in rotated copies and one base
+    // view's 4*S windows fit the LDS (at least the chunk budget, at most 62 KB)
+    const SymViews sv = bp_sym_views(sl.angles.data(), p.nView);
+    const size_t symNeed = static_cast<size_t>(4 * sv.S) * Wv * elem;
+    const size_t symLds = std::max(static_cast<size_t>(b.chunkBudget), symNeed);
+    d.sym = b.sym && b.P <= 2 && p.symImage && sv.S > 0 && m.W > 0 && symLds <= 62 * 1024;
+    d.symS = sv.S;
+    d.symK = sv.K;
+    maxBpLds = std::max(maxBpLds, d.sym ? symLds : static_cast<size_t>(chunk) * Wv * elem);
     MixedArgs& a = d.bp;
     a = MixedArgs{};
     a.nx = p.nx; a.ny = p.ny; a.nDet = p.nDet; a.nView = p.nView;
@@ -311,6 +368,16 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
     a.coef = sl.coef.ptr;
     a.image = b.images.empty() ? nullptr : b.images[k];
     a.overwrite = 1;
+    if (d.sym) {
+      a.half = p.nx / 2;
+      a.Q = p.nView / sv.S;
+      a.dirf = sv.dir;
+      a.tilesX = (a.half + tx - 1) / tx;
+      a.tilesY = (a.half + ty - 1) / ty;
+      a.viewsPerGroup = a.Q;
+      a.chunk = std::min(m.chunk, 16);
+      a.capW = static_cast<int>(symLds / elem);
+    }
   }
   // slices to XCDs, longest first
   std::vector<int> order(S);
@@ -360,9 +427,12 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
   CTSIM_HIP(hipMemset(b.d_error.ptr, 0, sizeof(int)));
   b.ldsB = maxBpLds;
   b.ldsAll = std::max(maxBpLds, static_cast<size_t>(b.ldsFilter));
-  const void* kern = b.P == 4 ? reinterpret_cast<const void*>(fbp_batch_kernel<4>)
-                              : (b.P == 1 ? reinterpret_cast<const void*>(fbp_batch_kernel<1>)
-                                          : reinterpret_cast<const void*>(fbp_batch_kernel<2>));
+  const bool symKernel = batch_sym_kernel(b);
+  const void* kern = b.P == 4 ? reinterpret_cast<const void*>(fbp_batch_kernel<4, false>)
+                     : b.P == 1 ? (symKernel ? reinterpret_cast<const void*>(fbp_batch_kernel<1, true>)
+                                             : reinterpret_cast<const void*>(fbp_batch_kernel<1, false>))
+                                : (symKernel ? reinterpret_cast<const void*>(fbp_batch_kernel<2, true>)
+                                             : reinterpret_cast<const void*>(fbp_batch_kernel<2, false>));
   b.gridF = occupancy_grid(kern, b.ldsFilter, ctx->numCUs);
   b.gridB = occupancy_grid(kern, b.ldsB, ctx->numCUs);
   b.gridAll = occupancy_grid(kern, b.ldsAll, ctx->numCUs);
@@ -374,20 +444,20 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
   return CTSIM_OK;
 }
 
-template <int P>
+template <int P, bool SYM>
 void launch_batch(ctsim_batch& b, hipStream_t st) {
   const int S = static_cast<int>(b.s.size());
   hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8 + S);
   if (b.fused) {
-    hipLaunchKernelGGL((fbp_batch_kernel<P>), dim3(b.gridAll), dim3(256), b.ldsAll, st,
+    hipLaunchKernelGGL((fbp_batch_kernel<P, SYM>), dim3(b.gridAll), dim3(256), b.ldsAll, st,
                        b.d_slices.as<SliceDev>(), b.d_itemsAll.as<Item>(), b.d_qAll.as<int>(),
                        b.d_counters.as<int>(), b.d_error.as<int>());
   } else {
-    hipLaunchKernelGGL((fbp_batch_kernel<P>), dim3(b.gridF), dim3(256), b.ldsFilter, st,
+    hipLaunchKernelGGL((fbp_batch_kernel<P, SYM>), dim3(b.gridF), dim3(256), b.ldsFilter, st,
                        b.d_slices.as<SliceDev>(), b.d_itemsF.as<Item>(), b.d_qF.as<int>(),
                        b.d_counters.as<int>(), b.d_error.as<int>());
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8);
-    hipLaunchKernelGGL((fbp_batch_kernel<P>), dim3(b.gridB), dim3(256), b.ldsB, st,
+    hipLaunchKernelGGL((fbp_batch_kernel<P, SYM>), dim3(b.gridB), dim3(256), b.ldsB, st,
                        b.d_slices.as<SliceDev>(), b.d_itemsB.as<Item>(), b.d_qB.as<int>(),
                        b.d_counters.as<int>(), b.d_error.as<int>());
   }
@@ -398,9 +468,10 @@ int batch_enqueue(ctsim_ctx* ctx, ctsim_batch& b) {
   if (!b.graph) {
     hipGraph_t g = nullptr;
     CTSIM_HIP(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
-    if (b.P == 4) launch_batch<4>(b, b.stream);
-    else if (b.P == 1) launch_batch<1>(b, b.stream);
-    else launch_batch<2>(b, b.stream);
+    const bool sym = batch_sym_kernel(b);
+    if (b.P == 4) launch_batch<4, false>(b, b.stream);
+    else if (b.P == 1) sym ? launch_batch<1, true>(b, b.stream) : launch_batch<1, false>(b, b.stream);
+    else sym ? launch_batch<2, true>(b, b.stream) : launch_batch<2, false>(b, b.stream);
     const hipError_t le = hipGetLastError();
     const hipError_t ce = hipStreamEndCapture(b.stream, &g);
     if (le != hipSuccess) return hip_fail(le, "batch launch", __FILE__, __LINE__);
@@ -509,7 +580,7 @@ int ctsim_batch_create(ctsim_ctx* ctx, ctsim_batch** out, const ctsim_slice_desc
     b->pvu += pvu;
   }
   if (rc == CTSIM_OK) {
-    b->ldsFilter = static_cast<int>(1.5 * maxN * sizeof(Cplx<float>));
+    b->ldsFilter = static_cast<int>((maxN + dev::fft_tw_entries<float>(maxN)) * sizeof(Cplx<float>));
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->evIn, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->evOut, hipEventDisableTiming) != hipSuccess)
@@ -530,7 +601,8 @@ void ctsim_batch_destroy(ctsim_batch* b) {
   delete b;
 }
 
-int ctsim_batch_set_shape(ctsim_ctx* ctx, ctsim_batch* b, int P, int lanesY, int chunkBytes, int fused) {
+int ctsim_batch_set_shape(ctsim_ctx* ctx, ctsim_batch* b, int P, int lanesY, int chunkBytes, int fused,
+                          int sym) {
   if (!ctx || !b) return fail(CTSIM_ERR_INVALID, "ctsim_batch_set_shape: NULL argument");
   if (!(P == 1 || P == 2 || P == 4) || !(lanesY == 8 || lanesY == 16 || lanesY == 32) ||
       chunkBytes < 1024 || chunkBytes > 62 * 1024)
@@ -541,6 +613,7 @@ int ctsim_batch_set_shape(ctsim_ctx* ctx, ctsim_batch* b, int P, int lanesY, int
   b->lanesY = lanesY;
   b->chunkBudget = chunkBytes;
   b->fused = fused != 0;
+  b->sym = sym != 0;
   return batch_layout(ctx, *b);
 }
 
@@ -577,21 +650,24 @@ int ctsim_batch_status(ctsim_ctx* ctx, ctsim_batch* b) {
 }
 
 int ctsim_batch_autotune(ctsim_ctx* ctx, ctsim_batch* b, const float* const* d_sinos,
-                         float* const* d_images, int* shape4, double* bestMs) {
+                         float* const* d_images, int* shape5, double* bestMs) {
   if (!ctx || !b) return fail(CTSIM_ERR_INVALID, "ctsim_batch_autotune: NULL argument");
   CTSIM_CHECK(ctx->activate());
   hipEvent_t e0, e1;
   CTSIM_HIP(hipEventCreate(&e0));
   CTSIM_HIP(hipEventCreate(&e1));
   float best = 1e30f;
-  int bP = b->P, bL = b->lanesY, bC = b->chunkBudget, bF = b->fused;
+  int bP = b->P, bL = b->lanesY, bC = b->chunkBudget, bF = b->fused, bS = b->sym;
   int rc = CTSIM_OK;
-  for (int P : {2, 4})
+  // P = 1 only with the symmetric body, P = 4 only without (it has none)
+  for (int P : {1, 2, 4})
+    for (int sym : {1, 0})
     for (int lanesY : {8, 16, 32})
       for (int chunk : {16 * 1024, 32 * 1024})
         for (int fused : {1, 0}) {
           if (rc != CTSIM_OK) break;
-          if ((rc = ctsim_batch_set_shape(ctx, b, P, lanesY, chunk, fused)) != CTSIM_OK) {
+          if ((P == 1 && !sym) || (P == 4 && sym)) continue;
+          if ((rc = ctsim_batch_set_shape(ctx, b, P, lanesY, chunk, fused, sym)) != CTSIM_OK) {
             if (rc == CTSIM_ERR_UNSUPPORTED) { rc = CTSIM_OK; continue; }
             break;
           }
@@ -612,14 +688,14 @@ int ctsim_batch_autotune(ctsim_ctx* ctx, ctsim_batch* b, const float* const* d_s
           }
           if (rc == CTSIM_OK && tot / 2 < best) {
             best = tot / 2;
-            bP = P; bL = lanesY; bC = chunk; bF = fused;
+            bP = P; bL = lanesY; bC = chunk; bF = fused; bS = sym;
           }
         }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != CTSIM_OK) return rc;
-  CTSIM_CHECK(ctsim_batch_set_shape(ctx, b, bP, bL, bC, bF));
-  if (shape4) { shape4[0] = bP; shape4[1] = bL; shape4[2] = bC; shape4[3] = bF; }
+  CTSIM_CHECK(ctsim_batch_set_shape(ctx, b, bP, bL, bC, bF, bS));
+  if (shape5) { shape5[0] = bP; shape5[1] = bL; shape5[2] = bC; shape5[3] = bF; shape5[4] = bS; }
   if (bestMs) *bestMs = best;
   return CTSIM_OK;
 }

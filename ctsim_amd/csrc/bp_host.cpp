@@ -175,8 +175,43 @@ int bp_plan_init(BpPlan& p, int geometry, int nDet, double detInc, int nView,
       if (tmax <= 1.0) fit_atan(p, tmax * tmax);
     }
   }
+  {
+    // the quarter turn (x, y) -> (-y, x) maps pixel (i, j) to (n-1-j, i)
+    // when ys == xs and xs[i] == -xs[n-1-i] (to a small fraction of a pixel)
+    bool sym = nx == ny && nx % 2 == 0 && nx >= 2;
+    const double tol = 1e-4 * fabs(p.xInc);
+    for (int i = 0; sym && i < nx; i++)
+      sym = fabs(static_cast<double>(p.xs[i]) + p.xs[nx - 1 - i]) <= tol &&
+            fabs(static_cast<double>(p.ys[i]) - p.xs[i]) <= tol;
+    p.symImage = sym;
+  }
   p.mixed = bp_default_shape(p, 256);
   return CTSIM_OK;
+}
+
+// Views v + s*Q turned by s quarter turns (S = 4, Q = nView / 4) or by s
+// quarter or half turns (S = 2), to 1e-9 rad for every view.
+SymViews bp_sym_views(const double* angles, int nView) {
+  SymViews out;
+  const double quarter = M_PI / 2, tol = 1e-9;
+  auto off = [](double d, double target) { return fabs(remainder(d - target, 2 * M_PI)); };
+  for (int S : {4, 2}) {
+    if (nView < S || nView % S != 0) continue;
+    const int Q = nView / S;
+    int k = 0;
+    for (int kk = 1; kk <= 3; kk++)
+      if (off(angles[Q] - angles[0], kk * quarter) < tol) k = kk;
+    if (k == 0 || (S == 4 && k == 2)) continue;
+    bool ok = true;
+    for (int v = 0; v < Q && ok; v++)
+      for (int s = 1; s < S && ok; s++) ok = off(angles[v + s * Q] - angles[v], s * k * quarter) < tol;
+    if (!ok) continue;
+    out.S = S;
+    out.K = k == 3 ? 1 : k;
+    out.dir = k == 3 ? -1.f : 1.f;
+    return out;
+  }
+  return out;
 }
 
 // Per-view constants of the FP64 path; the same expressions as

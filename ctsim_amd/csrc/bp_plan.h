@@ -34,8 +34,18 @@ struct ViewConstF32 {
 // (256 / lanesY) columns x (lanesY * P) rows; views are split into `split`
 // groups (separate workgroups whose partial sums are combined in a fixed
 // order) and staged `chunk` views at a time.
+// sym = 1 selects the rotationally symmetric kernel when the image and the
+// views allow it (P of 1 or 2; views staged up to chunk base views at a time).
 struct MixedShape {
-  int lanesY = 16, P = 4, split = 1, chunk = 16, W = 0;
+  int lanesY = 16, P = 4, split = 1, chunk = 16, W = 0, sym = 0;
+};
+
+// Rotational symmetry of a view set: views v + s*Q (s < S, Q = nView / S)
+// are view v turned by s*K quarter turns, counter-clockwise (dir = 1) or
+// clockwise (dir = -1); S = 0 when the views have no such symmetry.
+struct SymViews {
+  int S = 0, K = 0;
+  float dir = 1.f;
 };
 
 struct BpPlan {
@@ -57,6 +67,9 @@ struct BpPlan {
   // atanDeg (0: range-reduced generic polynomial) fitted on [0, tan^2 gmax].
   int atanDeg = 0;
   float atanCoef[9] = {};
+  // Square, even-sized pixel grid centred on the origin: the quarter turn
+  // maps pixel centres onto pixel centres.
+  bool symImage = false;
   MixedShape mixed;                 // current launch shape (tuned or default)
   bool mixedTuned = false;
   // Per-view constants of the last angle set, kept on the device so repeated
@@ -64,6 +77,7 @@ struct BpPlan {
   mutable std::vector<double> cachedAngles;
   mutable int cachedKind = -1;      // 0: FP64 constants, 1: FP32 constants
   mutable DevBuf d_vc;
+  mutable SymViews cachedSym;       // symmetry of the cached angles (FP32 constants)
 };
 
 // Host side (bp_host.cpp).
@@ -73,6 +87,7 @@ int bp_plan_init(BpPlan& p, int geometry, int nDet, double detInc, int nView,
 void bp_view_consts_f64(const BpPlan& p, const double* angles, int nView, ViewConstF64* out);
 void bp_view_consts_f32(const BpPlan& p, const double* angles, int nView, ViewConstF32* out);
 int bp_window_bound(const BpPlan& p, int tileX, int tileY);
+SymViews bp_sym_views(const double* angles, int nView);
 MixedShape bp_default_shape(const BpPlan& p, int numCUs);
 
 // Device side (backproject.hip).
@@ -82,6 +97,16 @@ int bp_launch(ctsim_ctx* ctx, const BpPlan& p, int precision, const void* d_filt
 int bp_launch_post(ctsim_ctx* ctx, const BpPlan& p, float* d_image);
 int bp_launch_mixed(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const float* d_filtered,
                     const ViewConstF32* d_vc, int nView, float* d_image);
+int bp_mixed_coef(ctsim_ctx* ctx, const BpPlan& p, const float* d_filtered, int nView);
+int bp_mixed_tiles(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const void* coef,
+                   const ViewConstF32* d_vc, int nView, float* d_image, double endScale,
+                   bool overwrite);
+int bp_view_consts_mixed(ctsim_ctx* ctx, const BpPlan& p, const double* h_angles, int nView,
+                         const ViewConstF32** out);
+// True when the mixed precision of this plan runs the tiled coefficient
+// kernel (not the FP64 index path) and one view's staging fits LDS.
+bool bp_mixed_tiled(const BpPlan& p);
+bool bp_sym_ok(const BpPlan& p, const MixedShape& m, int nView);
 
 }  // namespace ctsim
 

@@ -77,7 +77,8 @@ struct ctsim_ctx {
   ctsim::PinnedStage stage;
   ctsim::DevBuf viewConsts;   // per-view constants of the current launch
   ctsim::DevBuf scratch;      // kernel scratch (partials, counters)
-  ctsim::DevBuf scratch2;
+  ctsim::DevBuf scratch2;     // interval coefficients of the mixed path
+  ctsim::DevBuf scratch3;     // filtered views of ctsim_fbp_slice
   int numCUs = 256;
   int activate() const;       // hipSetDevice(device)
   // Kernel timing: while enabled, the main backprojection kernel of each

@@ -1,4 +1,4 @@
-// Radix-2 Stockham FFT in LDS and the two-view FFT filtering body, shared by
+// Radix-2/4 Stockham FFT in LDS and the two-view FFT filtering body, shared by
 // the single-slice filter kernel (filter.hip) and the batched persistent
 // kernel (batch.hip).
 #pragma once
@@ -143,19 +143,128 @@ __device__ void stockham4(Cplx<T>* z, const Cplx<T>* tw, int N, int logN, int di
   }
 }
 
+// Twiddles laid out per radix-4 pass (the passes of stockham4 with Ls > 1):
+// the pass with span Ls holds w^(j k N / (4 Ls)), j = 1, 2, 3, k < Ls, at
+// offset + (j - 1) Ls + k, offsets in pass order.  Lane k of a pass then
+// reads consecutive entries (no bank conflicts; lanes sharing k broadcast),
+// where the half table's index j k N / (4 Ls) strides across one bank.  At
+// most N - 2 entries.
+template <class T>
+__device__ void pass_twiddles(const Cplx<T>* __restrict__ twg, Cplx<T>* tw4, int N, int logN) {
+  const int half = N >> 1, quarter = N >> 2;
+  int off = 0;
+  for (int Ls = (logN & 1) ? 2 : 4; Ls <= quarter; Ls <<= 2) {
+    const int stride = quarter / Ls;
+    for (int e = threadIdx.x; e < 3 * Ls; e += blockDim.x) {
+      const int j = e / Ls + 1, k = e - (j - 1) * Ls;
+      tw4[off + e] = twiddle(twg, j * k * stride, half, -1);
+    }
+    off += 3 * Ls;
+  }
+}
+
+// stockham4 over the per-pass twiddle table of pass_twiddles.
+template <class T, int R>
+__device__ void stockham4p(Cplx<T>* z, const Cplx<T>* tw4, int N, int logN, int dir) {
+  const int half = N >> 1, quarter = N >> 2;
+  int Ls = 1, s = 0;
+  if (logN & 1) {
+    Cplx<T> ya[R], yb[R];
+    int idx[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int t = threadIdx.x + r * blockDim.x;
+      idx[r] = -1;
+      if (t < half) {
+        const Cplx<T> a = z[t], b = z[t + half];
+        ya[r] = {a.x + b.x, a.y + b.y};
+        yb[r] = {a.x - b.x, a.y - b.y};
+        idx[r] = t << 1;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (idx[r] >= 0) {
+        z[idx[r]] = ya[r];
+        z[idx[r] + 1] = yb[r];
+      }
+    }
+    __syncthreads();
+    Ls = 2;
+    s = 1;
+  }
+  constexpr int R4 = (R + 1) / 2;
+  int off = 0;
+  for (; s < logN; s += 2, Ls <<= 2) {
+    Cplx<T> y0[R4], y1[R4], y2[R4], y3[R4];
+    int idx[R4];
+#pragma unroll
+    for (int r = 0; r < R4; r++) {
+      const int t = threadIdx.x + r * blockDim.x;
+      idx[r] = -1;
+      if (t < quarter) {
+        const int k = t & (Ls - 1);
+        Cplx<T> a0 = z[t], a1 = z[t + quarter], a2 = z[t + 2 * quarter], a3 = z[t + 3 * quarter];
+        if (Ls > 1) {
+          Cplx<T> w1 = tw4[off + k], w2 = tw4[off + Ls + k], w3 = tw4[off + 2 * Ls + k];
+          if (dir > 0) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
+          a1 = cmul(w1, a1);
+          a2 = cmul(w2, a2);
+          a3 = cmul(w3, a3);
+        }
+        const Cplx<T> s02 = {a0.x + a2.x, a0.y + a2.y}, d02 = {a0.x - a2.x, a0.y - a2.y};
+        const Cplx<T> s13 = {a1.x + a3.x, a1.y + a3.y}, d13 = {a1.x - a3.x, a1.y - a3.y};
+        const Cplx<T> jd = dir < 0 ? Cplx<T>{d13.y, -d13.x} : Cplx<T>{-d13.y, d13.x};
+        y0[r] = {s02.x + s13.x, s02.y + s13.y};
+        y2[r] = {s02.x - s13.x, s02.y - s13.y};
+        y1[r] = {d02.x + jd.x, d02.y + jd.y};
+        y3[r] = {d02.x - jd.x, d02.y - jd.y};
+        idx[r] = ((t - k) << 2) + k;
+      }
+    }
+    if (Ls > 1) off += 3 * Ls;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R4; r++) {
+      if (idx[r] >= 0) {
+        z[idx[r]] = y0[r];
+        z[idx[r] + Ls] = y1[r];
+        z[idx[r] + 2 * Ls] = y2[r];
+        z[idx[r] + 3 * Ls] = y3[r];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Filters rows va and vb (vb < 0: none) of `in` (n cells each, stride n)
 // with one complex transform z = x_a + i x_b of length N (a power of two
 // >= n): optional FP64 pre-weights w, forward FFT, the two real spectra split
 // by Hermitian symmetry, real and imaginary parts multiplied by hre[k] and
 // him[k] (k <= N/2, 1/N folded in), recombined, inverse FFT.  On return the
 // filtered rows are z[j].x (va) and z[j].y (vb), j < n, in LDS; z and tw
-// (N/2 twiddles) are LDS arrays of the caller.
+// are LDS arrays of the caller, tw of fft_tw_entries<T>(N) twiddles: FP32
+// transforms up to 2048 points take the per-pass table (pass_twiddles),
+// longer ones and FP64 ones the half table (N/2 entries), which keeps a
+// 4096-point FP32 or a 2048-point FP64 transform within 48 KB of LDS.
+template <class T>
+__host__ __device__ constexpr bool fft_per_pass(int N) {
+  return sizeof(T) == 4 && N <= 2048;
+}
+template <class T>
+__host__ __device__ constexpr int fft_tw_entries(int N) {
+  return fft_per_pass<T>(N) ? N : N / 2;
+}
+
 template <class T, int R>
 __device__ void fft_filter_pair(const float* __restrict__ in, const double* __restrict__ w,
                                 const T* __restrict__ hre, const T* __restrict__ him,
                                 const Cplx<T>* __restrict__ twg, int va, int vb, int n, int N,
                                 int logN, Cplx<T>* z, Cplx<T>* tw) {
-  for (int k = threadIdx.x; k < N / 2; k += blockDim.x) tw[k] = twg[k];
+  const bool perPass = fft_per_pass<T>(N);
+  if (perPass) pass_twiddles(twg, tw, N, logN);
+  else for (int k = threadIdx.x; k < N / 2; k += blockDim.x) tw[k] = twg[k];
   for (int j = threadIdx.x; j < N; j += blockDim.x) {
     T a = 0, b = 0;
     if (j < n) {
@@ -171,7 +280,8 @@ __device__ void fft_filter_pair(const float* __restrict__ in, const double* __re
     z[j] = {a, b};
   }
   __syncthreads();
-  stockham4<T, R>(z, tw, N, logN, -1);
+  if (perPass) stockham4p<T, R>(z, tw, N, logN, -1);
+  else stockham4<T, R>(z, tw, N, logN, -1);
   // Split the two real spectra, apply the multipliers, recombine.
   for (int k = threadIdx.x; k <= N / 2; k += blockDim.x) {
     const int kn = (N - k) & (N - 1);
@@ -189,7 +299,8 @@ __device__ void fft_filter_pair(const float* __restrict__ in, const double* __re
     if (kn != k) z[kn] = wn;
   }
   __syncthreads();
-  stockham4<T, R>(z, tw, N, logN, +1);
+  if (perPass) stockham4p<T, R>(z, tw, N, logN, +1);
+  else stockham4<T, R>(z, tw, N, logN, +1);
 }
 
 }  // namespace dev

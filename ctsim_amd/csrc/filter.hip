@@ -15,6 +15,7 @@
 
 #include "filter_plan.h"
 #include "fft_device.h"
+#include "mixed_device.h"
 
 #pragma clang fp contract(off)
 
@@ -198,6 +199,30 @@ __global__ __launch_bounds__(256) void fft_filter_kernel(
 }
 
 
+// FFT filtering of two views whose output is the interval coefficients of
+// the backprojection's interpolation (mixed_device.h), formed straight from
+// the filtered rows in LDS: the fused single-slice FBP writes no filtered
+// rows.
+template <int INTERP, bool LAST_INCL, int R>
+__global__ __launch_bounds__(256) void fft_coef_kernel(
+    const float* __restrict__ in, const double* __restrict__ w, const float* __restrict__ hre,
+    const float* __restrict__ him, const Cplx<float>* __restrict__ twg, void* __restrict__ coef,
+    int nView, int n, int N, int logN) {
+  extern __shared__ unsigned char smem_raw[];
+  Cplx<float>* z = reinterpret_cast<Cplx<float>*>(smem_raw);
+  Cplx<float>* tw = z + N;
+  const int va = 2 * blockIdx.x, vb = va + 1 < nView ? va + 1 : -1;
+  dev::fft_filter_pair<float, R>(in, w, hre, him, twg, va, vb, n, N, logN, z, tw);
+  using CT = dev::CoefT<INTERP>;
+  CT* c = static_cast<CT*>(coef);
+  auto ra = [z](int j) { return z[j].x; };
+  auto rb = [z](int j) { return z[j].y; };
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    c[static_cast<size_t>(va) * n + j] = dev::interval_coef<INTERP, LAST_INCL>(ra, j, n);
+    if (vb >= 0) c[static_cast<size_t>(vb) * n + j] = dev::interval_coef<INTERP, LAST_INCL>(rb, j, n);
+  }
+}
+
 // ------------------------------------------- Toeplitz GEMM on the MFMA units
 // Mixed-precision convolution (procsignal.cpp:659-673 as a matrix product):
 //   Y[v][i] = sum_j X[v][j] * T[j][i],  T[j][i] = dx * h[i - j + n - 1] = g[j - i + n - 1]
@@ -326,7 +351,7 @@ int launch_fft(ctsim_ctx* ctx, const FilterPlan& p, const float* d_in, int nView
   if ((1 << logN) != N)
     return fail(CTSIM_ERR_UNSUPPORTED, "FFT filtering needs a power-of-two transform length "
                                        "(zeropad >= 1)");
-  const size_t lds = static_cast<size_t>(N) * sizeof(Cplx<T>) + static_cast<size_t>(N / 2) * sizeof(Cplx<T>);
+  const size_t lds = static_cast<size_t>(N + dev::fft_tw_entries<T>(N)) * sizeof(Cplx<T>);
   if (lds > 64 * 1024) return fail(CTSIM_ERR_UNSUPPORTED, "FFT filtering: transform too long for LDS");
   const T* hre = (sizeof(T) == 8 ? p.d_hmul64 : p.d_hmul32).template as<T>();
   const T* him = hre + (N / 2 + 1);
@@ -432,6 +457,49 @@ static int upload_dft_tables(FilterPlan& p) {
   std::vector<double>().swap(p.tfs);
   std::vector<double>().swap(p.tbc);
   std::vector<double>().swap(p.tbs);
+  return CTSIM_OK;
+}
+
+bool filter_can_coef(const FilterPlan& p) {
+  const int N = p.nfp;
+  return p.method != METHOD_CONVOLUTION && !p.hre.empty() && p.preinterp == 1 && N >= 2 && N <= 4096 &&
+         (N & (N - 1)) == 0;
+}
+
+int filter_launch_coef(ctsim_ctx* ctx, FilterPlan& p, const float* d_in, int nView, void* d_coef,
+                       int interp, bool lastIncl) {
+  if (nView <= 0) return CTSIM_OK;
+  if (!filter_can_coef(p))
+    return fail(CTSIM_ERR_UNSUPPORTED, "coefficient output needs a power-of-two FFT filter up to 4096 points");
+  const int N = p.nfp;
+  int logN = 0;
+  while ((1 << logN) < N) logN++;
+  const size_t lds = static_cast<size_t>(N + dev::fft_tw_entries<float>(N)) * sizeof(Cplx<float>);
+  const float* hre = p.d_hmul32.as<float>();
+  const float* him = hre + (N / 2 + 1);
+  const Cplx<float>* tw = p.d_tw32.as<Cplx<float>>();
+  const double* w = p.preweight.empty() ? nullptr : p.d_weight.as<double>();
+  const int grid = (nView + 1) / 2;
+  const int R = (N / 2 + 255) / 256;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, ctx->stream, d_in, w, hre, him, tw, d_coef,
+                       nView, p.n, N, logN);
+  };
+#define CTSIM_COEF_CASE(RR)                                                                   \
+  case RR:                                                                                    \
+    if (interp == INTERP_CUBIC) go(fft_coef_kernel<INTERP_CUBIC, false, RR>);                 \
+    else if (lastIncl) go(fft_coef_kernel<INTERP_LINEAR, true, RR>);                          \
+    else go(fft_coef_kernel<INTERP_LINEAR, false, RR>);                                        \
+    break;
+  switch (R) {
+    CTSIM_COEF_CASE(1)
+    CTSIM_COEF_CASE(2)
+    CTSIM_COEF_CASE(4)
+    CTSIM_COEF_CASE(8)
+    default: return fail(CTSIM_ERR_UNSUPPORTED, "FFT filtering: unsupported transform length");
+  }
+#undef CTSIM_COEF_CASE
+  CTSIM_HIP(hipGetLastError());
   return CTSIM_OK;
 }
 

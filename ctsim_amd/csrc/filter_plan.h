@@ -47,6 +47,14 @@ bool filter_is_fourier(const FilterPlan& p);
 int filter_upload(FilterPlan& p);
 int filter_launch(ctsim_ctx* ctx, FilterPlan& p, int precision, const float* d_in, int nView,
                   void* d_out, int outStride);
+// Mixed FFT filtering straight into backprojection interval coefficients
+// (float4 cubic / float2 linear, [nView][n]); CTSIM_ERR_UNSUPPORTED for
+// convolution filters or non-power-of-two transforms.
+int filter_launch_coef(ctsim_ctx* ctx, FilterPlan& p, const float* d_in, int nView, void* d_coef,
+                       int interp, bool lastIncl);
+// True when filter_launch_coef supports the plan: an FFT method with a
+// power-of-two transform of at most 4096 points and no pre-interpolation.
+bool filter_can_coef(const FilterPlan& p);
 
 }  // namespace ctsim
 

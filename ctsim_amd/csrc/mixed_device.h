@@ -83,6 +83,19 @@ struct MixedArgs {
   float* image;     // split == 1: image += endScale * sum (overwrite: image = endScale * sum)
   float* partial;   // split > 1: partial[group][pixel] = sum
   int overwrite;
+  // Rotational symmetry (bp_sym_tile): views v + s*Q (s < S) are view v
+  // turned by s*K quarter turns; tiles cover the quadrant [0, half)^2;
+  // capW coefficients of LDS; dirf = +1 (counter-clockwise steps) or -1.
+  int Q, half, capW;
+  float dirf;
+};
+
+// One window of staged coefficients in the symmetric kernel.
+struct SymWin {
+  int lo;        // first detector cell staged
+  int size;      // cells staged
+  int base;      // LDS offset of the window of row v (row v + s*Q at base + s*size)
+  float cshift;  // centre - lo
 };
 
 // atan on the whole line for FP32: odd polynomial t*P(t^2) on [0,1]
@@ -294,6 +307,235 @@ __device__ __forceinline__ void bp_mixed_tile(const MixedArgs& a, int tile, int 
 #pragma unroll
     for (int k = 0; k < P; k++)
       if (iyb + k < a.ny) out[base + k] = sum[k] - comp[k];
+  }
+}
+
+// Position and weight of orbit member m (pixel R^m (x, y), R the quarter
+// turn (x, y) -> (-y, x)) in one view, from A = x ca + y sa and
+// B = x sa - y ca of the base pixel: the member's r cos and F + r sin are
+// +-A, +-B and F -+ A, F -+ B.
+template <int GK, int DEG, int M>
+__device__ __forceinline__ void pos_member(const MixedArgs& a, float A, float B, float r2mF2,
+                                           float cshift, float& pos, float& wgt) {
+  const float rc = M == 0 ? A : (M == 1 ? B : (M == 2 ? -A : -B));
+  if constexpr (GK == 0) {
+    pos = rc + cshift;
+    wgt = 1.f;
+  } else {
+    const float fl = M == 0 ? a.F + B : (M == 1 ? a.F - A : (M == 2 ? a.F - B : a.F + A));
+    if constexpr (GK == 1) {
+      const float L2 = fmaf(a.twoF, fl, r2mF2);
+      const float rr = __builtin_amdgcn_rcpf(fl * L2);
+      pos = fan_angle<DEG>(a.atan, rc * (L2 * rr)) + cshift;
+      wgt = fl * rr;
+    } else {
+      const float q = __builtin_amdgcn_rcpf(fl);
+      pos = fmaf(rc * q, a.posScale, cshift);
+      const float u = a.F * q;
+      wgt = u * u;
+    }
+  }
+}
+
+// The symmetric body.  A pixel p of the quadrant and its images R p, R^2 p,
+// R^3 p form an orbit; when views v + s*Q are view v turned by s*K quarter
+// turns (s < S), pixel R^t p in view v + s*Q sees the detector exactly where
+// R^(t - sK) p sees it in view v.  So per base view v < Q a lane evaluates
+// four positions and weights (one per orbit member) and each serves S
+// updates: member m in row v + s*Q lands in orbit pixel (m + sK) mod 4.
+// With dirf = -1 the steps turn clockwise and the odd members swap roles.
+// Per chunk of base views, wave 0 computes each member tile's window
+// (corner positions, two cells of margin), lays the windows out in LDS by a
+// prefix sum and takes as many base views as fit in capW coefficients (one
+// always fits: capW >= 4*S*W); the workgroup stages rows v + s*Q of each
+// window; every lane then accumulates its 4P pixels in FP32 per chunk with a
+// Kahan fold across chunks.
+template <int GK, int INTERP, int P, int DEG, int S, int K>
+__device__ __forceinline__ void bp_sym_tile(const MixedArgs& a, int tile, int grp,
+                                            CoefT<INTERP>* s_w, float2* s_ab, SymWin* s_win,
+                                            int* s_n) {
+  using CT = CoefT<INTERP>;
+  static_assert(kMaxChunk >= 16, "window table holds 16 base views");
+  const int t = threadIdx.x;
+  const int h = a.half, n = 2 * a.half;
+  const int tx = 256 / a.lanesY, ty = a.lanesY * P;
+  const int ix0 = (tile / a.tilesY) * tx, iy0 = (tile % a.tilesY) * ty;
+  const int ix = ix0 + t / a.lanesY, iyb = iy0 + (t % a.lanesY) * P;
+  const int cx1 = min(ix0 + tx - 1, h - 1), cy1 = min(iy0 + ty - 1, h - 1);
+  const float x = a.xs[min(ix, h - 1)];
+  const float F2 = a.F * a.F;
+  float y[P], r2mF2[P];
+#pragma unroll
+  for (int k = 0; k < P; k++) {
+    y[k] = a.ys[min(iyb + k, h - 1)];
+    r2mF2[k] = fmaf(x, x, fmaf(y[k], y[k], -F2));
+  }
+  const int vBegin = grp * a.viewsPerGroup;
+  const int vEnd = min(a.Q, vBegin + a.viewsPerGroup);
+  const int maxChunk = min(a.chunk, 16);
+  float sum[4][P], comp[4][P];
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+#pragma unroll
+    for (int k = 0; k < P; k++) sum[m][k] = comp[m][k] = 0.f;
+  const CT* coef = static_cast<const CT*>(a.coef);
+  const float dirf = a.dirf;
+
+  for (int v0 = vBegin; v0 < vEnd;) {
+    const int cmax = min(maxChunk, vEnd - v0);
+    if (t < 64) {
+      // wave 0: lane (vv, m) = (t / 4, t % 4) computes member m's window in
+      // base view v0 + vv, then the lanes scan the staged sizes
+      const int vv = t >> 2, m = t & 3;
+      int lo = -2, size = 0;
+      if (t < 4 * cmax) {
+        const ViewConstF32 c = a.vc[v0 + vv];
+        if (t % 4 == 0) s_ab[vv] = make_float2(c.ca, c.sa);
+        size = a.W;
+        if (a.W < a.nDet + 4) {
+          const float xa = a.xs[ix0], xb = a.xs[cx1], ya = a.ys[iy0], yb = a.ys[cy1];
+          const float sg = (m & 1) ? dirf : 1.f;
+          float pmin = 3.0e38f, pmax = -3.0e38f;
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const float cx = (q & 1) ? xb : xa, cy = (q & 2) ? yb : ya;
+            const float A = sg * fmaf(cy, c.sa, cx * c.ca), B = sg * fmaf(-cy, c.ca, cx * c.sa);
+            const float r2 = fmaf(cx, cx, fmaf(cy, cy, -F2));
+            float pos, w;
+            switch (m) {
+              case 0: pos_member<GK, DEG, 0>(a, A, B, r2, a.center, pos, w); break;
+              case 1: pos_member<GK, DEG, 1>(a, A, B, r2, a.center, pos, w); break;
+              case 2: pos_member<GK, DEG, 2>(a, A, B, r2, a.center, pos, w); break;
+              default: pos_member<GK, DEG, 3>(a, A, B, r2, a.center, pos, w); break;
+            }
+            pmin = fminf(pmin, pos);
+            pmax = fmaxf(pmax, pos);
+          }
+          if (pmin > -1e9f && pmax < 1e9f) {
+            lo = static_cast<int>(floorf(pmin)) - 2;
+            size = min(static_cast<int>(floorf(pmax)) + 3 - lo, a.W);
+          } else {
+            lo = static_cast<int>(floorf(pmin > -1e9f && pmin < 1e9f ? pmin : 0.f)) - 2;
+          }
+        }
+      }
+      const int need = S * size;
+      int cum = need;  // inclusive prefix sum over the wave
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(cum, d, 64);
+        if (t >= d) cum += o;
+      }
+      // base view vv fits when the total through its last member does; the
+      // totals grow with the lane, so the fitting views are a prefix
+      const unsigned long long fit = __ballot(m == 3 && t < 4 * cmax && cum <= a.capW);
+      if (t < 4 * cmax) {
+        SymWin w;
+        w.lo = lo;
+        w.size = size;
+        w.base = cum - need;
+        w.cshift = a.center - static_cast<float>(lo);
+        s_win[t] = w;
+      }
+      if (t == 0) *s_n = max(1, __popcll(fit));
+    }
+    __syncthreads();
+    const int nv = *s_n;
+    {  // stage: wave w takes segments (vv, m, s) = w, w + 4, ...
+      const int wave = t >> 6, lane = t & 63;
+      for (int seg = wave; seg < nv * 4 * S; seg += 4) {
+        const int s = seg % S, vm = seg / S;
+        const SymWin w = s_win[vm];
+        const CT* crow = coef + static_cast<size_t>(v0 + (vm >> 2) + s * a.Q) * a.nDet;
+        CT* dst = s_w + w.base + s * w.size;
+        for (int j = lane; j < w.size; j += 64) {
+          const int d = w.lo + j;
+          CT val;
+          if (d >= 0 && d < a.nDet) val = crow[d];
+          else if constexpr (INTERP == INTERP_CUBIC) val = make_float4(0.f, 0.f, 0.f, 0.f);
+          else val = make_float2(0.f, 0.f);
+          dst[j] = val;
+        }
+      }
+    }
+    __syncthreads();
+    float part[4][P];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int k = 0; k < P; k++) part[m][k] = 0.f;
+    for (int vv = 0; vv < nv; vv++) {
+      const float2 ab = s_ab[vv];
+      const float xca = x * ab.x, xsa = x * ab.y;
+      SymWin w[4];
+#pragma unroll
+      for (int m = 0; m < 4; m++) w[m] = s_win[vv * 4 + m];
+#pragma unroll
+      for (int k = 0; k < P; k++) {
+        const float A = fmaf(y[k], ab.y, xca), B = fmaf(-y[k], ab.x, xsa);
+        const float Ao = dirf * A, Bo = dirf * B;
+        float u[4], wg[4];
+        int j[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          float pos;
+          const float Am = (m & 1) ? Ao : A, Bm = (m & 1) ? Bo : B;
+          if (m == 0) pos_member<GK, DEG, 0>(a, Am, Bm, r2mF2[k], w[m].cshift, pos, wg[m]);
+          else if (m == 1) pos_member<GK, DEG, 1>(a, Am, Bm, r2mF2[k], w[m].cshift, pos, wg[m]);
+          else if (m == 2) pos_member<GK, DEG, 2>(a, Am, Bm, r2mF2[k], w[m].cshift, pos, wg[m]);
+          else pos_member<GK, DEG, 3>(a, Am, Bm, r2mF2[k], w[m].cshift, pos, wg[m]);
+          u[m] = __builtin_amdgcn_fractf(pos);
+          j[m] = w[m].base +
+                 static_cast<int>(__builtin_amdgcn_fmed3f(pos, 0.f, static_cast<float>(w[m].size - 1)));
+        }
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+          CT q[4];
+#pragma unroll
+          for (int m = 0; m < 4; m++) q[m] = s_w[j[m] + s * w[m].size];
+#pragma unroll
+          for (int m = 0; m < 4; m++) {
+            float val;
+            if constexpr (INTERP == INTERP_CUBIC)
+              val = fmaf(u[m], fmaf(u[m], fmaf(u[m], q[m].w, q[m].z), q[m].y), q[m].x);
+            else
+              val = fmaf(u[m], q[m].y, q[m].x);
+            const int tgt = (m + s * K) & 3;
+            part[tgt][k] = fmaf(val, wg[m], part[tgt][k]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int k = 0; k < P; k++) {  // Kahan step with the chunk's sum
+        const float yk = part[m][k] - comp[m][k];
+        const float tk = sum[m][k] + yk;
+        comp[m][k] = (tk - sum[m][k]) - yk;
+        sum[m][k] = tk;
+      }
+    v0 += nv;
+    __syncthreads();
+  }
+  if (ix >= h) return;
+  float* out = a.split == 1 ? a.image : a.partial + static_cast<size_t>(grp) * n * n;
+#pragma unroll
+  for (int k = 0; k < P; k++) {
+    const int iy = iyb + k;
+    if (iy >= h) break;
+    // orbit pixels (column, row): R (i, j) = (n-1-j, i); clockwise steps
+    // swap the images of R and R^3
+    const int c1 = dirf > 0.f ? n - 1 - iy : iy, r1 = dirf > 0.f ? ix : n - 1 - ix;
+    const size_t idx[4] = {static_cast<size_t>(ix) * n + iy, static_cast<size_t>(c1) * n + r1,
+                           static_cast<size_t>(n - 1 - ix) * n + (n - 1 - iy),
+                           static_cast<size_t>(n - 1 - c1) * n + (n - 1 - r1)};
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const float v = sum[m][k] - comp[m][k];
+      if (a.split > 1) out[idx[m]] = v;
+      else out[idx[m]] = a.overwrite ? a.endScale * v : fmaf(a.endScale, v, out[idx[m]]);
+    }
   }
 }
 

@@ -22,12 +22,8 @@ def simulate(ctx: api.Context, spec: SliceSpec, precision: int = api.PREC_F64):
     return sino, ang, sc.info
 
 
-def reconstruct(ctx: api.Context, spec: SliceSpec, sino, angles, info, precision: int,
-                image=None):
-    """Reconstructor::reconstructAllViews on the device (filter, backproject,
-    post-scale) into a float32 [nx, ny] image."""
-    import torch
-    dev = sino.device
+def plans(ctx: api.Context, spec: SliceSpec, info: dict):
+    """The ProcessSignal and Backprojector of one slice (reconstruct.cpp:59-100)."""
     ps = api.ProcessSignal(ctx, spec.filter, spec.method, 1 / info["detInc"], info["detInc"],
                            spec.nDet, spec.filterParam, "spatial", spec.generation, spec.zeropad,
                            1, spec.geometry, info["focalLength"], info["sourceDetectorLength"])
@@ -35,6 +31,16 @@ def reconstruct(ctx: api.Context, spec: SliceSpec, sino, angles, info, precision
                            info["viewDiameter"], info["focalLength"],
                            info["sourceDetectorLength"], spec.backproj, spec.interp, spec.nx,
                            spec.ny)
+    return ps, bp
+
+
+def reconstruct(ctx: api.Context, spec: SliceSpec, sino, angles, info, precision: int,
+                image=None):
+    """Reconstructor::reconstructAllViews on the device (filter, backproject,
+    post-scale) into a float32 [nx, ny] image, as separate steps."""
+    import torch
+    dev = sino.device
+    ps, bp = plans(ctx, spec, info)
     dt = torch.float64 if precision == api.PREC_F64 else torch.float32
     filt = torch.empty((spec.nView, spec.nDet), dtype=dt, device=dev)
     if image is None:
@@ -44,6 +50,17 @@ def reconstruct(ctx: api.Context, spec: SliceSpec, sino, angles, info, precision
     ps.filter(sino, filt, precision)
     bp.backproject(filt, angles, image, precision)
     bp.postprocess(image)
+    return image
+
+
+def reconstruct_fused(ctx: api.Context, spec: SliceSpec, sino, angles, info, precision: int,
+                      image=None):
+    """The same reconstruction through the one-call slice entry point."""
+    import torch
+    ps, bp = plans(ctx, spec, info)
+    if image is None:
+        image = torch.empty((spec.nx, spec.ny), dtype=torch.float32, device=sino.device)
+    bp.fbp(ps, sino, angles, image, precision)
     return image
 
 

@@ -110,19 +110,35 @@ int ctsim_backproject_views(ctsim_ctx* ctx, const ctsim_backprojector* bp, int p
 int ctsim_backproject_postprocess(ctsim_ctx* ctx, const ctsim_backprojector* bp,
                                   float* d_image);
 
+/* One whole slice: d_image = the reconstruction of nView rows of d_sino
+ * (the reference's sequence for one slice: a zeroed ImageFile, filterSignal
+ * of every view, BackprojectView of every view, PostProcessing;
+ * reconstruct.cpp:156-239).  No single counterpart in the reference.  The
+ * mixed precision fuses the steps (FFT filters write the backprojector's
+ * interval coefficients straight from LDS; the backprojection overwrites the
+ * image with the post-scale applied); F64, nearest interpolation and Diff run
+ * the separate steps.  The filter's point count must equal the
+ * backprojector's detector count. */
+int ctsim_fbp_slice(ctsim_ctx* ctx, ctsim_procsignal* ps, const ctsim_backprojector* bp,
+                    int precision, const float* d_sino, const double* h_angles, int nView,
+                    float* d_image);
+
 /* Geometry the backprojector derived: xMin, xMax, yMin, yMax, xInc, yInc,
  * rotScale, iDetCenter (8 doubles). */
 int ctsim_backprojector_info(const ctsim_backprojector* bp, double* out8);
 /* No counterpart in the reference (launch tuning of the MIXED kernel).
  * Times candidate shapes (lanes along y, rows per lane, view groups, views
- * per LDS chunk) on nView float rows of d_filtered into a scratch image and
- * keeps the fastest for later MIXED launches of this plan; shape5 (optional)
- * receives lanesY, P, split, chunk, window and bestMs (optional) its time. */
+ * per LDS chunk, plain or rotationally symmetric kernel) on nView float rows
+ * of d_filtered into a scratch image and keeps the fastest for later MIXED
+ * launches of this plan; shape6 (optional) receives lanesY, P, split, chunk,
+ * window, sym and bestMs (optional) its time. */
 int ctsim_backprojector_autotune(ctsim_ctx* ctx, ctsim_backprojector* bp, const float* d_filtered,
-                                 const double* h_angles, int nView, int* shape5, double* bestMs);
+                                 const double* h_angles, int nView, int* shape6, double* bestMs);
 /* Sets the MIXED launch shape explicitly: lanesY (8, 16, 32, 64), P (1, 2,
- * 4), split (1..64), chunk (1..64). */
-int ctsim_backprojector_set_shape(ctsim_backprojector* bp, const int* shape4);
+ * 4), split (1..64), chunk (1..64), sym (0, 1: the symmetric kernel where the
+ * image is square and centred, the views come in 2 or 4 rotated copies, and
+ * P <= 2). */
+int ctsim_backprojector_set_shape(ctsim_backprojector* bp, const int* shape5);
 
 /* ------------------------------------------------------------ filtering */
 
@@ -233,13 +249,15 @@ int ctsim_batch_run(ctsim_ctx* ctx, ctsim_batch* b, const float* const* d_sinos,
  * waiting for its slice's filtering (never expected). */
 int ctsim_batch_status(ctsim_ctx* ctx, ctsim_batch* b);
 /* Launch shape: P rows per lane (1, 2, 4), lanesY (8, 16, 32), LDS bytes per
- * view chunk, fused (1) or two-phase (0). */
+ * view chunk, fused (1) or two-phase (0), sym (1: slices with a square
+ * centred image whose views come in 2 or 4 rotated copies run the
+ * rotationally symmetric body, when P <= 2). */
 int ctsim_batch_set_shape(ctsim_ctx* ctx, ctsim_batch* b, int P, int lanesY, int chunkBytes,
-                          int fused);
-/* Times candidate shapes on these inputs and keeps the fastest; shape4 gets
- * P, lanesY, chunkBytes, fused. */
+                          int fused, int sym);
+/* Times candidate shapes on these inputs and keeps the fastest; shape5 gets
+ * P, lanesY, chunkBytes, fused, sym. */
 int ctsim_batch_autotune(ctsim_ctx* ctx, ctsim_batch* b, const float* const* d_sinos,
-                         float* const* d_images, int* shape4, double* bestMs);
+                         float* const* d_images, int* shape5, double* bestMs);
 /* out4: slices, pixel-view updates per run, fused, persistent grid size. */
 int ctsim_batch_info(const ctsim_batch* b, double* out4);
 

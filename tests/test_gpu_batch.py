@@ -3,7 +3,8 @@ single-slice paths.  Bars: every slice within d <= 1e-5 of the FP64
 single-slice image (Herman's d; the batch filters convolutions by FP32 FFT
 convolution, measured d ~ 1e-6) and within d <= 1e-4 of the single-slice
 mixed image (which filters convolutions by the BF16x3 Toeplitz product);
-every launch shape gives the same images to FP32 summation-order noise;
+every launch shape, with and without the rotationally symmetric body, gives
+the same images to FP32 summation-order noise;
 graph replays are bitwise identical; new image buffers are picked up."""
 import numpy as np
 import pytest
@@ -67,19 +68,24 @@ def test_batch_matches_single_slice(gpu_ctx, batch_case):
     _check(gpu_ctx, specs, images, ref64, refmx)
 
 
-@pytest.mark.parametrize("P,lanesY,chunk,fused", [(2, 16, 32768, True), (4, 8, 16384, True),
-                                                   (2, 32, 16384, False), (4, 16, 32768, False),
-                                                   (1, 16, 8192, True)])
-def test_batch_shapes(gpu_ctx, batch_case, P, lanesY, chunk, fused):
+# sym: slices whose views come in rotated copies (here all but the 181- and
+# 97-view ones) run the rotationally symmetric body when P <= 2
+@pytest.mark.parametrize("P,lanesY,chunk,fused,sym",
+                         [(2, 16, 32768, True, True), (4, 8, 16384, True, False),
+                          (2, 32, 16384, False, True), (4, 16, 32768, False, False),
+                          (1, 16, 8192, True, True), (2, 16, 32768, True, False),
+                          (1, 8, 16384, False, True)])
+def test_batch_shapes(gpu_ctx, batch_case, P, lanesY, chunk, fused, sym):
     import torch
     specs, b, sinos, images, ref64, refmx = batch_case
-    print(f"case batch shape P={P} lanesY={lanesY} chunk={chunk} fused={fused}", flush=True)
-    b.set_shape(P, lanesY, chunk, fused)
+    print(f"case batch shape P={P} lanesY={lanesY} chunk={chunk} fused={fused} sym={sym}",
+          flush=True)
+    b.set_shape(P, lanesY, chunk, fused, sym)
     imgs = [torch.zeros_like(i) for i in images]
     b.run(sinos, imgs)
     b.status()
     _check(gpu_ctx, specs, imgs, ref64, refmx)
-    b.set_shape(2, 16, 32768, True)
+    b.set_shape(2, 16, 32768, True, False)
     base = [torch.zeros_like(i) for i in images]
     b.run(sinos, base)
     b.status()
@@ -90,7 +96,7 @@ def test_batch_shapes(gpu_ctx, batch_case, P, lanesY, chunk, fused):
 def test_batch_graph_replays(gpu_ctx, batch_case):
     import torch
     specs, b, sinos, images, ref64, refmx = batch_case
-    b.set_shape(2, 16, 32768, True)
+    b.set_shape(2, 16, 32768, True, True)
     b.run(sinos, images)
     b.status()
     first = [i.clone() for i in images]

@@ -1,8 +1,13 @@
 """The mixed backprojector under every launch shape the autotuner may pick
 (lanes along y, rows per lane, view groups incl. the one-group-per-XCD
-mapping, views per LDS chunk, windowed and full-row staging): each shape gives
-the same image (to FP32 summation-order noise), within d <= 1e-4 of the FP64
-path; and the autotuner's pick is one of them."""
+mapping, views per LDS chunk, windowed and full-row staging, the plain and
+the rotationally symmetric kernel): each shape gives the same image (to FP32
+summation-order noise), within d <= 1e-4 of the FP64 path; and the
+autotuner's pick is one of them.  The symmetric kernel serves square centred
+images whose views come in 4 quarter-turned copies (fan beam over 2 pi,
+nView % 4 == 0), 2 quarter-turned copies (parallel over pi) or 2 half-turned
+copies (fan beam, nView % 4 == 2), in either rotation direction; other view
+sets and odd image sizes fall back to the plain kernel."""
 import numpy as np
 import pytest
 
@@ -11,7 +16,9 @@ from conftest import load_golden
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(8, 1, 1, 4), (16, 4, 8, 16), (32, 2, 2, 32), (16, 2, 3, 5), (64, 1, 16, 1),
-          (16, 4, 1, 8)]
+          (16, 4, 1, 8),
+          # the symmetric kernel: lanesY, P, split (of the base views), base views per chunk
+          (16, 1, 1, 4, 1), (8, 2, 8, 16, 1), (32, 2, 3, 5, 1), (16, 2, 1, 1, 1), (8, 1, 16, 2, 1)]
 
 
 def _slice(ctx, geometry, nDet, nView, nx, interp, backproj, rot, filt="abs_hanning",
@@ -21,6 +28,7 @@ def _slice(ctx, geometry, nDet, nView, nx, interp, backproj, rot, filt="abs_hann
     from ctsim_amd import workloads as W
     elems = W.seeded_elements(24, 0xC7511 + 77)
     phm = ctsim_amd.Phantom(None, elems)
+    # a negative rotation length scans clockwise
     sc = ctsim_amd.Scanner(phm, geometry, nDet, nView, 0, 1, rot, 2.0, 2.0)
     info = sc.info
     sino = torch.empty((nView, nDet), dtype=torch.float32, device="cuda")
@@ -50,7 +58,11 @@ CASES = [("equiangular", 512, 360, 256, "cubic", "idiff", 2 * np.pi),
          ("equilinear", 383, 300, 192, "linear", "idiff", 2 * np.pi),
          ("parallel", 367, 320, 256, "linear", "idiff", np.pi),
          ("parallel", 300, 256, 128, "cubic", "trig", np.pi),
-         ("equiangular", 96, 64, 48, "linear", "table", 2 * np.pi)]
+         ("equiangular", 96, 64, 48, "linear", "table", 2 * np.pi),
+         ("equiangular", 300, 198, 128, "cubic", "idiff", 2 * np.pi),   # half turns
+         ("equilinear", 256, 240, 128, "cubic", "idiff", -2 * np.pi),   # clockwise
+         ("equiangular", 256, 181, 96, "cubic", "idiff", 2 * np.pi),    # no symmetry
+         ("equiangular", 256, 240, 95, "linear", "idiff", 2 * np.pi)]   # odd image
 
 
 @pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{c[4]}-{c[5]}-{c[3]}" for c in CASES])
