@@ -120,7 +120,8 @@ def main():
                          "are the large-size, batched and helical configurations")
     ap.add_argument("--slices", type=int, default=256, help="config4: slices in the batch")
     ap.add_argument("--shape", default="",
-                    help="config2: mixed launch shape lanesY,P,split,chunk instead of autotuning")
+                    help="config2: mixed launch shape lanesY,P,split,chunk[,sym[,stagingKB]] "
+                         "instead of autotuning")
     ap.add_argument("--batch", type=int, default=8,
                     help="config2: also time this many copies of the slice as one batch (0: skip)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -212,12 +213,13 @@ def main():
         vals = [int(x) for x in args.shape.split(",")]
         lanesY, P, split, chunk = vals[:4]
         sym = vals[4] if len(vals) > 4 else 0
-        bp.set_shape(lanesY, P, split, chunk, sym)
-        shape = (lanesY, P, split, chunk, -1, sym, float("nan"))
+        kb = vals[5] if len(vals) > 5 else 0
+        bp.set_shape(lanesY, P, split, chunk, sym, kb)
+        shape = (lanesY, P, split, chunk, -1, sym, kb, float("nan"))
     else:
         shape = bp.autotune(f32, ang)
     log(f"[bench] autotuned mixed shape: lanesY={shape[0]} P={shape[1]} split={shape[2]} "
-        f"chunk={shape[3]} window={shape[4]} sym={shape[5]} ({shape[6]:.3f} ms)")
+        f"chunk={shape[3]} window={shape[4]} sym={shape[5]} stagingKB={shape[6]} ({shape[7]:.3f} ms)")
 
     # --- mixed path as separate steps (zero, filter, backproject, scale)
     for _ in range(2):
@@ -317,7 +319,7 @@ def main():
                      "pvu_per_s": pvu / (bp_ms * 1e-3),
                      "launch_shape": {"lanesY": shape[0], "P": shape[1], "split": shape[2],
                                       "chunk": shape[3], "window": shape[4],
-                                      "symmetric": shape[5]}},
+                                      "symmetric": shape[5], "staging_kb": shape[6]}},
         "raysum": {"fp64_ms": t_raysum64 * 1e3, "closed_form_ms": t_raysum_m * 1e3,
                    "ray_element_pairs": n_re},
         "batched": batch,

@@ -295,7 +295,7 @@ class Backprojector:
 
     def autotune(self, filtered, angles: np.ndarray):
         """Times the MIXED kernel's launch shapes on these float32 rows and
-        keeps the fastest; returns (lanesY, P, split, chunk, window, sym, ms)."""
+        keeps the fastest; returns (lanesY, P, split, chunk, window, sym, stagingKB, ms)."""
         import torch
         if filtered.dtype != torch.float32 or filtered.dim() != 2 or \
                 filtered.shape[1] != self.nDet * self.interpFactor:
@@ -303,7 +303,7 @@ class Backprojector:
         ang = np.ascontiguousarray(angles, dtype=np.float64)
         if ang.shape[0] < filtered.shape[0]:
             raise ValueError("not enough view angles")
-        shape = np.zeros(6, np.int32)
+        shape = np.zeros(7, np.int32)
         ms = np.zeros(1, np.float64)
         check(lib.ctsim_backprojector_autotune(self.ctx.handle, self.handle,
                                                C.c_void_p(_dptr(filtered)), _hptr(ang),
@@ -311,10 +311,12 @@ class Backprojector:
               "ctsim_backprojector_autotune")
         return (*map(int, shape), float(ms[0]))
 
-    def set_shape(self, lanesY: int, P: int, split: int, chunk: int, sym: int = 0):
+    def set_shape(self, lanesY: int, P: int, split: int, chunk: int, sym: int = 0,
+                  stagingKB: int = 0):
         """Pins the MIXED launch shape; sym = 1 runs the rotationally symmetric
-        kernel when the image and the views allow it (P <= 2)."""
-        s = np.array([lanesY, P, split, chunk, sym], np.int32)
+        kernel when the image and the views allow it (P <= 2), staging its
+        windows in stagingKB KB of LDS (0: 62)."""
+        s = np.array([lanesY, P, split, chunk, sym, stagingKB], np.int32)
         check(lib.ctsim_backprojector_set_shape(self.handle, _hptr(s)), "ctsim_backprojector_set_shape")
 
     def postprocess(self, image):

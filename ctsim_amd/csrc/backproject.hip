@@ -419,6 +419,27 @@ __global__ __launch_bounds__(256) void bp_sym_kernel(MixedArgs a) {
                                              s_ab, s_win, &s_n);
 }
 
+// The software-pipelined symmetric variant (same mapping).
+template <int GK, int INTERP, int P, int DEG, int S, int K>
+__global__ __launch_bounds__(256) void bp_sym_pipe_kernel(MixedArgs a) {
+  extern __shared__ float4 s_dyn[];
+  __shared__ float2 s_ab[64];
+  __shared__ dev::SymWin s_win[256];
+  const int nTiles = a.tilesX * a.tilesY;
+  int tile, grp;
+  if ((a.split & 7) == 0) {
+    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+    grp = xcd * (a.split >> 3) + local / nTiles;
+    tile = local % nTiles;
+  } else {
+    const int bid = dev::xcd_tile(blockIdx.x, gridDim.x);
+    tile = bid % nTiles;
+    grp = bid / nTiles;
+  }
+  dev::bp_sym_tile_pipe<GK, INTERP, P, DEG, S, K>(a, tile, grp, reinterpret_cast<CoefT<INTERP>*>(s_dyn),
+                                                  s_ab, s_win);
+}
+
 // image += endScale * (sum of the view groups' partial images), groups added
 // in order in FP64 (overwrite: image = endScale * sum).
 __global__ void bp_combine_kernel(float* __restrict__ image, const float* __restrict__ partial,
@@ -502,28 +523,33 @@ void launch_mixed(int gk, int deg, int P, const MixedArgs& a, int grid, size_t l
 }
 
 template <int GK, int INTERP, int DEG, int S, int K>
-void launch_sym_p(int P, const MixedArgs& a, int grid, size_t lds, hipStream_t s) {
-  if (P == 1) hipLaunchKernelGGL((bp_sym_kernel<GK, INTERP, 1, DEG, S, K>), dim3(grid), dim3(256), lds, s, a);
-  else hipLaunchKernelGGL((bp_sym_kernel<GK, INTERP, 2, DEG, S, K>), dim3(grid), dim3(256), lds, s, a);
+void launch_sym_p(bool pipe, int P, const MixedArgs& a, int grid, size_t lds, hipStream_t s) {
+  if (pipe) {
+    if (P == 1) hipLaunchKernelGGL((bp_sym_pipe_kernel<GK, INTERP, 1, DEG, S, K>), dim3(grid), dim3(256), lds, s, a);
+    else hipLaunchKernelGGL((bp_sym_pipe_kernel<GK, INTERP, 2, DEG, S, K>), dim3(grid), dim3(256), lds, s, a);
+  } else {
+    if (P == 1) hipLaunchKernelGGL((bp_sym_kernel<GK, INTERP, 1, DEG, S, K>), dim3(grid), dim3(256), lds, s, a);
+    else hipLaunchKernelGGL((bp_sym_kernel<GK, INTERP, 2, DEG, S, K>), dim3(grid), dim3(256), lds, s, a);
+  }
 }
 
 template <int GK, int INTERP, int DEG>
-void launch_sym_sk(int S, int K, int P, const MixedArgs& a, int grid, size_t lds, hipStream_t s) {
-  if (S == 4) launch_sym_p<GK, INTERP, DEG, 4, 1>(P, a, grid, lds, s);
-  else if (K == 1) launch_sym_p<GK, INTERP, DEG, 2, 1>(P, a, grid, lds, s);
-  else launch_sym_p<GK, INTERP, DEG, 2, 2>(P, a, grid, lds, s);
+void launch_sym_sk(bool pipe, int S, int K, int P, const MixedArgs& a, int grid, size_t lds, hipStream_t s) {
+  if (S == 4) launch_sym_p<GK, INTERP, DEG, 4, 1>(pipe, P, a, grid, lds, s);
+  else if (K == 1) launch_sym_p<GK, INTERP, DEG, 2, 1>(pipe, P, a, grid, lds, s);
+  else launch_sym_p<GK, INTERP, DEG, 2, 2>(pipe, P, a, grid, lds, s);
 }
 
 template <int INTERP>
-void launch_sym(int gk, int deg, int S, int K, int P, const MixedArgs& a, int grid, size_t lds,
+void launch_sym(bool pipe, int gk, int deg, int S, int K, int P, const MixedArgs& a, int grid, size_t lds,
                 hipStream_t s) {
-  if (gk == 0) return launch_sym_sk<0, INTERP, 0>(S, K, P, a, grid, lds, s);
-  if (gk == 2) return launch_sym_sk<2, INTERP, 0>(S, K, P, a, grid, lds, s);
+  if (gk == 0) return launch_sym_sk<0, INTERP, 0>(pipe, S, K, P, a, grid, lds, s);
+  if (gk == 2) return launch_sym_sk<2, INTERP, 0>(pipe, S, K, P, a, grid, lds, s);
   switch (deg) {
-    case 4: launch_sym_sk<1, INTERP, 4>(S, K, P, a, grid, lds, s); break;
-    case 6: launch_sym_sk<1, INTERP, 6>(S, K, P, a, grid, lds, s); break;
-    case 8: launch_sym_sk<1, INTERP, 8>(S, K, P, a, grid, lds, s); break;
-    default: launch_sym_sk<1, INTERP, 0>(S, K, P, a, grid, lds, s); break;
+    case 4: launch_sym_sk<1, INTERP, 4>(pipe, S, K, P, a, grid, lds, s); break;
+    case 6: launch_sym_sk<1, INTERP, 6>(pipe, S, K, P, a, grid, lds, s); break;
+    case 8: launch_sym_sk<1, INTERP, 8>(pipe, S, K, P, a, grid, lds, s); break;
+    default: launch_sym_sk<1, INTERP, 0>(pipe, S, K, P, a, grid, lds, s); break;
   }
 }
 
@@ -641,7 +667,13 @@ bool bp_sym_ok(const BpPlan& p, const MixedShape& m, int nView) {
       p.cachedKind != 1)
     return false;
   const size_t elem = p.interp == INTERP_CUBIC ? sizeof(float4) : sizeof(float2);
-  return static_cast<size_t>(4 * sv.S) * m.W * elem <= kSymLds;
+  if (m.sym == 2) return m.W <= 128;  // the pipelined body stages 4 S W cells, two per lane per row
+  return static_cast<size_t>(4 * sv.S) * m.W * elem <= sym_lds(m);
+}
+
+size_t sym_lds(const MixedShape& m) {
+  return m.symKB > 0 && static_cast<size_t>(m.symKB) * 1024 < kSymLds ? static_cast<size_t>(m.symKB) * 1024
+                                                                      : kSymLds;
 }
 
 bool bp_mixed_tiled(const BpPlan& p) {
@@ -730,9 +762,9 @@ int bp_mixed_tiles(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const v
     a.split = m.split < 1 ? 1 : (m.split > a.Q ? a.Q : m.split);
     a.viewsPerGroup = (a.Q + a.split - 1) / a.split;
     a.split = (a.Q + a.viewsPerGroup - 1) / a.viewsPerGroup;
-    a.capW = static_cast<int>(kSymLds / elem);
+    a.capW = m.sym == 2 ? 4 * sv.S * a.W : static_cast<int>(sym_lds(m) / elem);
   }
-  const size_t lds = sym ? kSymLds : static_cast<size_t>(a.chunk) * a.W * elem;
+  const size_t lds = sym ? static_cast<size_t>(a.capW) * elem : static_cast<size_t>(a.chunk) * a.W * elem;
   if (lds > kMixedLdsMax) return fail(CTSIM_ERR_UNSUPPORTED, "backprojector: detector row too long for LDS");
   if (a.split > 1) {
     CTSIM_CHECK(ctx->scratch.reserve(static_cast<size_t>(a.split) * npix * sizeof(float)));
@@ -741,8 +773,9 @@ int bp_mixed_tiles(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const v
   const int grid = a.tilesX * a.tilesY * a.split;
   ctx->prof_begin();
   if (sym) {
-    if (cubic) launch_sym<INTERP_CUBIC>(gk, deg, sv.S, sv.K, P, a, grid, lds, s);
-    else launch_sym<INTERP_LINEAR>(gk, deg, sv.S, sv.K, P, a, grid, lds, s);
+    const bool pipe = m.sym == 2;
+    if (cubic) launch_sym<INTERP_CUBIC>(pipe, gk, deg, sv.S, sv.K, P, a, grid, lds, s);
+    else launch_sym<INTERP_LINEAR>(pipe, gk, deg, sv.S, sv.K, P, a, grid, lds, s);
   } else if (cubic) {
     launch_mixed<INTERP_CUBIC>(gk, deg, P, a, grid, lds, s);
   } else {
@@ -798,7 +831,7 @@ int bp_autotune(ctsim_ctx* ctx, BpPlan& p, const float* d_filtered, const double
   // candidates: the plain kernel over lanesY x P x split x chunk, then the
   // symmetric kernel (when the views and the image allow it) over
   // lanesY x P x split (of the base views) x base views per chunk
-  for (int sym = 0; sym <= 1 && rc == CTSIM_OK; sym++) {
+  for (int sym = 0; sym <= 2 && rc == CTSIM_OK; sym++) {
     for (int lanesY : {8, 16, 32}) {
       for (int P : {1, 2, 4}) {
         if (lanesY * P > 2 * p.ny && P > 1) continue;
@@ -807,13 +840,18 @@ int bp_autotune(ctsim_ctx* ctx, BpPlan& p, const float* d_filtered, const double
         const int Wv = W > 0 ? W : p.nDet + 4;
         MixedShape m;
         m.lanesY = lanesY; m.P = P; m.W = W; m.sym = sym;
-        if (sym && !bp_sym_ok(p, m, nView)) continue;
+        if (sym && !bp_sym_ok(p, m, nView)) continue;  // not even in 62 KB
         const int views = sym ? nView / p.cachedSym.S : nView;
         for (int split : {1, 2, 4, 8, 16}) {
           if (split > 1 && views / split < (sym ? 4 : 8)) continue;
           for (int chunk : {2, 4, 8, 16, 32}) {
             if (sym ? chunk > 16 : (chunk < 4 || static_cast<size_t>(chunk) * Wv * elem > kMixedLdsMax))
               continue;
+           for (int symKB : {16, 24, 32, 48, 62}) {
+            if (sym != 1 && symKB != 62) continue;  // only the chunked body stages into symKB
+            if (sym == 2 && chunk != 2) continue;   // the pipelined body has no chunk
+            m.symKB = symKB;
+            if (sym && !bp_sym_ok(p, m, nView)) continue;
             m.split = split; m.chunk = chunk;
             float ms = 0, tot = 0;
             for (int rep = 0; rep < 3 && rc == CTSIM_OK; rep++) {
@@ -828,6 +866,8 @@ int bp_autotune(ctsim_ctx* ctx, BpPlan& p, const float* d_filtered, const double
             }
             if (rc != CTSIM_OK) break;
             if (tot / 2 < bestT) { bestT = tot / 2; best = m; }
+           }
+           if (rc != CTSIM_OK) break;
           }
           if (rc != CTSIM_OK) break;
         }
@@ -903,27 +943,29 @@ int ctsim_backproject_postprocess(ctsim_ctx* ctx, const ctsim_backprojector* bp,
 }
 
 int ctsim_backprojector_autotune(ctsim_ctx* ctx, ctsim_backprojector* bp, const float* d_filtered,
-                                 const double* h_angles, int nView, int* shape6, double* bestMs) {
+                                 const double* h_angles, int nView, int* shape7, double* bestMs) {
   if (!ctx || !bp || !d_filtered || !h_angles)
     return fail(CTSIM_ERR_INVALID, "ctsim_backprojector_autotune: NULL argument");
   CTSIM_CHECK(ctx->activate());
   CTSIM_CHECK(bp_autotune(ctx, bp->plan, d_filtered, h_angles, nView, bestMs));
-  if (shape6) {
+  if (shape7) {
     const MixedShape& m = bp->plan.mixed;
-    shape6[0] = m.lanesY; shape6[1] = m.P; shape6[2] = m.split; shape6[3] = m.chunk; shape6[4] = m.W;
-    shape6[5] = m.sym;
+    shape7[0] = m.lanesY; shape7[1] = m.P; shape7[2] = m.split; shape7[3] = m.chunk; shape7[4] = m.W;
+    shape7[5] = m.sym; shape7[6] = m.sym ? static_cast<int>(sym_lds(m) / 1024) : 0;
   }
   return CTSIM_OK;
 }
 
-int ctsim_backprojector_set_shape(ctsim_backprojector* bp, const int* shape5) {
-  if (!bp || !shape5) return fail(CTSIM_ERR_INVALID, "ctsim_backprojector_set_shape: NULL argument");
-  const int lanesY = shape5[0], P = shape5[1], split = shape5[2], chunk = shape5[3], sym = shape5[4];
+int ctsim_backprojector_set_shape(ctsim_backprojector* bp, const int* shape6) {
+  if (!bp || !shape6) return fail(CTSIM_ERR_INVALID, "ctsim_backprojector_set_shape: NULL argument");
+  const int lanesY = shape6[0], P = shape6[1], split = shape6[2], chunk = shape6[3], sym = shape6[4],
+            symKB = shape6[5];
   if (!(lanesY == 8 || lanesY == 16 || lanesY == 32 || lanesY == 64) || !(P == 1 || P == 2 || P == 4) ||
-      split < 1 || split > 64 || chunk < 1 || chunk > kMaxChunk || (sym != 0 && sym != 1))
+      split < 1 || split > 64 || chunk < 1 || chunk > kMaxChunk || sym < 0 || sym > 2 || symKB < 0 ||
+      symKB > 62)
     return fail(CTSIM_ERR_INVALID, "ctsim_backprojector_set_shape: shape out of range");
   MixedShape m;
-  m.lanesY = lanesY; m.P = P; m.split = split; m.chunk = chunk; m.sym = sym;
+  m.lanesY = lanesY; m.P = P; m.split = split; m.chunk = chunk; m.sym = sym; m.symKB = symKB;
   m.W = bp_window_bound(bp->plan, 256 / lanesY, lanesY * P);
   const int Wv = m.W > 0 ? m.W : bp->plan.nDet + 4;
   if (static_cast<size_t>(chunk) * Wv * (bp->plan.interp == INTERP_CUBIC ? 16 : 8) > kMixedLdsMax)

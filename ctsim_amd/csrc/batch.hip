@@ -68,7 +68,7 @@ __global__ void batch_reset_kernel(int* counters, int n) {
 }
 
 template <int P>
-__device__ void bp_item(const MixedArgs& a, int gk, int interp, int deg, int tile, void* smem,
+__device__ __forceinline__ void bp_item(const MixedArgs& a, int gk, int interp, int deg, int tile, void* smem,
                         float4* s_v, int* s_lo) {
   using dev::bp_mixed_tile;
   if (interp == INTERP_CUBIC) {
@@ -92,7 +92,7 @@ __device__ void bp_item(const MixedArgs& a, int gk, int interp, int deg, int til
 
 // The symmetric body for slices whose views come in rotated copies.
 template <int GK, int INTERP, int P, int DEG>
-__device__ void bp_sym_sk(const SliceDev& s, int tile, void* smem, float2* s_ab, dev::SymWin* s_win,
+__device__ __forceinline__ void bp_sym_sk(const SliceDev& s, int tile, void* smem, float2* s_ab, dev::SymWin* s_win,
                           int* s_n) {
   auto* w = static_cast<CoefT<INTERP>*>(smem);
   if (s.symS == 4) dev::bp_sym_tile<GK, INTERP, P, DEG, 4, 1>(s.bp, tile, 0, w, s_ab, s_win, s_n);
@@ -101,7 +101,7 @@ __device__ void bp_sym_sk(const SliceDev& s, int tile, void* smem, float2* s_ab,
 }
 
 template <int P, int INTERP>
-__device__ void bp_sym_interp(const SliceDev& s, int tile, void* smem, float2* s_ab, dev::SymWin* s_win,
+__device__ __forceinline__ void bp_sym_interp(const SliceDev& s, int tile, void* smem, float2* s_ab, dev::SymWin* s_win,
                               int* s_n) {
   if (s.gk == 0) bp_sym_sk<0, INTERP, P, 0>(s, tile, smem, s_ab, s_win, s_n);
   else if (s.gk == 2) bp_sym_sk<2, INTERP, P, 0>(s, tile, smem, s_ab, s_win, s_n);
@@ -112,7 +112,7 @@ __device__ void bp_sym_interp(const SliceDev& s, int tile, void* smem, float2* s
 }
 
 template <int R>
-__device__ void filter_item(const SliceDev& s, int pair, Cplx<float>* z, Cplx<float>* tw) {
+__device__ __forceinline__ void filter_item(const SliceDev& s, int pair, Cplx<float>* z, Cplx<float>* tw) {
   const int va = 2 * pair, vb = va + 1 < s.nView ? va + 1 : -1;
   dev::fft_filter_pair<float, R>(s.sino, s.w, s.hre, s.him, s.tw, va, vb, s.n, s.N, s.logN, z, tw);
   // interval coefficients of both rows straight from LDS
@@ -141,8 +141,10 @@ __device__ void filter_item(const SliceDev& s, int pair, Cplx<float>* z, Cplx<fl
 
 // counters: [0, 8) queue heads, [8, 8 + nSlices) filter items done per slice.
 // SYM instantiates the symmetric body (P <= 2); the plain kernel leaves it
-// out and keeps its own register budget.
-template <int P, bool SYM>
+// out and keeps its own register budget.  DO_F / DO_B instantiate the filter
+// and the backprojection items: the fused launch takes both, the two-phase
+// launches one each, so each phase runs at its own register budget.
+template <int P, bool SYM, bool DO_F, bool DO_B>
 __global__ __launch_bounds__(256) void fbp_batch_kernel(const SliceDev* __restrict__ slices,
                                                         const Item* __restrict__ items,
                                                         const int* __restrict__ queueStart,
@@ -166,24 +168,28 @@ __global__ __launch_bounds__(256) void fbp_batch_kernel(const SliceDev* __restri
   while (true) {
     if (threadIdx.x == 0) s_item = qBegin + atomicAdd(&counters[q], 1);
     __syncthreads();
-    const int i = s_item;
+    // wave-uniform, so the item and its slice descriptor load into scalar
+    // registers instead of occupying vector registers in every lane
+    const int i = __builtin_amdgcn_readfirstlane(s_item);
     __syncthreads();
     if (i >= qEnd) break;  // every workgroup ends when its queue is drained
     const Item it = items[i];
     const SliceDev& s = slices[it.slice];
     if (it.kind == kFilterItem) {
-      Cplx<float>* z = reinterpret_cast<Cplx<float>*>(smem);
-      Cplx<float>* tw = z + s.N;
-      switch (s.R) {
-        case 1: filter_item<1>(s, it.a, z, tw); break;
-        case 2: filter_item<2>(s, it.a, z, tw); break;
-        case 4: filter_item<4>(s, it.a, z, tw); break;
-        default: filter_item<8>(s, it.a, z, tw); break;
+      if constexpr (DO_F) {
+        Cplx<float>* z = reinterpret_cast<Cplx<float>*>(smem);
+        Cplx<float>* tw = z + s.N;
+        switch (s.R) {
+          case 1: filter_item<1>(s, it.a, z, tw); break;
+          case 2: filter_item<2>(s, it.a, z, tw); break;
+          case 4: filter_item<4>(s, it.a, z, tw); break;
+          default: filter_item<8>(s, it.a, z, tw); break;
+        }
+        __threadfence();  // every lane releases its coefficient stores device-wide
+        __syncthreads();
+        if (threadIdx.x == 0) atomicAdd(&done[it.slice], 1);
       }
-      __threadfence();  // every lane releases its coefficient stores device-wide
-      __syncthreads();
-      if (threadIdx.x == 0) atomicAdd(&done[it.slice], 1);
-    } else {
+    } else if constexpr (DO_B) {
       if (threadIdx.x == 0) {
         int spins = 0;
         while (__hip_atomic_load(&done[it.slice], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
@@ -428,14 +434,21 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
   b.ldsB = maxBpLds;
   b.ldsAll = std::max(maxBpLds, static_cast<size_t>(b.ldsFilter));
   const bool symKernel = batch_sym_kernel(b);
-  const void* kern = b.P == 4 ? reinterpret_cast<const void*>(fbp_batch_kernel<4, false>)
-                     : b.P == 1 ? (symKernel ? reinterpret_cast<const void*>(fbp_batch_kernel<1, true>)
-                                             : reinterpret_cast<const void*>(fbp_batch_kernel<1, false>))
-                                : (symKernel ? reinterpret_cast<const void*>(fbp_batch_kernel<2, true>)
-                                             : reinterpret_cast<const void*>(fbp_batch_kernel<2, false>));
-  b.gridF = occupancy_grid(kern, b.ldsFilter, ctx->numCUs);
-  b.gridB = occupancy_grid(kern, b.ldsB, ctx->numCUs);
-  b.gridAll = occupancy_grid(kern, b.ldsAll, ctx->numCUs);
+  auto pick = [&](auto k4, auto k1s, auto k1, auto k2s, auto k2) -> const void* {
+    return b.P == 4 ? reinterpret_cast<const void*>(k4)
+           : b.P == 1 ? (symKernel ? reinterpret_cast<const void*>(k1s) : reinterpret_cast<const void*>(k1))
+                      : (symKernel ? reinterpret_cast<const void*>(k2s) : reinterpret_cast<const void*>(k2));
+  };
+  const void* kAll = pick(fbp_batch_kernel<4, false, true, true>, fbp_batch_kernel<1, true, true, true>,
+                          fbp_batch_kernel<1, false, true, true>, fbp_batch_kernel<2, true, true, true>,
+                          fbp_batch_kernel<2, false, true, true>);
+  const void* kB = pick(fbp_batch_kernel<4, false, false, true>, fbp_batch_kernel<1, true, false, true>,
+                        fbp_batch_kernel<1, false, false, true>, fbp_batch_kernel<2, true, false, true>,
+                        fbp_batch_kernel<2, false, false, true>);
+  const void* kF = reinterpret_cast<const void*>(fbp_batch_kernel<1, false, true, false>);
+  b.gridF = occupancy_grid(kF, b.ldsFilter, ctx->numCUs);
+  b.gridB = occupancy_grid(kB, b.ldsB, ctx->numCUs);
+  b.gridAll = occupancy_grid(kAll, b.ldsAll, ctx->numCUs);
   b.shapeDirty = false;
   if (b.graph) {
     (void)hipGraphExecDestroy(b.graph);
@@ -449,15 +462,15 @@ void launch_batch(ctsim_batch& b, hipStream_t st) {
   const int S = static_cast<int>(b.s.size());
   hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8 + S);
   if (b.fused) {
-    hipLaunchKernelGGL((fbp_batch_kernel<P, SYM>), dim3(b.gridAll), dim3(256), b.ldsAll, st,
+    hipLaunchKernelGGL((fbp_batch_kernel<P, SYM, true, true>), dim3(b.gridAll), dim3(256), b.ldsAll, st,
                        b.d_slices.as<SliceDev>(), b.d_itemsAll.as<Item>(), b.d_qAll.as<int>(),
                        b.d_counters.as<int>(), b.d_error.as<int>());
   } else {
-    hipLaunchKernelGGL((fbp_batch_kernel<P, SYM>), dim3(b.gridF), dim3(256), b.ldsFilter, st,
+    hipLaunchKernelGGL((fbp_batch_kernel<1, false, true, false>), dim3(b.gridF), dim3(256), b.ldsFilter, st,
                        b.d_slices.as<SliceDev>(), b.d_itemsF.as<Item>(), b.d_qF.as<int>(),
                        b.d_counters.as<int>(), b.d_error.as<int>());
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8);
-    hipLaunchKernelGGL((fbp_batch_kernel<P, SYM>), dim3(b.gridB), dim3(256), b.ldsB, st,
+    hipLaunchKernelGGL((fbp_batch_kernel<P, SYM, false, true>), dim3(b.gridB), dim3(256), b.ldsB, st,
                        b.d_slices.as<SliceDev>(), b.d_itemsB.as<Item>(), b.d_qB.as<int>(),
                        b.d_counters.as<int>(), b.d_error.as<int>());
   }

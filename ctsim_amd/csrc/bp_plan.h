@@ -35,9 +35,11 @@ struct ViewConstF32 {
 // groups (separate workgroups whose partial sums are combined in a fixed
 // order) and staged `chunk` views at a time.
 // sym = 1 selects the rotationally symmetric kernel when the image and the
-// views allow it (P of 1 or 2; views staged up to chunk base views at a time).
+// views allow it (P of 1 or 2; views staged up to chunk base views at a time
+// in symKB KB of LDS, 0: 62 KB); sym = 2 its software-pipelined form (one
+// base view at a time, windows of at most 128 cells).
 struct MixedShape {
-  int lanesY = 16, P = 4, split = 1, chunk = 16, W = 0, sym = 0;
+  int lanesY = 16, P = 4, split = 1, chunk = 16, W = 0, sym = 0, symKB = 0;
 };
 
 // Rotational symmetry of a view set: views v + s*Q (s < S, Q = nView / S)
@@ -107,6 +109,7 @@ int bp_view_consts_mixed(ctsim_ctx* ctx, const BpPlan& p, const double* h_angles
 // kernel (not the FP64 index path) and one view's staging fits LDS.
 bool bp_mixed_tiled(const BpPlan& p);
 bool bp_sym_ok(const BpPlan& p, const MixedShape& m, int nView);
+size_t sym_lds(const MixedShape& m);  // LDS bytes the symmetric kernel stages into
 
 }  // namespace ctsim
 

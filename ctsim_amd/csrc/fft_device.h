@@ -25,7 +25,7 @@ __device__ __forceinline__ Cplx<T> cmul(Cplx<T> a, Cplx<T> b) {
 // write-after-read hazards otherwise.  R = ceil(N / 2 / 256) butterflies per
 // lane.
 template <class T, int R>
-__device__ void stockham(Cplx<T>* z, const Cplx<T>* tw, int N, int logN, int dir) {
+__device__ __forceinline__ void stockham(Cplx<T>* z, const Cplx<T>* tw, int N, int logN, int dir) {
   const int half = N >> 1;
   for (int s = 0, Ls = 1; s < logN; s++, Ls <<= 1) {
     Cplx<T> ya[R], yb[R];
@@ -72,7 +72,7 @@ __device__ __forceinline__ Cplx<T> twiddle(const Cplx<T>* tw, int m, int half, i
 // Radix-R Stockham step for butterfly t: k = t mod Ls, inputs z[t + r N/R]
 // times w^(r k N/(R Ls)), an R-point DFT, outputs at (t - k) R + k + r Ls.
 template <class T, int R>
-__device__ void stockham4(Cplx<T>* z, const Cplx<T>* tw, int N, int logN, int dir) {
+__device__ __forceinline__ void stockham4(Cplx<T>* z, const Cplx<T>* tw, int N, int logN, int dir) {
   const int half = N >> 1, quarter = N >> 2;
   int Ls = 1, s = 0;
   if (logN & 1) {
@@ -143,98 +143,152 @@ __device__ void stockham4(Cplx<T>* z, const Cplx<T>* tw, int N, int logN, int di
   }
 }
 
-// Twiddles laid out per radix-4 pass (the passes of stockham4 with Ls > 1):
-// the pass with span Ls holds w^(j k N / (4 Ls)), j = 1, 2, 3, k < Ls, at
-// offset + (j - 1) Ls + k, offsets in pass order.  Lane k of a pass then
-// reads consecutive entries (no bank conflicts; lanes sharing k broadcast),
-// where the half table's index j k N / (4 Ls) strides across one bank.  At
-// most N - 2 entries.
+// Mixed-radix Stockham FFT for FP32 transforms up to 2048 points: one
+// radix-2 or radix-4 pass first when log2 N is not a multiple of 3, then
+// radix-8 passes (N = 2048: 4, 8, 8, 8), one butterfly per lane and pass, so
+// 4 LDS round trips per transform instead of 6.  Twiddles come from a table
+// laid out per pass: the pass with span Ls > 1 holds w^(r k N / (R Ls)),
+// r = 1..R-1, k < Ls, at offset + (r - 1) Ls + k (offsets in pass order, at
+// most N - 4 entries); lane k of a pass then reads consecutive entries (no
+// bank conflicts; lanes sharing k broadcast), where the half table's index
+// r k N / (R Ls) strides across one bank.
+__host__ __device__ constexpr int fft8_first_radix(int logN) {
+  return logN % 3 == 1 ? 2 : (logN % 3 == 2 ? 4 : 8);
+}
+
 template <class T>
-__device__ void pass_twiddles(const Cplx<T>* __restrict__ twg, Cplx<T>* tw4, int N, int logN) {
-  const int half = N >> 1, quarter = N >> 2;
-  int off = 0;
-  for (int Ls = (logN & 1) ? 2 : 4; Ls <= quarter; Ls <<= 2) {
-    const int stride = quarter / Ls;
-    for (int e = threadIdx.x; e < 3 * Ls; e += blockDim.x) {
-      const int j = e / Ls + 1, k = e - (j - 1) * Ls;
-      tw4[off + e] = twiddle(twg, j * k * stride, half, -1);
+__device__ __forceinline__ void pass_twiddles8(const Cplx<T>* __restrict__ twg, Cplx<T>* tw8, int N,
+                                               int logN) {
+  // the radix-8 passes with twiddles start at Ls = the first radix (a
+  // radix-8 first pass has Ls = 1 and needs none, the next has Ls = 8); at
+  // most N - 4 <= 2044 entries, so 8 per lane, all gathered before the stores
+  const int half = N >> 1;
+  const int l0 = fft8_first_radix(logN) == 2 ? 1 : (fft8_first_radix(logN) == 4 ? 2 : 3);
+  int total = 0;
+  for (int lg = l0; lg < logN; lg += 3) total += 7 << lg;
+  Cplx<T> v[8];
+  int dst[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int e = threadIdx.x + i * blockDim.x;
+    dst[i] = -1;
+    if (e < total) {
+      int lg = l0, off = 0;
+      while (e >= off + (7 << lg)) {  // at most three steps
+        off += 7 << lg;
+        lg += 3;
+      }
+      const int el = e - off, r = (el >> lg) + 1, k = el & ((1 << lg) - 1);
+      v[i] = twiddle(twg, (r * k) << (logN - 3 - lg), half, -1);  // r k N / (8 Ls)
+      dst[i] = e;
     }
-    off += 3 * Ls;
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+    if (dst[i] >= 0) tw8[dst[i]] = v[i];
+}
+
+// 4-point DFT in place (c1 - c3 times -i forward, +i inverse).
+template <class T>
+__device__ __forceinline__ void dft4(Cplx<T>& c0, Cplx<T>& c1, Cplx<T>& c2, Cplx<T>& c3, int dir) {
+  const Cplx<T> s02 = {c0.x + c2.x, c0.y + c2.y}, d02 = {c0.x - c2.x, c0.y - c2.y};
+  const Cplx<T> s13 = {c1.x + c3.x, c1.y + c3.y}, d13 = {c1.x - c3.x, c1.y - c3.y};
+  const Cplx<T> jd = dir < 0 ? Cplx<T>{d13.y, -d13.x} : Cplx<T>{-d13.y, d13.x};
+  c0 = {s02.x + s13.x, s02.y + s13.y};
+  c2 = {s02.x - s13.x, s02.y - s13.y};
+  c1 = {d02.x + jd.x, d02.y + jd.y};
+  c3 = {d02.x - jd.x, d02.y - jd.y};
+}
+
+// R-point DFT of a[0..R) in place, R = 2, 4, 8 (radix 8 by decimation in
+// frequency: even outputs are the 4-point DFT of a_r + a_{r+4}, odd outputs
+// that of (a_r - a_{r+4}) w8^r).
+template <class T, int R>
+__device__ __forceinline__ void dft_small(Cplx<T>* a, int dir) {
+  if constexpr (R == 2) {
+    const Cplx<T> s = {a[0].x + a[1].x, a[0].y + a[1].y}, d = {a[0].x - a[1].x, a[0].y - a[1].y};
+    a[0] = s;
+    a[1] = d;
+  } else if constexpr (R == 4) {
+    dft4(a[0], a[1], a[2], a[3], dir);
+  } else {
+    const T h = T(0.70710678118654752440);
+    Cplx<T> b[4], c[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      b[r] = {a[r].x + a[r + 4].x, a[r].y + a[r + 4].y};
+      c[r] = {a[r].x - a[r + 4].x, a[r].y - a[r + 4].y};
+    }
+    // c_r *= w8^r, w8 = (1 + dir i) / sqrt 2
+    const T sg = dir < 0 ? T(-1) : T(1);
+    c[1] = {h * (c[1].x - sg * c[1].y), h * (c[1].y + sg * c[1].x)};
+    c[2] = {-sg * c[2].y, sg * c[2].x};
+    c[3] = {-h * (c[3].x + sg * c[3].y), h * (sg * c[3].x - c[3].y)};
+    dft4(b[0], b[1], b[2], b[3], dir);
+    dft4(c[0], c[1], c[2], c[3], dir);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      a[2 * k] = b[k];
+      a[2 * k + 1] = c[k];
+    }
   }
 }
 
-// stockham4 over the per-pass twiddle table of pass_twiddles.
-template <class T, int R>
-__device__ void stockham4p(Cplx<T>* z, const Cplx<T>* tw4, int N, int logN, int dir) {
-  const int half = N >> 1, quarter = N >> 2;
-  int Ls = 1, s = 0;
-  if (logN & 1) {
-    Cplx<T> ya[R], yb[R];
-    int idx[R];
+// One radix-R Stockham pass over z[0..N): butterfly t < N/R (NB per lane)
+// reads z[t + r N/R], multiplies input r by tw[off + (r - 1) Ls + k]
+// (k = t mod Ls, when Ls > 1), takes the R-point DFT and writes output q to
+// (t - k) R + k + q Ls.  Reads, barrier, writes, barrier.
+template <class T, int R, int NB>
+__device__ __forceinline__ void radix_pass(Cplx<T>* z, const Cplx<T>* tw, int N, int Ls, int off, int dir) {
+  const int nb = N / R;
+  Cplx<T> y[NB][R];
+  int idx[NB];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-      const int t = threadIdx.x + r * blockDim.x;
-      idx[r] = -1;
-      if (t < half) {
-        const Cplx<T> a = z[t], b = z[t + half];
-        ya[r] = {a.x + b.x, a.y + b.y};
-        yb[r] = {a.x - b.x, a.y - b.y};
-        idx[r] = t << 1;
-      }
-    }
-    __syncthreads();
+  for (int b = 0; b < NB; b++) {
+    const int t = threadIdx.x + b * blockDim.x;
+    idx[b] = -1;
+    if (t < nb) {
+      const int k = t & (Ls - 1);
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-      if (idx[r] >= 0) {
-        z[idx[r]] = ya[r];
-        z[idx[r] + 1] = yb[r];
-      }
-    }
-    __syncthreads();
-    Ls = 2;
-    s = 1;
-  }
-  constexpr int R4 = (R + 1) / 2;
-  int off = 0;
-  for (; s < logN; s += 2, Ls <<= 2) {
-    Cplx<T> y0[R4], y1[R4], y2[R4], y3[R4];
-    int idx[R4];
+      for (int r = 0; r < R; r++) y[b][r] = z[t + r * nb];
+      if (Ls > 1) {
 #pragma unroll
-    for (int r = 0; r < R4; r++) {
-      const int t = threadIdx.x + r * blockDim.x;
-      idx[r] = -1;
-      if (t < quarter) {
-        const int k = t & (Ls - 1);
-        Cplx<T> a0 = z[t], a1 = z[t + quarter], a2 = z[t + 2 * quarter], a3 = z[t + 3 * quarter];
-        if (Ls > 1) {
-          Cplx<T> w1 = tw4[off + k], w2 = tw4[off + Ls + k], w3 = tw4[off + 2 * Ls + k];
-          if (dir > 0) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
-          a1 = cmul(w1, a1);
-          a2 = cmul(w2, a2);
-          a3 = cmul(w3, a3);
+        for (int r = 1; r < R; r++) {
+          Cplx<T> w = tw[off + (r - 1) * Ls + k];
+          if (dir > 0) w.y = -w.y;
+          y[b][r] = cmul(w, y[b][r]);
         }
-        const Cplx<T> s02 = {a0.x + a2.x, a0.y + a2.y}, d02 = {a0.x - a2.x, a0.y - a2.y};
-        const Cplx<T> s13 = {a1.x + a3.x, a1.y + a3.y}, d13 = {a1.x - a3.x, a1.y - a3.y};
-        const Cplx<T> jd = dir < 0 ? Cplx<T>{d13.y, -d13.x} : Cplx<T>{-d13.y, d13.x};
-        y0[r] = {s02.x + s13.x, s02.y + s13.y};
-        y2[r] = {s02.x - s13.x, s02.y - s13.y};
-        y1[r] = {d02.x + jd.x, d02.y + jd.y};
-        y3[r] = {d02.x - jd.x, d02.y - jd.y};
-        idx[r] = ((t - k) << 2) + k;
       }
+      dft_small<T, R>(y[b], dir);
+      idx[b] = (t - k) * R + k;
     }
-    if (Ls > 1) off += 3 * Ls;
-    __syncthreads();
+  }
+  __syncthreads();
 #pragma unroll
-    for (int r = 0; r < R4; r++) {
-      if (idx[r] >= 0) {
-        z[idx[r]] = y0[r];
-        z[idx[r] + Ls] = y1[r];
-        z[idx[r] + 2 * Ls] = y2[r];
-        z[idx[r] + 3 * Ls] = y3[r];
-      }
+  for (int b = 0; b < NB; b++) {
+    if (idx[b] >= 0) {
+#pragma unroll
+      for (int q = 0; q < R; q++) z[idx[b] + q * Ls] = y[b][q];
     }
-    __syncthreads();
+  }
+  __syncthreads();
+}
+
+// The transform (N <= 2048, 256 lanes) over the table of pass_twiddles8.
+template <class T>
+__device__ __forceinline__ void stockham8(Cplx<T>* z, const Cplx<T>* tw8, int N, int logN, int dir) {
+  int Ls = 1, off = 0;
+  const int first = fft8_first_radix(logN);
+  if (first == 2) {
+    radix_pass<T, 2, 4>(z, tw8, N, 1, 0, dir);
+    Ls = 2;
+  } else if (first == 4) {
+    radix_pass<T, 4, 2>(z, tw8, N, 1, 0, dir);
+    Ls = 4;
+  }
+  for (; Ls < N; Ls <<= 3) {
+    radix_pass<T, 8, 1>(z, tw8, N, Ls, off, dir);
+    if (Ls > 1) off += 7 * Ls;
   }
 }
 
@@ -245,7 +299,8 @@ __device__ void stockham4p(Cplx<T>* z, const Cplx<T>* tw4, int N, int logN, int 
 // him[k] (k <= N/2, 1/N folded in), recombined, inverse FFT.  On return the
 // filtered rows are z[j].x (va) and z[j].y (vb), j < n, in LDS; z and tw
 // are LDS arrays of the caller, tw of fft_tw_entries<T>(N) twiddles: FP32
-// transforms up to 2048 points take the per-pass table (pass_twiddles),
+// transforms up to 2048 points take the mixed-radix transform and its
+// per-pass table (pass_twiddles8),
 // longer ones and FP64 ones the half table (N/2 entries), which keeps a
 // 4096-point FP32 or a 2048-point FP64 transform within 48 KB of LDS.
 template <class T>
@@ -258,38 +313,66 @@ __host__ __device__ constexpr int fft_tw_entries(int N) {
 }
 
 template <class T, int R>
-__device__ void fft_filter_pair(const float* __restrict__ in, const double* __restrict__ w,
+__device__ __forceinline__ void fft_filter_pair(const float* __restrict__ in, const double* __restrict__ w,
                                 const T* __restrict__ hre, const T* __restrict__ him,
                                 const Cplx<T>* __restrict__ twg, int va, int vb, int n, int N,
                                 int logN, Cplx<T>* z, Cplx<T>* tw) {
+  // Every global read of the pair is issued before the first LDS store (a
+  // loop of load-then-store pays the L2 latency once per iteration): 256
+  // lanes, IT = N/256 input cells and HT spectrum multipliers per lane.
+  constexpr int IT = 2 * R, HT = R + 1;
   const bool perPass = fft_per_pass<T>(N);
-  if (perPass) pass_twiddles(twg, tw, N, logN);
-  else for (int k = threadIdx.x; k < N / 2; k += blockDim.x) tw[k] = twg[k];
-  for (int j = threadIdx.x; j < N; j += blockDim.x) {
-    T a = 0, b = 0;
+  float xa[IT], xb[IT];
+  double wj[IT];
+#pragma unroll
+  for (int i = 0; i < IT; i++) {
+    const int j = threadIdx.x + i * blockDim.x;
+    xa[i] = xb[i] = 0.f;
+    wj[i] = 1.0;
     if (j < n) {
-      double xa = static_cast<double>(in[static_cast<size_t>(va) * n + j]);
-      double xb = vb >= 0 ? static_cast<double>(in[static_cast<size_t>(vb) * n + j]) : 0.0;
-      if (w) {
-        xa *= w[j];
-        xb *= w[j];
-      }
-      a = static_cast<T>(xa);
-      b = static_cast<T>(xb);
+      xa[i] = in[static_cast<size_t>(va) * n + j];
+      if (vb >= 0) xb[i] = in[static_cast<size_t>(vb) * n + j];
+      if (w) wj[i] = w[j];
     }
-    z[j] = {a, b};
+  }
+  T mr[HT], mi[HT];
+#pragma unroll
+  for (int i = 0; i < HT; i++) {
+    const int k = threadIdx.x + i * blockDim.x;
+    mr[i] = mi[i] = T(0);
+    if (k <= N / 2) {
+      mr[i] = hre[k];
+      mi[i] = him[k];
+    }
+  }
+  if (perPass) pass_twiddles8(twg, tw, N, logN);
+  else for (int k = threadIdx.x; k < N / 2; k += blockDim.x) tw[k] = twg[k];
+#pragma unroll
+  for (int i = 0; i < IT; i++) {
+    const int j = threadIdx.x + i * blockDim.x;
+    if (j < N) {
+      double da = static_cast<double>(xa[i]), db = static_cast<double>(xb[i]);
+      if (w) {
+        da *= wj[i];
+        db *= wj[i];
+      }
+      z[j] = {static_cast<T>(da), static_cast<T>(db)};
+    }
   }
   __syncthreads();
-  if (perPass) stockham4p<T, R>(z, tw, N, logN, -1);
+  if (perPass) stockham8<T>(z, tw, N, logN, -1);
   else stockham4<T, R>(z, tw, N, logN, -1);
   // Split the two real spectra, apply the multipliers, recombine.
-  for (int k = threadIdx.x; k <= N / 2; k += blockDim.x) {
+#pragma unroll
+  for (int i = 0; i < HT; i++) {
+    const int k = threadIdx.x + i * blockDim.x;
+    if (k > N / 2) break;
     const int kn = (N - k) & (N - 1);
     const Cplx<T> zk = z[k], zn = z[kn];
     // X_a = (Z_k + conj Z_{N-k}) / 2, X_b = (Z_k - conj Z_{N-k}) / (2i)
     const T ar = (zk.x + zn.x) * T(0.5), ai = (zk.y - zn.y) * T(0.5);
     const T br = (zk.y + zn.y) * T(0.5), bi = (zn.x - zk.x) * T(0.5);
-    const T hr = hre[k], hi = him[k];
+    const T hr = mr[i], hi = mi[i];
     const T yar = hr * ar, yai = hi * ai, ybr = hr * br, ybi = hi * bi;
     // W_k = Y_a + i Y_b, W_{N-k} = conj(Y_a) + i conj(Y_b)
     const Cplx<T> wk = {yar - ybi, yai + ybr};
@@ -299,7 +382,7 @@ __device__ void fft_filter_pair(const float* __restrict__ in, const double* __re
     if (kn != k) z[kn] = wn;
   }
   __syncthreads();
-  if (perPass) stockham4p<T, R>(z, tw, N, logN, +1);
+  if (perPass) stockham8<T>(z, tw, N, logN, +1);
   else stockham4<T, R>(z, tw, N, logN, +1);
 }
 

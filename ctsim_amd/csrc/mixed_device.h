@@ -90,11 +90,31 @@ struct MixedArgs {
   float dirf;
 };
 
+// Loads of one row of interval coefficients through a buffer resource: the
+// hardware range check returns zeros for cells outside [0, n) (a negative
+// cell's byte offset wraps to a large unsigned value), which is the zero
+// padding the windows need, without a compare per cell.  The row base must be
+// wave-uniform; n = 0 makes every load return zeros.
+template <class CT>
+struct RowLoad {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ RowLoad(const void* coef, size_t rowStart, int n)
+      : r(__builtin_amdgcn_make_buffer_rsrc(
+            const_cast<CT*>(static_cast<const CT*>(coef) + rowStart), 0,
+            n * static_cast<int>(sizeof(CT)), 0x00020000)) {}
+  __device__ CT operator()(int cell) const {
+    if constexpr (sizeof(CT) == 16)
+      return __builtin_bit_cast(CT, __builtin_amdgcn_raw_buffer_load_b128(r, cell * 16, 0, 0));
+    else
+      return __builtin_bit_cast(CT, __builtin_amdgcn_raw_buffer_load_b64(r, cell * 8, 0, 0));
+  }
+};
+
 // One window of staged coefficients in the symmetric kernel.
 struct SymWin {
-  int lo;        // first detector cell staged
+  float sizeM1;  // cells staged - 1 (the window-relative clamp)
   int size;      // cells staged
-  int base;      // LDS offset of the window of row v (row v + s*Q at base + s*size)
+  int base;      // LDS offset of the window: cell lo + j of row v + s*Q at base + s*size + j
   float cshift;  // centre - lo
 };
 
@@ -216,16 +236,26 @@ __device__ __forceinline__ void bp_mixed_tile(const MixedArgs& a, int tile, int 
       s_v[t] = make_float4(c.ca, c.sa, a.center - static_cast<float>(lo), 0.f);
     }
     __syncthreads();
-    for (int vv = 0; vv < nv; vv++) {
-      const int lo = s_lo[vv];
-      const CT* crow = coef + static_cast<size_t>(v0 + vv) * a.nDet;
-      for (int j = t; j < W; j += 256) {
-        const int d = lo + j;
-        CT val;
-        if (d >= 0 && d < a.nDet) val = crow[d];
-        else if constexpr (INTERP == INTERP_CUBIC) val = make_float4(0.f, 0.f, 0.f, 0.f);
-        else val = make_float2(0.f, 0.f);
-        s_w[vv * W + j] = val;
+    // stage the windows, four views and two cells per lane at a time (eight
+    // loads in flight; cells outside the detector read as zero)
+    for (int vq = 0; vq < nv; vq += 4) {
+      for (int jb = 0; jb < W; jb += 512) {
+        CT val[4][2];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int vv = vq + u;
+          const RowLoad<CT> row(coef, static_cast<size_t>(v0 + min(vv, nv - 1)) * a.nDet, vv < nv ? a.nDet : 0);
+          const int lo = s_lo[min(vv, nv - 1)];
+          val[u][0] = row(lo + jb + t);
+          val[u][1] = row(lo + jb + t + 256);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+          for (int h = 0; h < 2; h++) {
+            const int j = jb + t + 256 * h;
+            if (vq + u < nv && j < W) s_w[(vq + u) * W + j] = val[u][h];
+          }
       }
     }
     __syncthreads();
@@ -348,8 +378,11 @@ __device__ __forceinline__ void pos_member(const MixedArgs& a, float A, float B,
 // (corner positions, two cells of margin), lays the windows out in LDS by a
 // prefix sum and takes as many base views as fit in capW coefficients (one
 // always fits: capW >= 4*S*W); the workgroup stages rows v + s*Q of each
-// window; every lane then accumulates its 4P pixels in FP32 per chunk with a
-// Kahan fold across chunks.
+// window (rows of one window adjacent; interleaving them so that a member's
+// S reads differ only in an immediate offset saves an add per update but
+// spreads a wave's reads over four times the banks, and measured slower);
+// every lane then accumulates its 4P pixels in FP32 and folds the sum into
+// Kahan-compensated totals every 8 or more base views.
 template <int GK, int INTERP, int P, int DEG, int S, int K>
 __device__ __forceinline__ void bp_sym_tile(const MixedArgs& a, int tile, int grp,
                                             CoefT<INTERP>* s_w, float2* s_ab, SymWin* s_win,
@@ -373,11 +406,14 @@ __device__ __forceinline__ void bp_sym_tile(const MixedArgs& a, int tile, int gr
   const int vBegin = grp * a.viewsPerGroup;
   const int vEnd = min(a.Q, vBegin + a.viewsPerGroup);
   const int maxChunk = min(a.chunk, 16);
-  float sum[4][P], comp[4][P];
+  // part: FP32 sums of up to about 8 base views (32 updates per pixel when
+  // S = 4), folded into the Kahan-compensated sum, comp
+  float sum[4][P], comp[4][P], part[4][P];
 #pragma unroll
   for (int m = 0; m < 4; m++)
 #pragma unroll
-    for (int k = 0; k < P; k++) sum[m][k] = comp[m][k] = 0.f;
+    for (int k = 0; k < P; k++) sum[m][k] = comp[m][k] = part[m][k] = 0.f;
+  int pending = 0;
   const CT* coef = static_cast<const CT*>(a.coef);
   const float dirf = a.dirf;
 
@@ -431,7 +467,7 @@ __device__ __forceinline__ void bp_sym_tile(const MixedArgs& a, int tile, int gr
       const unsigned long long fit = __ballot(m == 3 && t < 4 * cmax && cum <= a.capW);
       if (t < 4 * cmax) {
         SymWin w;
-        w.lo = lo;
+        w.sizeM1 = static_cast<float>(size - 1);
         w.size = size;
         w.base = cum - need;
         w.cshift = a.center - static_cast<float>(lo);
@@ -440,30 +476,36 @@ __device__ __forceinline__ void bp_sym_tile(const MixedArgs& a, int tile, int gr
       if (t == 0) *s_n = max(1, __popcll(fit));
     }
     __syncthreads();
-    const int nv = *s_n;
-    {  // stage: wave w takes segments (vv, m, s) = w, w + 4, ...
-      const int wave = t >> 6, lane = t & 63;
-      for (int seg = wave; seg < nv * 4 * S; seg += 4) {
-        const int s = seg % S, vm = seg / S;
-        const SymWin w = s_win[vm];
-        const CT* crow = coef + static_cast<size_t>(v0 + (vm >> 2) + s * a.Q) * a.nDet;
-        CT* dst = s_w + w.base + s * w.size;
-        for (int j = lane; j < w.size; j += 64) {
-          const int d = w.lo + j;
-          CT val;
-          if (d >= 0 && d < a.nDet) val = crow[d];
-          else if constexpr (INTERP == INTERP_CUBIC) val = make_float4(0.f, 0.f, 0.f, 0.f);
-          else val = make_float2(0.f, 0.f);
-          dst[j] = val;
+    const int nv = __builtin_amdgcn_readfirstlane(*s_n);
+    {  // stage: wave w takes segments (vv, m, s) = w, w + 4, ..., two at a
+       // time with two cells per lane each, so four loads are in flight
+      const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
+      const int nseg = nv * 4 * S;
+      for (int seg = wave; seg < nseg; seg += 8) {
+        const bool two = seg + 4 < nseg;
+        const SymWin w0 = s_win[seg / S];
+        const SymWin w1 = s_win[two ? (seg + 4) / S : seg / S];
+        // first cell staged: lo = centre - cshift (integers below 2^24, exact)
+        const int lo0 = __builtin_amdgcn_readfirstlane(static_cast<int>(a.center - w0.cshift));
+        const int n0 = __builtin_amdgcn_readfirstlane(w0.size);
+        const int lo1 = __builtin_amdgcn_readfirstlane(static_cast<int>(a.center - w1.cshift));
+        const int n1 = two ? __builtin_amdgcn_readfirstlane(w1.size) : 0;
+        const RowLoad<CT> r0(coef, static_cast<size_t>(v0 + (seg / S >> 2) + (seg % S) * a.Q) * a.nDet, a.nDet);
+        const RowLoad<CT> r1(coef, static_cast<size_t>(v0 + ((seg + 4) / S >> 2) + ((seg + 4) % S) * a.Q) * a.nDet,
+                             two ? a.nDet : 0);
+        CT* d0 = s_w + __builtin_amdgcn_readfirstlane(w0.base) + (seg % S) * n0;
+        CT* d1 = s_w + __builtin_amdgcn_readfirstlane(w1.base) + ((seg + 4) % S) * n1;
+        for (int j0 = 0; j0 < max(n0, n1); j0 += 128) {
+          const int ja = j0 + lane, jb = ja + 64;
+          const CT a0 = r0(lo0 + ja), a1 = r0(lo0 + jb), b0 = r1(lo1 + ja), b1 = r1(lo1 + jb);
+          if (ja < n0) d0[ja] = a0;
+          if (jb < n0) d0[jb] = a1;
+          if (ja < n1) d1[ja] = b0;
+          if (jb < n1) d1[jb] = b1;
         }
       }
     }
     __syncthreads();
-    float part[4][P];
-#pragma unroll
-    for (int m = 0; m < 4; m++)
-#pragma unroll
-      for (int k = 0; k < P; k++) part[m][k] = 0.f;
     for (int vv = 0; vv < nv; vv++) {
       const float2 ab = s_ab[vv];
       const float xca = x * ab.x, xsa = x * ab.y;
@@ -486,7 +528,7 @@ __device__ __forceinline__ void bp_sym_tile(const MixedArgs& a, int tile, int gr
           else pos_member<GK, DEG, 3>(a, Am, Bm, r2mF2[k], w[m].cshift, pos, wg[m]);
           u[m] = __builtin_amdgcn_fractf(pos);
           j[m] = w[m].base +
-                 static_cast<int>(__builtin_amdgcn_fmed3f(pos, 0.f, static_cast<float>(w[m].size - 1)));
+                 static_cast<int>(__builtin_amdgcn_fmed3f(pos, 0.f, w[m].sizeM1));
         }
 #pragma unroll
         for (int s = 0; s < S; s++) {
@@ -506,16 +548,21 @@ __device__ __forceinline__ void bp_sym_tile(const MixedArgs& a, int tile, int gr
         }
       }
     }
-#pragma unroll
-    for (int m = 0; m < 4; m++)
-#pragma unroll
-      for (int k = 0; k < P; k++) {  // Kahan step with the chunk's sum
-        const float yk = part[m][k] - comp[m][k];
-        const float tk = sum[m][k] + yk;
-        comp[m][k] = (tk - sum[m][k]) - yk;
-        sum[m][k] = tk;
-      }
     v0 += nv;
+    pending += nv;
+    if (pending >= 8 || v0 >= vEnd) {  // Kahan step with the last few chunks' sum
+#pragma unroll
+      for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int k = 0; k < P; k++) {
+          const float yk = part[m][k] - comp[m][k];
+          const float tk = sum[m][k] + yk;
+          comp[m][k] = (tk - sum[m][k]) - yk;
+          sum[m][k] = tk;
+          part[m][k] = 0.f;
+        }
+      pending = 0;
+    }
     __syncthreads();
   }
   if (ix >= h) return;
@@ -526,6 +573,202 @@ __device__ __forceinline__ void bp_sym_tile(const MixedArgs& a, int tile, int gr
     if (iy >= h) break;
     // orbit pixels (column, row): R (i, j) = (n-1-j, i); clockwise steps
     // swap the images of R and R^3
+    const int c1 = dirf > 0.f ? n - 1 - iy : iy, r1 = dirf > 0.f ? ix : n - 1 - ix;
+    const size_t idx[4] = {static_cast<size_t>(ix) * n + iy, static_cast<size_t>(c1) * n + r1,
+                           static_cast<size_t>(n - 1 - ix) * n + (n - 1 - iy),
+                           static_cast<size_t>(n - 1 - c1) * n + (n - 1 - r1)};
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const float v = sum[m][k] - comp[m][k];
+      if (a.split > 1) out[idx[m]] = v;
+      else out[idx[m]] = a.overwrite ? a.endScale * v : fmaf(a.endScale, v, out[idx[m]]);
+    }
+  }
+}
+
+// Window of orbit member m of a base tile (corners xa..xb, ya..yb) in one
+// view: the first cell and the number of cells (at most W) its pixels can
+// read, with two cells of margin on each side.
+template <int GK, int DEG>
+__device__ __forceinline__ void member_window(const MixedArgs& a, int m, float ca, float sa, float xa,
+                                              float xb, float ya, float yb, float dirf, int& lo,
+                                              int& size) {
+  lo = -2;
+  size = a.W;
+  if (a.W >= a.nDet + 4) return;  // full rows
+  const float F2 = a.F * a.F;
+  const float sg = (m & 1) ? dirf : 1.f;
+  float pmin = 3.0e38f, pmax = -3.0e38f;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const float cx = (q & 1) ? xb : xa, cy = (q & 2) ? yb : ya;
+    const float A = sg * fmaf(cy, sa, cx * ca), B = sg * fmaf(-cy, ca, cx * sa);
+    const float r2 = fmaf(cx, cx, fmaf(cy, cy, -F2));
+    float pos, w;
+    switch (m) {
+      case 0: pos_member<GK, DEG, 0>(a, A, B, r2, a.center, pos, w); break;
+      case 1: pos_member<GK, DEG, 1>(a, A, B, r2, a.center, pos, w); break;
+      case 2: pos_member<GK, DEG, 2>(a, A, B, r2, a.center, pos, w); break;
+      default: pos_member<GK, DEG, 3>(a, A, B, r2, a.center, pos, w); break;
+    }
+    pmin = fminf(pmin, pos);
+    pmax = fmaxf(pmax, pos);
+  }
+  if (pmin > -1e9f && pmax < 1e9f) {
+    lo = static_cast<int>(floorf(pmin)) - 2;
+    size = min(static_cast<int>(floorf(pmax)) + 3 - lo, a.W);
+  } else {
+    lo = static_cast<int>(floorf(pmin > -1e9f && pmin < 1e9f ? pmin : 0.f)) - 2;
+  }
+}
+
+// The symmetric body of bp_sym_tile, software-pipelined one base view deep:
+// while the lanes interpolate base view v from LDS, the global loads of
+// view v + 1's windows are in flight into registers; they go to LDS after a
+// barrier.  Wave w stages member w's S rows (at most 128 cells each, two per
+// lane), so each LDS slot (member m, row s) sits at a fixed offset
+// (m S + s) W and needs no layout pass; windows and view constants are
+// computed for 64 base views at a time, one (view, member) per lane.  The
+// LDS holds 4 S W coefficients (capW >= 4 S W, W <= 128).
+template <int GK, int INTERP, int P, int DEG, int S, int K>
+__device__ __forceinline__ void bp_sym_tile_pipe(const MixedArgs& a, int tile, int grp,
+                                                 CoefT<INTERP>* s_w, float2* s_ab, SymWin* s_win) {
+  using CT = CoefT<INTERP>;
+  const int t = threadIdx.x;
+  const int h = a.half, n = 2 * a.half, W = a.W;
+  const int tx = 256 / a.lanesY, ty = a.lanesY * P;
+  const int ix0 = (tile / a.tilesY) * tx, iy0 = (tile % a.tilesY) * ty;
+  const int ix = ix0 + t / a.lanesY, iyb = iy0 + (t % a.lanesY) * P;
+  const int cx1 = min(ix0 + tx - 1, h - 1), cy1 = min(iy0 + ty - 1, h - 1);
+  const float xa = a.xs[ix0], xb = a.xs[cx1], ya = a.ys[iy0], yb = a.ys[cy1];
+  const float x = a.xs[min(ix, h - 1)];
+  const float F2 = a.F * a.F;
+  float y[P], r2mF2[P];
+#pragma unroll
+  for (int k = 0; k < P; k++) {
+    y[k] = a.ys[min(iyb + k, h - 1)];
+    r2mF2[k] = fmaf(x, x, fmaf(y[k], y[k], -F2));
+  }
+  const int vBegin = grp * a.viewsPerGroup;
+  const int vEnd = min(a.Q, vBegin + a.viewsPerGroup);
+  float sum[4][P], comp[4][P], part[4][P];
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+#pragma unroll
+    for (int k = 0; k < P; k++) sum[m][k] = comp[m][k] = part[m][k] = 0.f;
+  int pending = 0;
+  const float dirf = a.dirf;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
+  CT* slot = s_w + wave * S * W;  // this wave's member rows
+
+  // global loads of member `wave`'s S rows of base view v0 + vv
+  auto fetch = [&](int vv, int v0, CT (&r)[S][2]) {
+    const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(a.center - s_win[vv * 4 + wave].cshift));
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const RowLoad<CT> row(a.coef, static_cast<size_t>(v0 + vv + s * a.Q) * a.nDet, a.nDet);
+      r[s][0] = row(lo + lane);
+      r[s][1] = row(lo + lane + 64);
+    }
+  };
+  auto place = [&](int vv, const CT (&r)[S][2]) {
+    const int size = __builtin_amdgcn_readfirstlane(s_win[vv * 4 + wave].size);
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      if (lane < size) slot[s * W + lane] = r[s][0];
+      if (lane + 64 < size) slot[s * W + lane + 64] = r[s][1];
+    }
+  };
+
+  for (int v0 = vBegin; v0 < vEnd; v0 += 64) {
+    const int nb = min(64, vEnd - v0);
+    __syncthreads();  // the previous block's readers of s_win and s_w are done
+    if (t < 4 * nb) {
+      const int vv = t >> 2, m = t & 3;
+      const ViewConstF32 c = a.vc[v0 + vv];
+      if (m == 0) s_ab[vv] = make_float2(c.ca, c.sa);
+      int lo, size;
+      member_window<GK, DEG>(a, m, c.ca, c.sa, xa, xb, ya, yb, dirf, lo, size);
+      SymWin w;
+      w.sizeM1 = static_cast<float>(size - 1);
+      w.size = size;
+      w.base = 0;
+      w.cshift = a.center - static_cast<float>(lo);
+      s_win[t] = w;
+    }
+    __syncthreads();
+    CT r[S][2];
+    fetch(0, v0, r);
+    place(0, r);
+    __syncthreads();
+    for (int vv = 0; vv < nb; vv++) {
+      const bool more = vv + 1 < nb;
+      if (more) fetch(vv + 1, v0, r);  // in flight during the interpolation below
+      const float2 ab = s_ab[vv];
+      const float xca = x * ab.x, xsa = x * ab.y;
+      SymWin w[4];
+#pragma unroll
+      for (int m = 0; m < 4; m++) w[m] = s_win[vv * 4 + m];
+#pragma unroll
+      for (int k = 0; k < P; k++) {
+        const float A = fmaf(y[k], ab.y, xca), B = fmaf(-y[k], ab.x, xsa);
+        const float Ao = dirf * A, Bo = dirf * B;
+        float u[4], wg[4];
+        int j[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          float pos;
+          const float Am = (m & 1) ? Ao : A, Bm = (m & 1) ? Bo : B;
+          if (m == 0) pos_member<GK, DEG, 0>(a, Am, Bm, r2mF2[k], w[m].cshift, pos, wg[m]);
+          else if (m == 1) pos_member<GK, DEG, 1>(a, Am, Bm, r2mF2[k], w[m].cshift, pos, wg[m]);
+          else if (m == 2) pos_member<GK, DEG, 2>(a, Am, Bm, r2mF2[k], w[m].cshift, pos, wg[m]);
+          else pos_member<GK, DEG, 3>(a, Am, Bm, r2mF2[k], w[m].cshift, pos, wg[m]);
+          u[m] = __builtin_amdgcn_fractf(pos);
+          j[m] = m * S * W + static_cast<int>(__builtin_amdgcn_fmed3f(pos, 0.f, w[m].sizeM1));
+        }
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+          CT q[4];
+#pragma unroll
+          for (int m = 0; m < 4; m++) q[m] = s_w[j[m] + s * W];
+#pragma unroll
+          for (int m = 0; m < 4; m++) {
+            float val;
+            if constexpr (INTERP == INTERP_CUBIC)
+              val = fmaf(u[m], fmaf(u[m], fmaf(u[m], q[m].w, q[m].z), q[m].y), q[m].x);
+            else
+              val = fmaf(u[m], q[m].y, q[m].x);
+            const int tgt = (m + s * K) & 3;
+            part[tgt][k] = fmaf(val, wg[m], part[tgt][k]);
+          }
+        }
+      }
+      if (++pending >= 8 || vv + 1 == nb) {  // Kahan step with the last few views' sum
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+          for (int k = 0; k < P; k++) {
+            const float yk = part[m][k] - comp[m][k];
+            const float tk = sum[m][k] + yk;
+            comp[m][k] = (tk - sum[m][k]) - yk;
+            sum[m][k] = tk;
+            part[m][k] = 0.f;
+          }
+        pending = 0;
+      }
+      __syncthreads();  // every lane is done reading view vv
+      if (more) {
+        place(vv + 1, r);
+        __syncthreads();
+      }
+    }
+  }
+  if (ix >= h) return;
+  float* out = a.split == 1 ? a.image : a.partial + static_cast<size_t>(grp) * n * n;
+#pragma unroll
+  for (int k = 0; k < P; k++) {
+    const int iy = iyb + k;
+    if (iy >= h) break;
     const int c1 = dirf > 0.f ? n - 1 - iy : iy, r1 = dirf > 0.f ? ix : n - 1 - ix;
     const size_t idx[4] = {static_cast<size_t>(ix) * n + iy, static_cast<size_t>(c1) * n + r1,
                            static_cast<size_t>(n - 1 - ix) * n + (n - 1 - iy),

@@ -128,17 +128,19 @@ int ctsim_fbp_slice(ctsim_ctx* ctx, ctsim_procsignal* ps, const ctsim_backprojec
 int ctsim_backprojector_info(const ctsim_backprojector* bp, double* out8);
 /* No counterpart in the reference (launch tuning of the MIXED kernel).
  * Times candidate shapes (lanes along y, rows per lane, view groups, views
- * per LDS chunk, plain or rotationally symmetric kernel) on nView float rows
- * of d_filtered into a scratch image and keeps the fastest for later MIXED
- * launches of this plan; shape6 (optional) receives lanesY, P, split, chunk,
- * window, sym and bestMs (optional) its time. */
+ * per LDS chunk, plain or rotationally symmetric kernel and the latter's LDS
+ * staging size) on nView float rows of d_filtered into a scratch image and
+ * keeps the fastest for later MIXED launches of this plan; shape7 (optional)
+ * receives lanesY, P, split, chunk, window, sym, staging KB and bestMs
+ * (optional) its time. */
 int ctsim_backprojector_autotune(ctsim_ctx* ctx, ctsim_backprojector* bp, const float* d_filtered,
-                                 const double* h_angles, int nView, int* shape6, double* bestMs);
+                                 const double* h_angles, int nView, int* shape7, double* bestMs);
 /* Sets the MIXED launch shape explicitly: lanesY (8, 16, 32, 64), P (1, 2,
- * 4), split (1..64), chunk (1..64), sym (0, 1: the symmetric kernel where the
+ * 4), split (1..64), chunk (1..64), sym (0; 1: the symmetric kernel where the
  * image is square and centred, the views come in 2 or 4 rotated copies, and
- * P <= 2). */
-int ctsim_backprojector_set_shape(ctsim_backprojector* bp, const int* shape5);
+ * P <= 2; 2: its software-pipelined form, windows of at most 128 cells),
+ * staging KB of the chunked symmetric kernel (0..62, 0: 62). */
+int ctsim_backprojector_set_shape(ctsim_backprojector* bp, const int* shape6);
 
 /* ------------------------------------------------------------ filtering */
 

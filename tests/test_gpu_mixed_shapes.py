@@ -17,8 +17,12 @@ pytestmark = pytest.mark.gpu
 
 SHAPES = [(8, 1, 1, 4), (16, 4, 8, 16), (32, 2, 2, 32), (16, 2, 3, 5), (64, 1, 16, 1),
           (16, 4, 1, 8),
-          # the symmetric kernel: lanesY, P, split (of the base views), base views per chunk
-          (16, 1, 1, 4, 1), (8, 2, 8, 16, 1), (32, 2, 3, 5, 1), (16, 2, 1, 1, 1), (8, 1, 16, 2, 1)]
+          # the symmetric kernel: lanesY, P, split (of the base views), base views per
+          # chunk, 1, staging KB
+          (16, 1, 1, 4, 1, 0), (8, 2, 8, 16, 1, 24), (32, 2, 3, 5, 1, 0), (16, 2, 1, 1, 1, 16),
+          (8, 1, 16, 2, 1, 32),
+          # its software-pipelined form (windows of at most 128 cells)
+          (16, 1, 1, 2, 2, 0), (8, 2, 8, 2, 2, 0), (8, 1, 16, 2, 2, 0), (32, 2, 3, 2, 2, 0)]
 
 
 def _slice(ctx, geometry, nDet, nView, nx, interp, backproj, rot, filt="abs_hanning",
@@ -100,6 +104,7 @@ def test_bad_shape_rejected(gpu_ctx):
     info = g["info"]
     b = ctsim_amd.Backprojector(gpu_ctx, "parallel", 65, float(info[0]), 60, float(info[3]),
                                 float(info[4]), float(info[5]), "idiff", "linear", 48, 48)
-    for s in [(12, 1, 1, 4), (16, 3, 1, 4), (16, 1, 0, 4), (16, 1, 1, 65)]:
+    for s in [(12, 1, 1, 4), (16, 3, 1, 4), (16, 1, 0, 4), (16, 1, 1, 65), (16, 1, 1, 4, 3),
+              (16, 1, 1, 4, 1, 63), (16, 1, 1, 4, 1, -1)]:
         with pytest.raises(ctsim_amd.CtsimError):
             b.set_shape(*s)
