@@ -12,6 +12,10 @@
 
 namespace ctsim {
 
+// LDS a workgroup may allocate on gfx950 (160 KB per CU, all of it
+// addressable by one workgroup).
+constexpr size_t kLdsPerGroup = 160 * 1024;
+
 // ---------------------------------------------------------------- constants
 // include/ctsupport.h:167-177
 constexpr double kPi = 3.14159265358979323846;

@@ -352,7 +352,7 @@ int launch_fft(ctsim_ctx* ctx, const FilterPlan& p, const float* d_in, int nView
     return fail(CTSIM_ERR_UNSUPPORTED, "FFT filtering needs a power-of-two transform length "
                                        "(zeropad >= 1)");
   const size_t lds = static_cast<size_t>(N + dev::fft_tw_entries<T>(N)) * sizeof(Cplx<T>);
-  if (lds > 64 * 1024) return fail(CTSIM_ERR_UNSUPPORTED, "FFT filtering: transform too long for LDS");
+  if (lds > kLdsPerGroup) return fail(CTSIM_ERR_UNSUPPORTED, "FFT filtering: transform too long for LDS");
   const T* hre = (sizeof(T) == 8 ? p.d_hmul64 : p.d_hmul32).template as<T>();
   const T* him = hre + (N / 2 + 1);
   const Cplx<T>* tw = (sizeof(T) == 8 ? p.d_tw64 : p.d_tw32).template as<Cplx<T>>();

@@ -69,8 +69,8 @@ def test_fft_methods_against_oracle(gpu_ctx, oracle_lib, method):
     rng = np.random.default_rng(4)
     # transform lengths 64 ... 4096 (zeropad 1): every first pass of the FP32
     # mixed-radix transform (radix 8 for 64 and 512, 2 for 128 and 1024, 4
-    # for 2048) and the radix-4 half-table transform of 4096 points (FP32
-    # only: a 4096-point FP64 transform needs 96 KB of LDS and is refused)
+    # for 2048) and the radix-4 half-table transform of 4096 points (the
+    # FP64 one in 96 KB of LDS)
     cases = list(itertools.product((33, 100, 256), (0, 1, 2), ("abs_hanning", "shepp")))
     cases += [(n, 2, "abs_hanning") for n in (700, 1024, 2100)]
     for n, geom, f in cases:
@@ -79,15 +79,9 @@ def test_fft_methods_against_oracle(gpu_ctx, oracle_lib, method):
         args = dict(bw=1 / 0.013, zeropad=1, geometry=geom)
         ref, _, _ = O.filter_views("oracle", sig, f, method, inc=0.013, focal=2.1, sdl=4.2, **args)
         scale = np.abs(ref).max()
-        if n > 2048:
-            with pytest.raises(ctsim_amd.CtsimError, match="too long for LDS"):
-                _filter(gpu_ctx, sig, ctsim_amd.PREC_F64, filter=f, method=method,
-                        signalInc=0.013, n=n, focalLength=2.1, sourceDetectorLength=4.2, **args)
-        else:
-            out, _, _ = _filter(gpu_ctx, sig, ctsim_amd.PREC_F64, filter=f, method=method,
-                                signalInc=0.013, n=n, focalLength=2.1, sourceDetectorLength=4.2,
-                                **args)
-            assert np.abs(out - ref).max() <= 1e-12 * scale
+        out, _, _ = _filter(gpu_ctx, sig, ctsim_amd.PREC_F64, filter=f, method=method,
+                            signalInc=0.013, n=n, focalLength=2.1, sourceDetectorLength=4.2, **args)
+        assert np.abs(out - ref).max() <= 1e-12 * scale
         mix, _, _ = _filter(gpu_ctx, sig, ctsim_amd.PREC_MIXED, filter=f, method=method,
                             signalInc=0.013, n=n, focalLength=2.1, sourceDetectorLength=4.2, **args)
         assert np.abs(mix - ref).max() <= 2e-6 * scale
