@@ -43,13 +43,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "bp_mixed_equiangular_cubic_512_traffic.txt")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "bp_sym_pipe_equiangular_cubic_512_traffic.txt")
+# the kernel a mixed launch shape runs (its last field: 0 plain, 1 chunked
+# symmetric, 2 pipelined symmetric)
+KERNEL_BY_SYM = {0: "bp_mixed_kernel<equiangular, cubic>",
+                 1: "bp_sym_kernel<equiangular, cubic, 4 quarter turns>",
+                 2: "bp_sym_pipe_kernel<equiangular, cubic, 4 quarter turns>"}
 
 
 def measured_traffic():
-    """HBM bytes per launch of the main backprojection kernel, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or
-    None when the summary is absent."""
+    """HBM bytes per launch of the pipelined symmetric backprojection kernel
+    on this workload, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (copy-calibrated), or None when the summary is absent."""
     try:
         with open(TRAFFIC_SUMMARY) as fh:
             for line in fh:
@@ -310,8 +315,8 @@ def main():
         "accuracy_vs_fp64": {"d": d, "r": r, "e": e, "target_d": 1e-4},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": traffic,
-                     "kernel": "bp_mixed_kernel<equiangular, cubic>", "kernel_ms": bp_ms,
+                     "traffic": traffic if shape[5] == 2 else None,
+                     "kernel": KERNEL_BY_SYM[shape[5]], "kernel_ms": bp_ms,
                      "flops_per_pvu": FLOPS_PER_PVU_EQUIANGULAR_CUBIC_KAHAN,
                      "pvu_per_launch": pvu,
                      "hbm_algorithmic_bytes": hbm_bytes,
@@ -377,15 +382,15 @@ def bench_config3(ctx, args):
     img64 = torch.zeros((spec.nx, spec.ny), dtype=torch.float32, device="cuda")
     img32 = torch.zeros_like(img64)
     imgb = torch.zeros_like(img64)
-    t64 = _time(lambda: fbp.reconstruct(ctx, spec, sino, ang, info, ctsim_amd.PREC_F64, img64), 1, 1)
+    # plans (filter generation, geometry tables) are set-up, outside the timing
+    ps, bp = fbp.plans(ctx, spec, info)
+    t64 = _time(lambda: bp.fbp(ps, sino, ang, img64, ctsim_amd.PREC_F64), 1, 1)
     log(f"[bench] FP64 path: {t64 * 1e3:.1f} ms/slice")
-    ps = ctsim_amd.ProcessSignal(ctx, spec.filter, spec.method, 1 / info["detInc"], info["detInc"],
-                                 spec.nDet, -1.0, "spatial", "direct", 1, 1, spec.geometry,
-                                 info["focalLength"], info["sourceDetectorLength"])
     f32 = torch.empty((spec.nView, spec.nDet), dtype=torch.float32, device="cuda")
     t_toe = _time(lambda: ps.filter(sino, f32, ctsim_amd.PREC_MIXED), args.steps, args.warmup)
-    t_mix = _time(lambda: fbp.reconstruct(ctx, spec, sino, ang, info, ctsim_amd.PREC_MIXED, img32),
-                  args.steps, args.warmup)
+    shape = bp.autotune(f32, ang)
+    log(f"[bench] autotuned mixed shape {shape}")
+    t_mix = _time(lambda: bp.fbp(ps, sino, ang, img32, ctsim_amd.PREC_MIXED), args.steps, args.warmup)
     b = ctsim_amd.Batch(ctx, [fbp.slice_desc(spec, info, ang)])
     b.autotune([sino], [imgb])
     t_fft = _time(lambda: b.run([sino], [imgb]), args.steps, args.warmup)
@@ -402,6 +407,7 @@ def bench_config3(ctx, args):
                  fp64_gpu={"slices_per_s": 1 / t64, "ms_per_slice": t64 * 1e3},
                  toeplitz_filter={"ms": t_toe * 1e3, "macs": macs,
                                   "bf16x3_tflops": 6 * macs / t_toe / 1e12},
+                 launch_shape=list(shape),
                  fft_convolution_batch={"ms_per_slice": t_fft * 1e3, "d": d_f[0]},
                  accuracy_vs_fp64={"d": d_t[0], "r": d_t[1], "e": d_t[2]})
 

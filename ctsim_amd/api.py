@@ -223,7 +223,8 @@ class Batch:
     def status(self):
         check(lib.ctsim_batch_status(self.ctx.handle, self.handle), "ctsim_batch_status")
 
-    def set_shape(self, P: int, lanesY: int, chunkBytes: int, fused: bool, sym: bool = True):
+    def set_shape(self, P: int, lanesY: int, chunkBytes: int, fused: bool, sym: int = 1):
+        """sym: 0 plain body; 1, 2, 4 symmetric body with that many view groups."""
         check(lib.ctsim_batch_set_shape(self.ctx.handle, self.handle, P, lanesY, chunkBytes,
                                         int(fused), int(sym)), "ctsim_batch_set_shape")
 

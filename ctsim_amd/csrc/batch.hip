@@ -10,17 +10,25 @@
 //                 then the interval coefficients of the slice's
 //                 interpolation, written for its backprojection;
 //   BP items      one tile of one slice over all its views (the mixed
-//                 backprojection body), image = rotScale * sum over views.
-// Slices are dealt to the 8 XCDs by longest-processing-time first; each XCD
-// has its own queue (its filter items, then its BP items, largest slices
-// first), so a slice's coefficients are produced and consumed under one L2.
-// Workgroups pull items from their XCD's queue with an atomic counter.  A BP
-// item waits (bounded spin, error flag on expiry) until its slice's filter
-// items are done; every item it can wait on sits earlier in the same queue
-// and is held by a running workgroup, so the wait always ends.  Fused mode
-// runs both item kinds in one launch; two-phase mode launches the filter
-// items and then the BP items (no waiting, smaller LDS for the BP launch).
-// The autotuner picks the mode, the tile shape and the views per LDS chunk.
+//                 backprojection body), image = rotScale * sum over views;
+//                 with the rotationally symmetric body, one quadrant tile
+//                 and one group of base views, the tile's last group adding
+//                 the groups' partial images.
+// Slices are dealt to the 8 XCDs by longest-processing-time first, so a
+// slice's coefficients are produced and consumed under one L2; a slice that
+// costs more than 1/32 of the batch is dealt item by item over all eight.
+// Each XCD has its own queue (its filter items, then its BP items, largest
+// slices first); workgroups pull items from their XCD's queue with an
+// atomic counter.  A BP item waits (bounded spin, error flag on expiry)
+// until its slice's filter items are done.  Filter items never wait, and in
+// every queue they precede all BP items, so by the time a workgroup takes a
+// BP item every filter item of its queue has been taken by a running
+// workgroup; each filter item a BP item can wait on (in any queue) is
+// therefore finished or held by a running workgroup, and the wait ends.
+// Fused mode runs both item kinds in one launch; two-phase mode launches
+// the filter items and then the BP items (no waiting, smaller LDS for the
+// BP launch).  The autotuner picks the mode, the tile shape, the views per
+// LDS chunk and the symmetric body's view groups.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -67,6 +75,22 @@ __global__ void batch_reset_kernel(int* counters, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) counters[i] = 0;
 }
 
+// The view groups of the symmetric slices added after the backprojection
+// (a kernel boundary orders them, where a device-scope fence per workgroup
+// would write back the XCD's L2 each time): slice list[blockIdx.y],
+// image = rotScale * sum of its group partial images in group order, FP64.
+__global__ __launch_bounds__(256) void batch_combine_kernel(const SliceDev* __restrict__ slices,
+                                                            const int* __restrict__ list) {
+  const dev::MixedArgs& a = slices[list[blockIdx.y]].bp;
+  const size_t npix = static_cast<size_t>(a.nx) * a.ny;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < npix;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    double v = 0;
+    for (int g = 0; g < a.split; g++) v += a.partial[g * npix + i];
+    a.image[i] = static_cast<float>(static_cast<double>(a.endScale) * v);
+  }
+}
+
 template <int P>
 __device__ __forceinline__ void bp_item(const MixedArgs& a, int gk, int interp, int deg, int tile, void* smem,
                         float4* s_v, int* s_lo) {
@@ -90,31 +114,37 @@ __device__ __forceinline__ void bp_item(const MixedArgs& a, int gk, int interp, 
   }
 }
 
-// The symmetric body for slices whose views come in rotated copies.
+// The symmetric body (software-pipelined) for slices whose views come in
+// rotated copies.
 template <int GK, int INTERP, int P, int DEG>
-__device__ __forceinline__ void bp_sym_sk(const SliceDev& s, int tile, void* smem, float2* s_ab, dev::SymWin* s_win,
-                          int* s_n) {
+__device__ __forceinline__ void bp_sym_sk(const SliceDev& s, int tile, int grp, void* smem, float2* s_ab,
+                                          dev::SymWin* s_win) {
   auto* w = static_cast<CoefT<INTERP>*>(smem);
-  if (s.symS == 4) dev::bp_sym_tile<GK, INTERP, P, DEG, 4, 1>(s.bp, tile, 0, w, s_ab, s_win, s_n);
-  else if (s.symK == 1) dev::bp_sym_tile<GK, INTERP, P, DEG, 2, 1>(s.bp, tile, 0, w, s_ab, s_win, s_n);
-  else dev::bp_sym_tile<GK, INTERP, P, DEG, 2, 2>(s.bp, tile, 0, w, s_ab, s_win, s_n);
+  if (s.symS == 4) dev::bp_sym_tile_pipe<GK, INTERP, P, DEG, 4, 1>(s.bp, tile, grp, w, s_ab, s_win);
+  else if (s.symK == 1) dev::bp_sym_tile_pipe<GK, INTERP, P, DEG, 2, 1>(s.bp, tile, grp, w, s_ab, s_win);
+  else dev::bp_sym_tile_pipe<GK, INTERP, P, DEG, 2, 2>(s.bp, tile, grp, w, s_ab, s_win);
 }
 
 template <int P, int INTERP>
-__device__ __forceinline__ void bp_sym_interp(const SliceDev& s, int tile, void* smem, float2* s_ab, dev::SymWin* s_win,
-                              int* s_n) {
-  if (s.gk == 0) bp_sym_sk<0, INTERP, P, 0>(s, tile, smem, s_ab, s_win, s_n);
-  else if (s.gk == 2) bp_sym_sk<2, INTERP, P, 0>(s, tile, smem, s_ab, s_win, s_n);
-  else if (s.deg == 4) bp_sym_sk<1, INTERP, P, 4>(s, tile, smem, s_ab, s_win, s_n);
-  else if (s.deg == 6) bp_sym_sk<1, INTERP, P, 6>(s, tile, smem, s_ab, s_win, s_n);
-  else if (s.deg == 8) bp_sym_sk<1, INTERP, P, 8>(s, tile, smem, s_ab, s_win, s_n);
-  else bp_sym_sk<1, INTERP, P, 0>(s, tile, smem, s_ab, s_win, s_n);
+__device__ __forceinline__ void bp_sym_interp(const SliceDev& s, int tile, int grp, void* smem,
+                                              float2* s_ab, dev::SymWin* s_win) {
+  if (s.gk == 0) bp_sym_sk<0, INTERP, P, 0>(s, tile, grp, smem, s_ab, s_win);
+  else if (s.gk == 2) bp_sym_sk<2, INTERP, P, 0>(s, tile, grp, smem, s_ab, s_win);
+  else if (s.deg == 4) bp_sym_sk<1, INTERP, P, 4>(s, tile, grp, smem, s_ab, s_win);
+  else if (s.deg == 6) bp_sym_sk<1, INTERP, P, 6>(s, tile, grp, smem, s_ab, s_win);
+  else if (s.deg == 8) bp_sym_sk<1, INTERP, P, 8>(s, tile, grp, smem, s_ab, s_win);
+  else bp_sym_sk<1, INTERP, P, 0>(s, tile, grp, smem, s_ab, s_win);
 }
 
+// R > 0: the runtime-length transform with R butterflies per lane; R < 0:
+// the compile-time transform of 2^-R points.
 template <int R>
 __device__ __forceinline__ void filter_item(const SliceDev& s, int pair, Cplx<float>* z, Cplx<float>* tw) {
   const int va = 2 * pair, vb = va + 1 < s.nView ? va + 1 : -1;
-  dev::fft_filter_pair<float, R>(s.sino, s.w, s.hre, s.him, s.tw, va, vb, s.n, s.N, s.logN, z, tw);
+  if constexpr (R < 0)
+    dev::fft_filter_pair_c<-R>(s.sino, s.w, s.hre, s.him, s.tw, va, vb, s.n, z, tw);
+  else
+    dev::fft_filter_pair<float, R>(s.sino, s.w, s.hre, s.him, s.tw, va, vb, s.n, s.N, s.logN, z, tw);
   // interval coefficients of both rows straight from LDS
   const int n = s.n;
   auto ra = [z](int j) { return z[j].x; };
@@ -145,23 +175,21 @@ __device__ __forceinline__ void filter_item(const SliceDev& s, int pair, Cplx<fl
 // and the backprojection items: the fused launch takes both, the two-phase
 // launches one each, so each phase runs at its own register budget.
 template <int P, bool SYM, bool DO_F, bool DO_B>
-__global__ __launch_bounds__(256) void fbp_batch_kernel(const SliceDev* __restrict__ slices,
-                                                        const Item* __restrict__ items,
-                                                        const int* __restrict__ queueStart,
-                                                        int* counters, int* error) {
+__device__ __forceinline__ void fbp_batch_body(const SliceDev* __restrict__ slices,
+                                               const Item* __restrict__ items,
+                                               const int* __restrict__ queueStart, int* counters,
+                                               int* error) {
   extern __shared__ float4 smem[];
   // s_meta holds the plain body's per-view constants (kMaxChunk + 2 float4)
-  // or the symmetric body's 64 windows and 16 view constants, so the static
-  // LDS stays under 1.5 KB beside up to 62 KB of staging
+  // or the symmetric body's 256 windows and 64 view constants
   static_assert(sizeof(dev::SymWin) == 16, "window record is one float4");
-  constexpr int kMeta = (dev::kMaxChunk + 2) > 72 ? (dev::kMaxChunk + 2) : 72;
+  constexpr int kMeta = SYM ? 288 : dev::kMaxChunk + 2;
   __shared__ float4 s_meta[kMeta];
   __shared__ int s_lo[dev::kMaxChunk];
-  __shared__ int s_n;
   __shared__ int s_item;
   float4* s_v = s_meta;
   dev::SymWin* s_win = reinterpret_cast<dev::SymWin*>(s_meta);
-  float2* s_ab = reinterpret_cast<float2*>(s_meta + 64);
+  float2* s_ab = reinterpret_cast<float2*>(s_meta + 256);
   const int q = blockIdx.x & 7;
   const int qBegin = queueStart[q], qEnd = queueStart[q + 1];
   int* done = counters + 8;
@@ -179,11 +207,20 @@ __global__ __launch_bounds__(256) void fbp_batch_kernel(const SliceDev* __restri
       if constexpr (DO_F) {
         Cplx<float>* z = reinterpret_cast<Cplx<float>*>(smem);
         Cplx<float>* tw = z + s.N;
-        switch (s.R) {
-          case 1: filter_item<1>(s, it.a, z, tw); break;
-          case 2: filter_item<2>(s, it.a, z, tw); break;
-          case 4: filter_item<4>(s, it.a, z, tw); break;
-          default: filter_item<8>(s, it.a, z, tw); break;
+        switch (s.logN) {
+          case 6: filter_item<-6>(s, it.a, z, tw); break;
+          case 7: filter_item<-7>(s, it.a, z, tw); break;
+          case 8: filter_item<-8>(s, it.a, z, tw); break;
+          case 9: filter_item<-9>(s, it.a, z, tw); break;
+          case 10: filter_item<-10>(s, it.a, z, tw); break;
+          case 11: filter_item<-11>(s, it.a, z, tw); break;
+          default:
+            switch (s.R) {
+              case 1: filter_item<1>(s, it.a, z, tw); break;
+              case 2: filter_item<2>(s, it.a, z, tw); break;
+              case 4: filter_item<4>(s, it.a, z, tw); break;
+              default: filter_item<8>(s, it.a, z, tw); break;
+            }
         }
         __threadfence();  // every lane releases its coefficient stores device-wide
         __syncthreads();
@@ -205,8 +242,8 @@ __global__ __launch_bounds__(256) void fbp_batch_kernel(const SliceDev* __restri
       __syncthreads();
       if constexpr (SYM && P <= 2) {
         if (s.sym) {
-          if (s.interp == INTERP_CUBIC) bp_sym_interp<P, INTERP_CUBIC>(s, it.a, smem, s_ab, s_win, &s_n);
-          else bp_sym_interp<P, INTERP_LINEAR>(s, it.a, smem, s_ab, s_win, &s_n);
+          if (s.interp == INTERP_CUBIC) bp_sym_interp<P, INTERP_CUBIC>(s, it.a, it.pad, smem, s_ab, s_win);
+          else bp_sym_interp<P, INTERP_LINEAR>(s, it.a, it.pad, smem, s_ab, s_win);
         } else {
           bp_item<P>(s.bp, s.gk, s.interp, s.deg, it.a, smem, s_v, s_lo);
         }
@@ -216,6 +253,23 @@ __global__ __launch_bounds__(256) void fbp_batch_kernel(const SliceDev* __restri
     }
     __syncthreads();  // LDS is reused by the next item
   }
+}
+
+// The fused and the filter launches, and the backprojection launch.
+template <int P, bool SYM, bool DO_F, bool DO_B>
+__global__ __launch_bounds__(256) void fbp_batch_kernel(const SliceDev* __restrict__ slices,
+                                                        const Item* __restrict__ items,
+                                                        const int* __restrict__ queueStart,
+                                                        int* counters, int* error) {
+  fbp_batch_body<P, SYM, DO_F, DO_B>(slices, items, queueStart, counters, error);
+}
+
+template <int P, bool SYM>
+__global__ __launch_bounds__(256) void fbp_batch_bp_kernel(const SliceDev* __restrict__ slices,
+                                                              const Item* __restrict__ items,
+                                                              const int* __restrict__ queueStart,
+                                                              int* counters, int* error) {
+  fbp_batch_body<P, SYM, false, true>(slices, items, queueStart, counters, error);
 }
 
 // ------------------------------------------------------------------ host
@@ -257,10 +311,14 @@ struct ctsim_batch {
   };
   std::vector<Slice> s;
   int P = 2, lanesY = 16, chunkBudget = 32 * 1024;
-  bool fused = true, sym = true;
+  bool fused = true;
+  int sym = 1;  // 0: plain body; 1, 2, 4: symmetric body with that many view groups per tile
   int ldsFilter = 0;
   std::vector<SliceDev> host;
   DevBuf d_slices, d_itemsF, d_itemsB, d_itemsAll, d_qF, d_qB, d_qAll, d_counters, d_error;
+  DevBuf d_partial;      // view-group partial images of the symmetric slices
+  DevBuf d_combine;      // slices whose view groups are added after the backprojection
+  int nCombine = 0;
   int gridF = 0, gridB = 0, gridAll = 0;
   size_t ldsB = 0, ldsAll = 0;
   std::vector<const float*> sinos;
@@ -303,6 +361,8 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
   const int S = static_cast<int>(b.s.size());
   b.host.assign(S, SliceDev{});
   size_t maxBpLds = 0;
+  std::vector<size_t> partialOff(S, 0);
+  size_t partialTotal = 0;
   for (int k = 0; k < S; k++) {
     auto& sl = b.s[k];
     const BpPlan& p = sl.bp;
@@ -341,12 +401,11 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
     m.chunk = chunk;
     if (static_cast<size_t>(chunk) * Wv * elem > 62 * 1024)
       return fail(CTSIM_ERR_UNSUPPORTED, "batch: a slice's detector window does not fit in LDS");
-    // the symmetric body when the views come in rotated copies and one base
-    // view's 4*S windows fit the LDS (at least the chunk budget, at most 62 KB)
+    // the (pipelined) symmetric body when the views come in rotated copies
+    // and the windows hold at most 128 cells; it stages 4 S W cells
     const SymViews sv = bp_sym_views(sl.angles.data(), p.nView);
-    const size_t symNeed = static_cast<size_t>(4 * sv.S) * Wv * elem;
-    const size_t symLds = std::max(static_cast<size_t>(b.chunkBudget), symNeed);
-    d.sym = b.sym && b.P <= 2 && p.symImage && sv.S > 0 && m.W > 0 && symLds <= 62 * 1024;
+    const size_t symLds = static_cast<size_t>(4 * sv.S) * Wv * elem;
+    d.sym = b.sym > 0 && b.P <= 2 && p.symImage && sv.S > 0 && m.W > 0 && m.W <= 128;
     d.symS = sv.S;
     d.symK = sv.K;
     maxBpLds = std::max(maxBpLds, d.sym ? symLds : static_cast<size_t>(chunk) * Wv * elem);
@@ -380,21 +439,61 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
       a.dirf = sv.dir;
       a.tilesX = (a.half + tx - 1) / tx;
       a.tilesY = (a.half + ty - 1) / ty;
-      a.viewsPerGroup = a.Q;
-      a.chunk = std::min(m.chunk, 16);
-      a.capW = static_cast<int>(symLds / elem);
+      a.capW = 4 * sv.S * Wv;
+      // view groups of at least 8 base views; each group's partial image
+      // goes to the batch's partial buffer, the tile's last group adds them
+      int G = std::max(1, std::min(b.sym, a.Q / 8));
+      a.viewsPerGroup = (a.Q + G - 1) / G;
+      G = (a.Q + a.viewsPerGroup - 1) / a.viewsPerGroup;
+      a.split = G;
+      if (G > 1) {
+        partialOff[k] = partialTotal;
+        partialTotal += static_cast<size_t>(G) * p.nx * p.ny;
+      }
     }
   }
-  // slices to XCDs, longest first
+  std::vector<int> combine;
+  if (partialTotal > 0) {
+    CTSIM_CHECK(b.d_partial.reserve(partialTotal * sizeof(float)));
+    for (int k = 0; k < S; k++)
+      if (b.host[k].sym && b.host[k].bp.split > 1) {
+        b.host[k].bp.partial = b.d_partial.as<float>() + partialOff[k];
+        combine.push_back(k);
+      }
+  }
+  b.nCombine = static_cast<int>(combine.size());
+  // Slices to XCDs, longest first, so that a slice's coefficients are made
+  // and read under one L2; a slice costing more than a quarter of an XCD's
+  // share is dealt item by item over all eight instead (one big slice on
+  // one XCD would leave the other seven idle).
   std::vector<int> order(S);
   for (int k = 0; k < S; k++) order[k] = k;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b.s[x].cost > b.s[y].cost; });
-  std::vector<std::vector<int>> xcdSlices(8);
+  double total = 0;
+  for (int k = 0; k < S; k++) total += b.s[k].cost;
+  std::vector<std::vector<Item>> f(8), bpq(8);
   std::vector<double> load(8, 0.0);
+  auto bpItems = [&](int k, std::vector<Item>& out, int x0, int stride) {
+    const int tiles = b.host[k].bp.tilesX * b.host[k].bp.tilesY;
+    const int groups = b.host[k].sym ? b.host[k].bp.split : 1;
+    int i = 0;
+    for (int t = 0; t < tiles; t++)
+      for (int g = 0; g < groups; g++, i++)
+        if (stride == 1 || i % stride == x0) out.push_back({kBpItem, k, t, g});
+  };
   for (int k : order) {
-    const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
-    xcdSlices[x].push_back(k);
-    load[x] += b.s[k].cost;
+    if (b.s[k].cost > total / 32) {
+      for (int pr = 0; pr < b.host[k].nFilterItems; pr++) f[pr % 8].push_back({kFilterItem, k, pr, 0});
+      for (int x = 0; x < 8; x++) {
+        bpItems(k, bpq[x], x, 8);
+        load[x] += b.s[k].cost / 8;
+      }
+    } else {
+      const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+      for (int pr = 0; pr < b.host[k].nFilterItems; pr++) f[x].push_back({kFilterItem, k, pr, 0});
+      bpItems(k, bpq[x], 0, 1);
+      load[x] += b.s[k].cost;
+    }
   }
   std::vector<Item> itemsF, itemsB, itemsAll;
   std::vector<int> qF(9), qB(9), qAll(9);
@@ -402,16 +501,10 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
     qF[x] = static_cast<int>(itemsF.size());
     qB[x] = static_cast<int>(itemsB.size());
     qAll[x] = static_cast<int>(itemsAll.size());
-    std::vector<Item> f, bp;
-    for (int k : xcdSlices[x]) {
-      for (int pr = 0; pr < b.host[k].nFilterItems; pr++) f.push_back({kFilterItem, k, pr, 0});
-      const int tiles = b.host[k].bp.tilesX * b.host[k].bp.tilesY;
-      for (int t = 0; t < tiles; t++) bp.push_back({kBpItem, k, t, 0});
-    }
-    itemsF.insert(itemsF.end(), f.begin(), f.end());
-    itemsB.insert(itemsB.end(), bp.begin(), bp.end());
-    itemsAll.insert(itemsAll.end(), f.begin(), f.end());
-    itemsAll.insert(itemsAll.end(), bp.begin(), bp.end());
+    itemsF.insert(itemsF.end(), f[x].begin(), f[x].end());
+    itemsB.insert(itemsB.end(), bpq[x].begin(), bpq[x].end());
+    itemsAll.insert(itemsAll.end(), f[x].begin(), f[x].end());
+    itemsAll.insert(itemsAll.end(), bpq[x].begin(), bpq[x].end());
   }
   qF[8] = static_cast<int>(itemsF.size());
   qB[8] = static_cast<int>(itemsB.size());
@@ -429,6 +522,7 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
   CTSIM_CHECK(up(b.d_qB, qB.data(), qB.size() * sizeof(int)));
   CTSIM_CHECK(up(b.d_qAll, qAll.data(), qAll.size() * sizeof(int)));
   CTSIM_CHECK(b.d_counters.reserve((8 + S) * sizeof(int)));
+  if (!combine.empty()) CTSIM_CHECK(up(b.d_combine, combine.data(), combine.size() * sizeof(int)));
   CTSIM_CHECK(b.d_error.reserve(sizeof(int)));
   CTSIM_HIP(hipMemset(b.d_error.ptr, 0, sizeof(int)));
   b.ldsB = maxBpLds;
@@ -442,9 +536,9 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
   const void* kAll = pick(fbp_batch_kernel<4, false, true, true>, fbp_batch_kernel<1, true, true, true>,
                           fbp_batch_kernel<1, false, true, true>, fbp_batch_kernel<2, true, true, true>,
                           fbp_batch_kernel<2, false, true, true>);
-  const void* kB = pick(fbp_batch_kernel<4, false, false, true>, fbp_batch_kernel<1, true, false, true>,
-                        fbp_batch_kernel<1, false, false, true>, fbp_batch_kernel<2, true, false, true>,
-                        fbp_batch_kernel<2, false, false, true>);
+  const void* kB = pick(fbp_batch_bp_kernel<4, false>, fbp_batch_bp_kernel<1, true>,
+                        fbp_batch_bp_kernel<1, false>, fbp_batch_bp_kernel<2, true>,
+                        fbp_batch_bp_kernel<2, false>);
   const void* kF = reinterpret_cast<const void*>(fbp_batch_kernel<1, false, true, false>);
   b.gridF = occupancy_grid(kF, b.ldsFilter, ctx->numCUs);
   b.gridB = occupancy_grid(kB, b.ldsB, ctx->numCUs);
@@ -470,10 +564,13 @@ void launch_batch(ctsim_batch& b, hipStream_t st) {
                        b.d_slices.as<SliceDev>(), b.d_itemsF.as<Item>(), b.d_qF.as<int>(),
                        b.d_counters.as<int>(), b.d_error.as<int>());
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8);
-    hipLaunchKernelGGL((fbp_batch_kernel<P, SYM, false, true>), dim3(b.gridB), dim3(256), b.ldsB, st,
+    hipLaunchKernelGGL((fbp_batch_bp_kernel<P, SYM>), dim3(b.gridB), dim3(256), b.ldsB, st,
                        b.d_slices.as<SliceDev>(), b.d_itemsB.as<Item>(), b.d_qB.as<int>(),
                        b.d_counters.as<int>(), b.d_error.as<int>());
   }
+  if (b.nCombine > 0)
+    hipLaunchKernelGGL(batch_combine_kernel, dim3(256, b.nCombine), dim3(256), 0, st,
+                       b.d_slices.as<SliceDev>(), b.d_combine.as<int>());
 }
 
 int batch_enqueue(ctsim_ctx* ctx, ctsim_batch& b) {
@@ -618,7 +715,7 @@ int ctsim_batch_set_shape(ctsim_ctx* ctx, ctsim_batch* b, int P, int lanesY, int
                           int sym) {
   if (!ctx || !b) return fail(CTSIM_ERR_INVALID, "ctsim_batch_set_shape: NULL argument");
   if (!(P == 1 || P == 2 || P == 4) || !(lanesY == 8 || lanesY == 16 || lanesY == 32) ||
-      chunkBytes < 1024 || chunkBytes > 62 * 1024)
+      chunkBytes < 1024 || chunkBytes > 62 * 1024 || !(sym == 0 || sym == 1 || sym == 2 || sym == 4))
     return fail(CTSIM_ERR_INVALID, "ctsim_batch_set_shape: shape out of range");
   CTSIM_CHECK(ctx->activate());
   CTSIM_HIP(hipStreamSynchronize(b->stream));
@@ -626,7 +723,7 @@ int ctsim_batch_set_shape(ctsim_ctx* ctx, ctsim_batch* b, int P, int lanesY, int
   b->lanesY = lanesY;
   b->chunkBudget = chunkBytes;
   b->fused = fused != 0;
-  b->sym = sym != 0;
+  b->sym = sym;
   return batch_layout(ctx, *b);
 }
 
@@ -674,7 +771,7 @@ int ctsim_batch_autotune(ctsim_ctx* ctx, ctsim_batch* b, const float* const* d_s
   int rc = CTSIM_OK;
   // P = 1 only with the symmetric body, P = 4 only without (it has none)
   for (int P : {1, 2, 4})
-    for (int sym : {1, 0})
+    for (int sym : {1, 4, 0})
     for (int lanesY : {8, 16, 32})
       for (int chunk : {16 * 1024, 32 * 1024})
         for (int fused : {1, 0}) {

@@ -386,5 +386,178 @@ __device__ __forceinline__ void fft_filter_pair(const float* __restrict__ in, co
   else stockham4<T, R>(z, tw, N, logN, +1);
 }
 
+// ---------------------------------------------------------------------
+// The FP32 transform with the length and the direction known at compile time
+// (N = 2^LOGN <= 2048, 256 lanes): the pass sequence unrolls, LDS offsets
+// become immediates, and no lane bound or direction select is evaluated.
+// Same passes, tables and arithmetic as stockham8.
+
+template <int DIR, class T>
+__device__ __forceinline__ void dft4c(Cplx<T>& c0, Cplx<T>& c1, Cplx<T>& c2, Cplx<T>& c3) {
+  const Cplx<T> s02 = {c0.x + c2.x, c0.y + c2.y}, d02 = {c0.x - c2.x, c0.y - c2.y};
+  const Cplx<T> s13 = {c1.x + c3.x, c1.y + c3.y}, d13 = {c1.x - c3.x, c1.y - c3.y};
+  const Cplx<T> jd = DIR < 0 ? Cplx<T>{d13.y, -d13.x} : Cplx<T>{-d13.y, d13.x};
+  c0 = {s02.x + s13.x, s02.y + s13.y};
+  c2 = {s02.x - s13.x, s02.y - s13.y};
+  c1 = {d02.x + jd.x, d02.y + jd.y};
+  c3 = {d02.x - jd.x, d02.y - jd.y};
+}
+
+template <int DIR, int R, class T>
+__device__ __forceinline__ void dft_smallc(Cplx<T>* a) {
+  if constexpr (R == 2) {
+    const Cplx<T> s = {a[0].x + a[1].x, a[0].y + a[1].y}, d = {a[0].x - a[1].x, a[0].y - a[1].y};
+    a[0] = s;
+    a[1] = d;
+  } else if constexpr (R == 4) {
+    dft4c<DIR>(a[0], a[1], a[2], a[3]);
+  } else {
+    const T h = T(0.70710678118654752440);
+    constexpr T sg = DIR < 0 ? T(-1) : T(1);
+    Cplx<T> b[4], c[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      b[r] = {a[r].x + a[r + 4].x, a[r].y + a[r + 4].y};
+      c[r] = {a[r].x - a[r + 4].x, a[r].y - a[r + 4].y};
+    }
+    c[1] = {h * (c[1].x - sg * c[1].y), h * (c[1].y + sg * c[1].x)};
+    c[2] = {-sg * c[2].y, sg * c[2].x};
+    c[3] = {-h * (c[3].x + sg * c[3].y), h * (sg * c[3].x - c[3].y)};
+    dft4c<DIR>(b[0], b[1], b[2], b[3]);
+    dft4c<DIR>(c[0], c[1], c[2], c[3]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      a[2 * k] = b[k];
+      a[2 * k + 1] = c[k];
+    }
+  }
+}
+
+template <int DIR, int R, int LOGN, int LG, int OFF, class T>
+__device__ __forceinline__ void radix_passc(Cplx<T>* z, const Cplx<T>* tw) {
+  constexpr int N = 1 << LOGN, Ls = 1 << LG, nb = N / R;
+  constexpr int NB = nb >= 256 ? nb / 256 : 1;
+  Cplx<T> y[NB][R];
+  int idx[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    const int t = threadIdx.x + b * 256;
+    idx[b] = -1;
+    if (nb >= 256 || t < nb) {
+      const int k = t & (Ls - 1);
+#pragma unroll
+      for (int r = 0; r < R; r++) y[b][r] = z[t + r * nb];
+      if constexpr (Ls > 1) {
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+          Cplx<T> w = tw[OFF + (r - 1) * Ls + k];
+          if constexpr (DIR > 0) w.y = -w.y;
+          y[b][r] = cmul(w, y[b][r]);
+        }
+      }
+      dft_smallc<DIR, R>(y[b]);
+      idx[b] = (t - k) * R + k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    if (nb >= 256 || idx[b] >= 0) {
+#pragma unroll
+      for (int q = 0; q < R; q++) z[idx[b] + q * Ls] = y[b][q];
+    }
+  }
+  __syncthreads();
+}
+
+// radix-8 passes with span 2^LG, 2^(LG+3), ... < N; table offset OFF
+template <int DIR, int LOGN, int LG, int OFF, class T>
+__device__ __forceinline__ void radix8_passesc(Cplx<T>* z, const Cplx<T>* tw) {
+  if constexpr (LG < LOGN) {
+    radix_passc<DIR, 8, LOGN, LG, OFF>(z, tw);
+    radix8_passesc<DIR, LOGN, LG + 3, (LG > 0 ? OFF + 7 * (1 << LG) : OFF)>(z, tw);
+  }
+}
+
+template <int DIR, int LOGN, class T>
+__device__ __forceinline__ void stockham8c(Cplx<T>* z, const Cplx<T>* tw8) {
+  constexpr int first = fft8_first_radix(LOGN);
+  if constexpr (first == 2) {
+    radix_passc<DIR, 2, LOGN, 0, 0>(z, tw8);
+    radix8_passesc<DIR, LOGN, 1, 0>(z, tw8);
+  } else if constexpr (first == 4) {
+    radix_passc<DIR, 4, LOGN, 0, 0>(z, tw8);
+    radix8_passesc<DIR, LOGN, 2, 0>(z, tw8);
+  } else {
+    radix8_passesc<DIR, LOGN, 0, 0>(z, tw8);
+  }
+}
+
+// fft_filter_pair for FP32 with N = 2^LOGN <= 2048 at compile time.
+template <int LOGN>
+__device__ __forceinline__ void fft_filter_pair_c(const float* __restrict__ in, const double* __restrict__ w,
+                                                  const float* __restrict__ hre, const float* __restrict__ him,
+                                                  const Cplx<float>* __restrict__ twg, int va, int vb, int n,
+                                                  Cplx<float>* z, Cplx<float>* tw) {
+  static_assert(LOGN >= 1 && LOGN <= 11, "FP32 compile-time transforms up to 2048 points");
+  constexpr int N = 1 << LOGN;
+  constexpr int IT = N >= 256 ? N / 256 : 1, HT = (N / 2 + 1 + 255) / 256;
+  float xa[IT], xb[IT];
+  double wj[IT];
+#pragma unroll
+  for (int i = 0; i < IT; i++) {
+    const int j = threadIdx.x + i * 256;
+    xa[i] = xb[i] = 0.f;
+    wj[i] = 1.0;
+    if (j < n) {
+      xa[i] = in[static_cast<size_t>(va) * n + j];
+      if (vb >= 0) xb[i] = in[static_cast<size_t>(vb) * n + j];
+      if (w) wj[i] = w[j];
+    }
+  }
+  float mr[HT], mi[HT];
+#pragma unroll
+  for (int i = 0; i < HT; i++) {
+    const int k = threadIdx.x + i * 256;
+    mr[i] = mi[i] = 0.f;
+    if (k <= N / 2) {
+      mr[i] = hre[k];
+      mi[i] = him[k];
+    }
+  }
+  pass_twiddles8(twg, tw, N, LOGN);
+#pragma unroll
+  for (int i = 0; i < IT; i++) {
+    const int j = threadIdx.x + i * 256;
+    if (N >= 256 || j < N) {
+      double da = static_cast<double>(xa[i]), db = static_cast<double>(xb[i]);
+      if (w) {
+        da *= wj[i];
+        db *= wj[i];
+      }
+      z[j] = {static_cast<float>(da), static_cast<float>(db)};
+    }
+  }
+  __syncthreads();
+  stockham8c<-1, LOGN>(z, tw);
+#pragma unroll
+  for (int i = 0; i < HT; i++) {
+    const int k = threadIdx.x + i * 256;
+    if (k > N / 2) break;
+    const int kn = (N - k) & (N - 1);
+    const Cplx<float> zk = z[k], zn = z[kn];
+    const float ar = (zk.x + zn.x) * 0.5f, ai = (zk.y - zn.y) * 0.5f;
+    const float br = (zk.y + zn.y) * 0.5f, bi = (zn.x - zk.x) * 0.5f;
+    const float hr = mr[i], hi = mi[i];
+    const float yar = hr * ar, yai = hi * ai, ybr = hr * br, ybi = hi * bi;
+    const Cplx<float> wk = {yar - ybi, yai + ybr};
+    const Cplx<float> wn = {yar + ybi, ybr - yai};
+    z[k] = wk;
+    if (kn != k) z[kn] = wn;
+  }
+  __syncthreads();
+  stockham8c<+1, LOGN>(z, tw);
+}
+
 }  // namespace dev
 }  // namespace ctsim

@@ -223,6 +223,45 @@ __global__ __launch_bounds__(256) void fft_coef_kernel(
   }
 }
 
+// The FP32 kernels with the transform length at compile time (64 ... 2048
+// points): same results as fft_filter_kernel / fft_coef_kernel.
+template <int LOGN>
+__global__ __launch_bounds__(256) void fft_filter_kernel_c(
+    const float* __restrict__ in, const double* __restrict__ w, const float* __restrict__ hre,
+    const float* __restrict__ him, const Cplx<float>* __restrict__ twg, float* __restrict__ out,
+    int nView, int n, int outStride) {
+  extern __shared__ unsigned char smem_raw[];
+  Cplx<float>* z = reinterpret_cast<Cplx<float>*>(smem_raw);
+  Cplx<float>* tw = z + (1 << LOGN);
+  const int va = 2 * blockIdx.x, vb = va + 1 < nView ? va + 1 : -1;
+  dev::fft_filter_pair_c<LOGN>(in, w, hre, him, twg, va, vb, n, z, tw);
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const Cplx<float> r = z[j];
+    out[static_cast<size_t>(va) * outStride + j] = r.x;
+    if (vb >= 0) out[static_cast<size_t>(vb) * outStride + j] = r.y;
+  }
+}
+
+template <int INTERP, bool LAST_INCL, int LOGN>
+__global__ __launch_bounds__(256) void fft_coef_kernel_c(
+    const float* __restrict__ in, const double* __restrict__ w, const float* __restrict__ hre,
+    const float* __restrict__ him, const Cplx<float>* __restrict__ twg, void* __restrict__ coef,
+    int nView, int n) {
+  extern __shared__ unsigned char smem_raw[];
+  Cplx<float>* z = reinterpret_cast<Cplx<float>*>(smem_raw);
+  Cplx<float>* tw = z + (1 << LOGN);
+  const int va = 2 * blockIdx.x, vb = va + 1 < nView ? va + 1 : -1;
+  dev::fft_filter_pair_c<LOGN>(in, w, hre, him, twg, va, vb, n, z, tw);
+  using CT = dev::CoefT<INTERP>;
+  CT* c = static_cast<CT*>(coef);
+  auto ra = [z](int j) { return z[j].x; };
+  auto rb = [z](int j) { return z[j].y; };
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    c[static_cast<size_t>(va) * n + j] = dev::interval_coef<INTERP, LAST_INCL>(ra, j, n);
+    if (vb >= 0) c[static_cast<size_t>(vb) * n + j] = dev::interval_coef<INTERP, LAST_INCL>(rb, j, n);
+  }
+}
+
 // ------------------------------------------- Toeplitz GEMM on the MFMA units
 // Mixed-precision convolution (procsignal.cpp:659-673 as a matrix product):
 //   Y[v][i] = sum_j X[v][j] * T[j][i],  T[j][i] = dx * h[i - j + n - 1] = g[j - i + n - 1]
@@ -359,6 +398,26 @@ int launch_fft(ctsim_ctx* ctx, const FilterPlan& p, const float* d_in, int nView
   const double* w = p.preweight.empty() ? nullptr : p.d_weight.as<double>();
   const int grid = (nView + 1) / 2;
   const int R = (N / 2 + 255) / 256;
+  if constexpr (sizeof(T) == 4 && sizeof(TO) == 4) {
+    if (logN >= 6 && logN <= 11) {
+#define CTSIM_FFT_C(L)                                                                         \
+  case L:                                                                                      \
+    hipLaunchKernelGGL((fft_filter_kernel_c<L>), dim3(grid), dim3(256), lds, ctx->stream, d_in, w, \
+                       hre, him, tw, reinterpret_cast<float*>(d_out), nView, p.n, outStride);  \
+    break;
+      switch (logN) {
+        CTSIM_FFT_C(6)
+        CTSIM_FFT_C(7)
+        CTSIM_FFT_C(8)
+        CTSIM_FFT_C(9)
+        CTSIM_FFT_C(10)
+        CTSIM_FFT_C(11)
+      }
+#undef CTSIM_FFT_C
+      CTSIM_HIP(hipGetLastError());
+      return CTSIM_OK;
+    }
+  }
 #define CTSIM_FFT_CASE(RR)                                                                     \
   case RR:                                                                                     \
     hipLaunchKernelGGL((fft_filter_kernel<T, TO, RR>), dim3(grid), dim3(256), lds, ctx->stream, \
@@ -481,6 +540,29 @@ int filter_launch_coef(ctsim_ctx* ctx, FilterPlan& p, const float* d_in, int nVi
   const double* w = p.preweight.empty() ? nullptr : p.d_weight.as<double>();
   const int grid = (nView + 1) / 2;
   const int R = (N / 2 + 255) / 256;
+  if (logN >= 6 && logN <= 11) {
+    auto goc = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, ctx->stream, d_in, w, hre, him, tw, d_coef,
+                         nView, p.n);
+    };
+#define CTSIM_COEF_C(L)                                                                       \
+  case L:                                                                                     \
+    if (interp == INTERP_CUBIC) goc(fft_coef_kernel_c<INTERP_CUBIC, false, L>);              \
+    else if (lastIncl) goc(fft_coef_kernel_c<INTERP_LINEAR, true, L>);                       \
+    else goc(fft_coef_kernel_c<INTERP_LINEAR, false, L>);                                     \
+    break;
+    switch (logN) {
+      CTSIM_COEF_C(6)
+      CTSIM_COEF_C(7)
+      CTSIM_COEF_C(8)
+      CTSIM_COEF_C(9)
+      CTSIM_COEF_C(10)
+      CTSIM_COEF_C(11)
+    }
+#undef CTSIM_COEF_C
+    CTSIM_HIP(hipGetLastError());
+    return CTSIM_OK;
+  }
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, ctx->stream, d_in, w, hre, him, tw, d_coef,
                        nView, p.n, N, logN);

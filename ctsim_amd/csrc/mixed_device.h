@@ -763,21 +763,22 @@ __device__ __forceinline__ void bp_sym_tile_pipe(const MixedArgs& a, int tile, i
       }
     }
   }
-  if (ix >= h) return;
-  float* out = a.split == 1 ? a.image : a.partial + static_cast<size_t>(grp) * n * n;
+  if (ix < h) {
+    float* out = a.split == 1 ? a.image : a.partial + static_cast<size_t>(grp) * n * n;
 #pragma unroll
-  for (int k = 0; k < P; k++) {
-    const int iy = iyb + k;
-    if (iy >= h) break;
-    const int c1 = dirf > 0.f ? n - 1 - iy : iy, r1 = dirf > 0.f ? ix : n - 1 - ix;
-    const size_t idx[4] = {static_cast<size_t>(ix) * n + iy, static_cast<size_t>(c1) * n + r1,
-                           static_cast<size_t>(n - 1 - ix) * n + (n - 1 - iy),
-                           static_cast<size_t>(n - 1 - c1) * n + (n - 1 - r1)};
+    for (int k = 0; k < P; k++) {
+      const int iy = iyb + k;
+      if (iy >= h) break;
+      const int c1 = dirf > 0.f ? n - 1 - iy : iy, r1 = dirf > 0.f ? ix : n - 1 - ix;
+      const size_t idx[4] = {static_cast<size_t>(ix) * n + iy, static_cast<size_t>(c1) * n + r1,
+                             static_cast<size_t>(n - 1 - ix) * n + (n - 1 - iy),
+                             static_cast<size_t>(n - 1 - c1) * n + (n - 1 - r1)};
 #pragma unroll
-    for (int m = 0; m < 4; m++) {
-      const float v = sum[m][k] - comp[m][k];
-      if (a.split > 1) out[idx[m]] = v;
-      else out[idx[m]] = a.overwrite ? a.endScale * v : fmaf(a.endScale, v, out[idx[m]]);
+      for (int m = 0; m < 4; m++) {
+        const float v = sum[m][k] - comp[m][k];
+        if (a.split > 1) out[idx[m]] = v;
+        else out[idx[m]] = a.overwrite ? a.endScale * v : fmaf(a.endScale, v, out[idx[m]]);
+      }
     }
   }
 }

@@ -251,9 +251,10 @@ int ctsim_batch_run(ctsim_ctx* ctx, ctsim_batch* b, const float* const* d_sinos,
  * waiting for its slice's filtering (never expected). */
 int ctsim_batch_status(ctsim_ctx* ctx, ctsim_batch* b);
 /* Launch shape: P rows per lane (1, 2, 4), lanesY (8, 16, 32), LDS bytes per
- * view chunk, fused (1) or two-phase (0), sym (1: slices with a square
- * centred image whose views come in 2 or 4 rotated copies run the
- * rotationally symmetric body, when P <= 2). */
+ * view chunk, fused (1) or two-phase (0), sym (0; 1, 2 or 4: slices with a
+ * square centred image whose views come in 2 or 4 rotated copies run the
+ * rotationally symmetric body, when P <= 2, with each quadrant tile's views
+ * in that many groups whose partial images the tile's last group adds). */
 int ctsim_batch_set_shape(ctsim_ctx* ctx, ctsim_batch* b, int P, int lanesY, int chunkBytes,
                           int fused, int sym);
 /* Times candidate shapes on these inputs and keeps the fastest; shape5 gets

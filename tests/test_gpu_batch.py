@@ -69,12 +69,14 @@ def test_batch_matches_single_slice(gpu_ctx, batch_case):
 
 
 # sym: slices whose views come in rotated copies (here all but the 181- and
-# 97-view ones) run the rotationally symmetric body when P <= 2
+# 97-view ones) run the rotationally symmetric body when P <= 2, with each
+# quadrant tile's views in sym groups (1, 2, 4) combined by the last group
 @pytest.mark.parametrize("P,lanesY,chunk,fused,sym",
-                         [(2, 16, 32768, True, True), (4, 8, 16384, True, False),
-                          (2, 32, 16384, False, True), (4, 16, 32768, False, False),
-                          (1, 16, 8192, True, True), (2, 16, 32768, True, False),
-                          (1, 8, 16384, False, True)])
+                         [(2, 16, 32768, True, 1), (4, 8, 16384, True, 0),
+                          (2, 32, 16384, False, 1), (4, 16, 32768, False, 0),
+                          (1, 16, 8192, True, 1), (2, 16, 32768, True, 0),
+                          (1, 8, 16384, False, 1), (2, 8, 16384, True, 4),
+                          (1, 16, 16384, False, 2), (2, 8, 32768, False, 4)])
 def test_batch_shapes(gpu_ctx, batch_case, P, lanesY, chunk, fused, sym):
     import torch
     specs, b, sinos, images, ref64, refmx = batch_case
