@@ -174,7 +174,7 @@ __device__ __forceinline__ void filter_item(const SliceDev& s, int pair, Cplx<fl
 // out and keeps its own register budget.  DO_F / DO_B instantiate the filter
 // and the backprojection items: the fused launch takes both, the two-phase
 // launches one each, so each phase runs at its own register budget.
-template <int P, bool SYM, bool DO_F, bool DO_B>
+template <int P, bool SYM, bool DO_F, bool DO_B, int FL = 0>
 __device__ __forceinline__ void fbp_batch_body(const SliceDev* __restrict__ slices,
                                                const Item* __restrict__ items,
                                                const int* __restrict__ queueStart, int* counters,
@@ -207,7 +207,9 @@ __device__ __forceinline__ void fbp_batch_body(const SliceDev* __restrict__ slic
       if constexpr (DO_F) {
         Cplx<float>* z = reinterpret_cast<Cplx<float>*>(smem);
         Cplx<float>* tw = z + s.N;
-        switch (s.logN) {
+        if constexpr (FL > 0) {
+          filter_item<-FL>(s, it.a, z, tw);  // every slice of this launch has 2^FL points
+        } else switch (s.logN) {
           case 6: filter_item<-6>(s, it.a, z, tw); break;
           case 7: filter_item<-7>(s, it.a, z, tw); break;
           case 8: filter_item<-8>(s, it.a, z, tw); break;
@@ -222,24 +224,31 @@ __device__ __forceinline__ void fbp_batch_body(const SliceDev* __restrict__ slic
               default: filter_item<8>(s, it.a, z, tw); break;
             }
         }
-        __threadfence();  // every lane releases its coefficient stores device-wide
-        __syncthreads();
-        if (threadIdx.x == 0) atomicAdd(&done[it.slice], 1);
+        if constexpr (DO_B) {
+          // fused launch: release the coefficients to the backprojection
+          // items (a device-scope fence writes back the XCD's L2; the
+          // two-phase launches need none, their kernel boundary orders them)
+          __threadfence();  // every lane releases its coefficient stores device-wide
+          __syncthreads();
+          if (threadIdx.x == 0) atomicAdd(&done[it.slice], 1);
+        }
       }
     } else if constexpr (DO_B) {
-      if (threadIdx.x == 0) {
-        int spins = 0;
-        while (__hip_atomic_load(&done[it.slice], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-               s.nFilterItems) {
-          __builtin_amdgcn_s_sleep(2);
-          if (++spins > kSpinLimit) {
-            atomicOr(error, 1);
-            break;
+      if constexpr (DO_F) {  // fused launch: wait for the slice's filter items
+        if (threadIdx.x == 0) {
+          int spins = 0;
+          while (__hip_atomic_load(&done[it.slice], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                 s.nFilterItems) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > kSpinLimit) {
+              atomicOr(error, 1);
+              break;
+            }
           }
+          __threadfence();  // acquire: the coefficients are visible below
         }
-        __threadfence();  // acquire: the coefficients are visible below
+        __syncthreads();
       }
-      __syncthreads();
       if constexpr (SYM && P <= 2) {
         if (s.sym) {
           if (s.interp == INTERP_CUBIC) bp_sym_interp<P, INTERP_CUBIC>(s, it.a, it.pad, smem, s_ab, s_win);
@@ -262,6 +271,16 @@ __global__ __launch_bounds__(256) void fbp_batch_kernel(const SliceDev* __restri
                                                         const int* __restrict__ queueStart,
                                                         int* counters, int* error) {
   fbp_batch_body<P, SYM, DO_F, DO_B>(slices, items, queueStart, counters, error);
+}
+
+// The filter launch of the slices whose transforms have 2^FL points (FL in
+// 6..11), at that length's register budget.
+template <int FL>
+__global__ __launch_bounds__(256) void fbp_batch_filter_kernel(const SliceDev* __restrict__ slices,
+                                                               const Item* __restrict__ items,
+                                                               const int* __restrict__ queueStart,
+                                                               int* counters, int* error) {
+  fbp_batch_body<1, false, true, false, FL>(slices, items, queueStart, counters, error);
 }
 
 template <int P, bool SYM>
@@ -316,6 +335,14 @@ struct ctsim_batch {
   int ldsFilter = 0;
   std::vector<SliceDev> host;
   DevBuf d_slices, d_itemsF, d_itemsB, d_itemsAll, d_qF, d_qB, d_qAll, d_counters, d_error;
+  // two-phase filter launches, one per transform length 2^6 ... 2^11
+  // (specialised kernels) and one for the other lengths (logN 0)
+  struct FilterClass {
+    int logN = 0, qOff = 0, grid = 0;
+    size_t lds = 0;
+  };
+  std::vector<FilterClass> fclass;
+  DevBuf d_itemsFc, d_qFc;
   DevBuf d_partial;      // view-group partial images of the symmetric slices
   DevBuf d_combine;      // slices whose view groups are added after the backprojection
   int nCombine = 0;
@@ -338,6 +365,38 @@ struct ctsim_batch {
 
 namespace ctsim {
 namespace {
+
+// The filter-only kernel of one transform length class.
+const void* filter_kernel_of(int logN) {
+  switch (logN) {
+    case 6: return reinterpret_cast<const void*>(fbp_batch_filter_kernel<6>);
+    case 7: return reinterpret_cast<const void*>(fbp_batch_filter_kernel<7>);
+    case 8: return reinterpret_cast<const void*>(fbp_batch_filter_kernel<8>);
+    case 9: return reinterpret_cast<const void*>(fbp_batch_filter_kernel<9>);
+    case 10: return reinterpret_cast<const void*>(fbp_batch_filter_kernel<10>);
+    case 11: return reinterpret_cast<const void*>(fbp_batch_filter_kernel<11>);
+    default: return reinterpret_cast<const void*>(fbp_batch_kernel<1, false, true, false>);
+  }
+}
+
+void launch_filter_class(const ctsim_batch& b, const ctsim_batch::FilterClass& fc, hipStream_t st) {
+  const SliceDev* sl = b.d_slices.as<SliceDev>();
+  const Item* it = b.d_itemsFc.as<Item>();
+  const int* q = b.d_qFc.as<int>() + fc.qOff;
+  int* c = b.d_counters.as<int>();
+  int* e = b.d_error.as<int>();
+  const dim3 g(fc.grid), blk(256);
+  switch (fc.logN) {
+    case 6: hipLaunchKernelGGL(fbp_batch_filter_kernel<6>, g, blk, fc.lds, st, sl, it, q, c, e); break;
+    case 7: hipLaunchKernelGGL(fbp_batch_filter_kernel<7>, g, blk, fc.lds, st, sl, it, q, c, e); break;
+    case 8: hipLaunchKernelGGL(fbp_batch_filter_kernel<8>, g, blk, fc.lds, st, sl, it, q, c, e); break;
+    case 9: hipLaunchKernelGGL(fbp_batch_filter_kernel<9>, g, blk, fc.lds, st, sl, it, q, c, e); break;
+    case 10: hipLaunchKernelGGL(fbp_batch_filter_kernel<10>, g, blk, fc.lds, st, sl, it, q, c, e); break;
+    case 11: hipLaunchKernelGGL(fbp_batch_filter_kernel<11>, g, blk, fc.lds, st, sl, it, q, c, e); break;
+    default:
+      hipLaunchKernelGGL((fbp_batch_kernel<1, false, true, false>), g, blk, fc.lds, st, sl, it, q, c, e);
+  }
+}
 
 // The symmetric instantiation runs when some slice takes the symmetric body.
 bool batch_sym_kernel(const ctsim_batch& b) {
@@ -441,7 +500,7 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
       a.tilesY = (a.half + ty - 1) / ty;
       a.capW = 4 * sv.S * Wv;
       // view groups of at least 8 base views; each group's partial image
-      // goes to the batch's partial buffer, the tile's last group adds them
+      // goes to the batch's partial buffer, added by batch_combine_kernel
       int G = std::max(1, std::min(b.sym, a.Q / 8));
       a.viewsPerGroup = (a.Q + G - 1) / G;
       G = (a.Q + a.viewsPerGroup - 1) / a.viewsPerGroup;
@@ -495,6 +554,33 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
       load[x] += b.s[k].cost;
     }
   }
+  // the two-phase filter launches by transform length class
+  std::vector<Item> itemsFc;
+  std::vector<int> qFc;
+  b.fclass.clear();
+  auto fclassOf = [&](int k) { return b.host[k].logN >= 6 && b.host[k].logN <= 11 ? b.host[k].logN : 0; };
+  for (int cls : {6, 7, 8, 9, 10, 11, 0}) {
+    ctsim_batch::FilterClass fc;
+    fc.logN = cls;
+    fc.qOff = static_cast<int>(qFc.size());
+    int maxN = 0;
+    const size_t first = itemsFc.size();
+    for (int x = 0; x < 8; x++) {
+      qFc.push_back(static_cast<int>(itemsFc.size()));
+      for (const Item& it : f[x])
+        if (fclassOf(it.slice) == cls) {
+          itemsFc.push_back(it);
+          maxN = std::max(maxN, b.host[it.slice].N);
+        }
+    }
+    qFc.push_back(static_cast<int>(itemsFc.size()));
+    if (itemsFc.size() == first) {
+      qFc.resize(fc.qOff);
+      continue;
+    }
+    fc.lds = static_cast<size_t>(maxN + dev::fft_tw_entries<float>(maxN)) * sizeof(Cplx<float>);
+    b.fclass.push_back(fc);
+  }
   std::vector<Item> itemsF, itemsB, itemsAll;
   std::vector<int> qF(9), qB(9), qAll(9);
   for (int x = 0; x < 8; x++) {
@@ -516,6 +602,8 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
   };
   CTSIM_CHECK(up(b.d_slices, b.host.data(), b.host.size() * sizeof(SliceDev)));
   CTSIM_CHECK(up(b.d_itemsF, itemsF.data(), itemsF.size() * sizeof(Item)));
+  CTSIM_CHECK(up(b.d_itemsFc, itemsFc.data(), itemsFc.size() * sizeof(Item)));
+  CTSIM_CHECK(up(b.d_qFc, qFc.data(), qFc.size() * sizeof(int)));
   CTSIM_CHECK(up(b.d_itemsB, itemsB.data(), itemsB.size() * sizeof(Item)));
   CTSIM_CHECK(up(b.d_itemsAll, itemsAll.data(), itemsAll.size() * sizeof(Item)));
   CTSIM_CHECK(up(b.d_qF, qF.data(), qF.size() * sizeof(int)));
@@ -541,6 +629,7 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
                         fbp_batch_bp_kernel<2, false>);
   const void* kF = reinterpret_cast<const void*>(fbp_batch_kernel<1, false, true, false>);
   b.gridF = occupancy_grid(kF, b.ldsFilter, ctx->numCUs);
+  for (auto& fc : b.fclass) fc.grid = occupancy_grid(filter_kernel_of(fc.logN), fc.lds, ctx->numCUs);
   b.gridB = occupancy_grid(kB, b.ldsB, ctx->numCUs);
   b.gridAll = occupancy_grid(kAll, b.ldsAll, ctx->numCUs);
   b.shapeDirty = false;
@@ -560,9 +649,12 @@ void launch_batch(ctsim_batch& b, hipStream_t st) {
                        b.d_slices.as<SliceDev>(), b.d_itemsAll.as<Item>(), b.d_qAll.as<int>(),
                        b.d_counters.as<int>(), b.d_error.as<int>());
   } else {
-    hipLaunchKernelGGL((fbp_batch_kernel<1, false, true, false>), dim3(b.gridF), dim3(256), b.ldsFilter, st,
-                       b.d_slices.as<SliceDev>(), b.d_itemsF.as<Item>(), b.d_qF.as<int>(),
-                       b.d_counters.as<int>(), b.d_error.as<int>());
+    // one filter launch per transform length class (queue heads reset in
+    // between; the per-slice done counts carry over), then the backprojection
+    for (size_t c = 0; c < b.fclass.size(); c++) {
+      if (c > 0) hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8);
+      launch_filter_class(b, b.fclass[c], st);
+    }
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8);
     hipLaunchKernelGGL((fbp_batch_bp_kernel<P, SYM>), dim3(b.gridB), dim3(256), b.ldsB, st,
                        b.d_slices.as<SliceDev>(), b.d_itemsB.as<Item>(), b.d_qB.as<int>(),
