@@ -33,6 +33,7 @@ METRIC = "FBP slices/s at 512² fan-beam: mixed vs FP64 GPU path at d ≤1e-4 vs
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak
 PEAK_FP64_TFLOPS = 78.6    # MI355X published FP64 vector peak
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
+PEAK_BF16_TFLOPS = 2500.0   # MI355X published dense BF16 MFMA peak
 # Algorithmic FP32 flops per pixel-view update of the mixed equiangular cubic
 # backprojector with Kahan accumulation: cubic interpolation 28 + equiangular
 # geometry 16 (3 special: divide, atan, reciprocal) + Kahan 3.
@@ -318,6 +319,9 @@ def main():
                      "traffic": traffic if shape[5] == 2 else None,
                      "kernel": KERNEL_BY_SYM[shape[5]], "kernel_ms": bp_ms,
                      "flops_per_pvu": FLOPS_PER_PVU_EQUIANGULAR_CUBIC_KAHAN,
+                     "flops_counting": "algorithmic: one position, weight and cubic interpolation "
+                                       "per update; the symmetric kernels evaluate a position "
+                                       "once per 4 updates and issue fewer instructions",
                      "pvu_per_launch": pvu,
                      "hbm_algorithmic_bytes": hbm_bytes,
                      "hbm_frac": hbm_bytes / (bp_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
@@ -406,7 +410,9 @@ def bench_config3(ctx, args):
                               "convolution, IntDiff linear, 2048x2048"},
                  fp64_gpu={"slices_per_s": 1 / t64, "ms_per_slice": t64 * 1e3},
                  toeplitz_filter={"ms": t_toe * 1e3, "macs": macs,
-                                  "bf16x3_tflops": 6 * macs / t_toe / 1e12},
+                                  "bf16_products_per_mac": 6,
+                                  "mfma_tflops": 12 * macs / t_toe / 1e12,
+                                  "mfma_frac": 12 * macs / t_toe / 1e12 / PEAK_BF16_TFLOPS},
                  launch_shape=list(shape),
                  fft_convolution_batch={"ms_per_slice": t_fft * 1e3, "d": d_f[0]},
                  accuracy_vs_fp64={"d": d_t[0], "r": d_t[1], "e": d_t[2]})
