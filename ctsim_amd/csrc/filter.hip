@@ -271,24 +271,56 @@ __global__ __launch_bounds__(256) void fft_coef_kernel_c(
 // hi*lo + lo*hi + mid*mid, with FP32 accumulation: six
 // v_mfma_f32_32x32x16_bf16 per tile and k-step, a product accurate to about
 // 2^-24 (a two-part split leaves 2^-17 per product, which the ramp filter's
-// cancellation amplifies past 1e-4 in the image).  A workgroup (4 waves)
-// computes a 64-view x 128-output tile; wave (wm, wn) owns rows wm*32.. and
-// two 32-column MFMA tiles at wn*64.  The Toeplitz operand is never
-// materialised: for a k-chunk the window of g it needs is staged in LDS as 8
-// copies shifted by one element, so the 8 consecutive values a lane needs
-// always start at an 8-element (16-byte) boundary of one copy.
+// cancellation amplifies past 1e-4 in the image).
+//
+// A workgroup (4 waves) computes a 64-view x 128-output tile; wave (wm, wn)
+// owns rows wm*32.. and two 32-column MFMA tiles at wn*64.  The Toeplitz
+// operand is never materialised: lane l of a B fragment needs the 8
+// consecutive values g[q..q+8) with q = k0 - col + n - 1, so the plan keeps a
+// record table rec[part][q + kRecOff] = g_part[q..q+8) (16 bytes, zero outside
+// g) and every fragment is one 16-byte load that the L1/L2 serve.  Tile 1
+// (32 columns further) of k-chunk j0 needs exactly tile 0's records of chunk
+// j0 - 32, so each chunk loads records for tile 0 only and tile 1 reuses the
+// previous chunk's.  Projection values and records of the next chunk are
+// loaded while the current chunk's MFMAs run; there is no LDS and no barrier.
+// Workgroups are numbered so that each XCD runs a contiguous range of row
+// blocks: the projection rows are read through one XCD's L2.
 #pragma clang fp contract(fast)
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx16_t __attribute__((ext_vector_type(16)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
 
 constexpr int kTM = 64, kTN = 128, kTK = 32;
-constexpr int kWin = kTN + kTK - 1;           // window of g per k-chunk
-constexpr int kWinPad = ((kWin + 7) / 8) * 8 + 8;
+constexpr int kRecOff = 192;  // record of g index q sits at q + kRecOff
 
-__device__ __forceinline__ unsigned short bf16_bits(float x) {
-  __bf16 b = static_cast<__bf16>(x);
-  return __builtin_bit_cast(unsigned short, b);
+int toeplitz_records(int n) { return 2 * n + 256; }
+
+struct ToeChunk {
+  floatx4_t x[2][2];   // [k-step][half]: X[row][j0 + 16 s + 8 kg + 4 h + 0..3]
+  floatx4_t w[2][2];   // the same positions of the preweight (PW only)
+  u16x8_t b[2][3];     // [k-step][part]: tile-0 records
+};
+
+template <bool PW>
+__device__ __forceinline__ void toe_load(ToeChunk& c, __amdgpu_buffer_rsrc_t rsA,
+                                         __amdgpu_buffer_rsrc_t rsW, __amdgpu_buffer_rsrc_t rsB,
+                                         int aOff, int wOff, int bIdx, int j0, int nrec) {
+#pragma unroll
+  for (int st = 0; st < 2; st++) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      c.x[st][h] = __builtin_bit_cast(
+          floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(rsA, (aOff + j0 + 16 * st + 4 * h) * 4, 0, 0));
+      if constexpr (PW)
+        c.w[st][h] = __builtin_bit_cast(
+            floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(rsW, (wOff + j0 + 16 * st + 4 * h) * 4, 0, 0));
+    }
+#pragma unroll
+    for (int part = 0; part < 3; part++)
+      c.b[st][part] = __builtin_bit_cast(
+          u16x8_t, __builtin_amdgcn_raw_buffer_load_b128(rsB, (part * nrec + bIdx + j0 + 16 * st) * 16, 0, 0));
+  }
 }
 
 __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
@@ -298,71 +330,86 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
   l = static_cast<__bf16>(r1 - static_cast<float>(m));
 }
 
+__device__ __forceinline__ void toe_products(floatx16_t& acc, const bf16x8_t& ah, const bf16x8_t& am,
+                                             const bf16x8_t& al, const u16x8_t* b) {
+  const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, b[0]);
+  const bf16x8_t bm = __builtin_bit_cast(bf16x8_t, b[1]);
+  const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, b[2]);
+  // smallest terms first
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bm, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bm, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+}
+
+// grid: 8 * ceil(tiles / 8) workgroups of 256 lanes, tiles = ceil(n / 128) *
+// ceil(nView / 64); rec: 3 * nrec records (hi, mid, lo), nrec = toeplitz_records(n);
+// PW: the inputs are multiplied by the preweight w32 first.
+template <bool PW>
 __global__ __launch_bounds__(256) void toeplitz_mfma_kernel(
-    const float* __restrict__ in, const float* __restrict__ w32,
-    const unsigned short* __restrict__ gh, const unsigned short* __restrict__ gm,
-    const unsigned short* __restrict__ gl, float* __restrict__ out, int nView, int n,
-    int outStride) {
-  __shared__ __attribute__((aligned(16))) unsigned short s_gh[8][kWinPad];
-  __shared__ __attribute__((aligned(16))) unsigned short s_gm[8][kWinPad];
-  __shared__ __attribute__((aligned(16))) unsigned short s_gl[8][kWinPad];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const float* __restrict__ in, const float* __restrict__ w32, const u16x8_t* __restrict__ rec,
+    int nrec, float* __restrict__ out, int nView, int n, int outStride) {
+  const int ncb = (n + kTN - 1) / kTN;
+  const int tiles = ncb * ((nView + kTM - 1) / kTM);
+  const int perXcd = (tiles + 7) / 8;
+  const int t = (blockIdx.x & 7) * perXcd + (blockIdx.x >> 3);
+  if (t >= tiles) return;
+  const int m0 = (t / ncb) * kTM, n0 = (t % ncb) * kTN;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
-  const int row = m0 + wm * 32 + (lane & 31);
   const int kg = lane >> 5;
+  const int rows = min(kTM, nView - m0);
+  // rows past the last view read as zero (buffer range); values past the
+  // row end (k >= n) are cleared in the last chunk
+  const __amdgpu_buffer_rsrc_t rsA = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(in + static_cast<size_t>(m0) * n), 0, rows * n * 4, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<u16x8_t*>(rec), 0, 3 * nrec * 16, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rsW = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(w32), 0, PW ? n * 4 : 0, 0x00020000);
+  const int wOff = 8 * kg;
+  const int aOff = (wm * 32 + (lane & 31)) * n + 8 * kg;
+  const int bIdx = 8 * kg - (n0 + wn * 64 + (lane & 31)) + n - 1 + kRecOff;
   floatx16_t acc0 = {}, acc1 = {};
-  const int nq = 2 * n - 1;
-  for (int j0 = 0; j0 < n; j0 += kTK) {
-    const int qb = j0 - n0 - (kTN - 1) + n - 1;  // g index of window position 0
-    __syncthreads();
-    for (int idx = tid; idx < 8 * kWinPad; idx += 256) {
-      const int c = idx / kWinPad, m = idx % kWinPad;
-      const int q = qb + c + m;
-      const bool ok = q >= 0 && q < nq;
-      s_gh[c][m] = ok ? gh[q] : 0;
-      s_gm[c][m] = ok ? gm[q] : 0;
-      s_gl[c][m] = ok ? gl[q] : 0;
-    }
-    __syncthreads();
+  ToeChunk cur, nxt;
+  u16x8_t prev[2][3];  // tile-0 records of the previous chunk = tile 1's of this one
+  toe_load<PW>(nxt, rsA, rsW, rsB, aOff, wOff, bIdx, -kTK, nrec);  // only its records are used
 #pragma unroll
-    for (int s = 0; s < kTK / 16; s++) {
-      // A fragment: X[row][j0 + 16 s + 8 kg + t], t = 0..7
-      const int k0 = j0 + 16 * s + 8 * kg;
+  for (int st = 0; st < 2; st++)
+#pragma unroll
+    for (int part = 0; part < 3; part++) prev[st][part] = nxt.b[st][part];
+  toe_load<PW>(cur, rsA, rsW, rsB, aOff, wOff, bIdx, 0, nrec);
+  const int nc = (n + kTK - 1) / kTK;
+  for (int c = 0; c < nc; c++) {
+    const int j0 = c * kTK;
+    toe_load<PW>(nxt, rsA, rsW, rsB, aOff, wOff, bIdx, j0 + kTK, nrec);  // past the end: zeros, unused
+    __builtin_amdgcn_sched_barrier(0);  // keep the next chunk's loads ahead of this chunk's MFMAs
+#pragma unroll
+    for (int st = 0; st < 2; st++) {
+      const int k0 = j0 + 16 * st + 8 * kg;
       bf16x8_t ah, am, al;
 #pragma unroll
-      for (int t = 0; t < 8; t++) {
-        float x = 0.f;
-        const int k = k0 + t;
-        if (row < nView && k < n) {
-          x = in[static_cast<size_t>(row) * n + k];
-          if (w32) x *= w32[k];
-        }
+      for (int e = 0; e < 8; e++) {
+        float x = cur.x[st][e >> 2][e & 3];
+        if constexpr (PW) x *= cur.w[st][e >> 2][e & 3];
+        if (j0 + kTK > n && k0 + e >= n) x = 0.f;
         __bf16 h, m, l;
         split3(x, h, m, l);
-        ah[t] = h;
-        am[t] = m;
-        al[t] = l;
+        ah[e] = h;
+        am[e] = m;
+        al[e] = l;
       }
-#pragma unroll
-      for (int tile = 0; tile < 2; tile++) {
-        // B fragment: g[j - i + n - 1] for j = k0 + t, i = column of this lane
-        const int cbase = wn * 64 + tile * 32;
-        const int p0 = (kTN - 1) + 16 * s + 8 * kg - cbase - (lane & 31);
-        const int c = p0 & 7, mpos = p0 - c;
-        const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u16x8_t*>(&s_gh[c][mpos]));
-        const bf16x8_t bm = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u16x8_t*>(&s_gm[c][mpos]));
-        const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u16x8_t*>(&s_gl[c][mpos]));
-        floatx16_t& acc = tile == 0 ? acc0 : acc1;
-        // smallest terms first
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bm, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bm, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
-      }
+      toe_products(acc0, ah, am, al, cur.b[st]);
+      toe_products(acc1, ah, am, al, prev[st]);
     }
+    __builtin_amdgcn_sched_barrier(0);  // rotate only after every MFMA is issued
+#pragma unroll
+    for (int st = 0; st < 2; st++)
+#pragma unroll
+      for (int part = 0; part < 3; part++) prev[st][part] = cur.b[st][part];
+    cur = nxt;
   }
   // D layout of the 32x32 tile: register r of lane l is row 8(r/4) + 4(l/32) + r%4,
   // column l%32.
@@ -452,8 +499,9 @@ int filter_upload(FilterPlan& p) {
   }
   if (p.method == METHOD_CONVOLUTION) {
     // Toeplitz operand of the mixed convolution: g[q] = dx * h[2n-2-q],
-    // split into bf16 hi, mid, lo from its double value
-    const int nq = 2 * p.n - 1;
+    // split into bf16 hi, mid, lo from its double value, stored as records of
+    // 8 consecutive values: record i of a part holds g_part[i - kRecOff + 0..7]
+    const int nq = 2 * p.n - 1, nrec = toeplitz_records(p.n);
     std::vector<unsigned short> g16(3 * static_cast<size_t>(nq));
     for (int q = 0; q < nq; q++) {
       const double g = p.inc * p.filter[2 * p.n - 2 - q];
@@ -465,8 +513,16 @@ int filter_upload(FilterPlan& p) {
       g16[nq + q] = __builtin_bit_cast(unsigned short, mid);
       g16[2 * static_cast<size_t>(nq) + q] = __builtin_bit_cast(unsigned short, lo);
     }
-    CTSIM_CHECK(p.d_toeplitz.reserve(g16.size() * sizeof(unsigned short)));
-    CTSIM_HIP(hipMemcpy(p.d_toeplitz.ptr, g16.data(), g16.size() * sizeof(unsigned short),
+    std::vector<unsigned short> recs(3 * static_cast<size_t>(nrec) * 8, 0);
+    for (int part = 0; part < 3; part++)
+      for (int i = 0; i < nrec; i++)
+        for (int e = 0; e < 8; e++) {
+          const int q = i - kRecOff + e;
+          if (q >= 0 && q < nq)
+            recs[(static_cast<size_t>(part) * nrec + i) * 8 + e] = g16[static_cast<size_t>(part) * nq + q];
+        }
+    CTSIM_CHECK(p.d_toeplitz.reserve(recs.size() * sizeof(unsigned short)));
+    CTSIM_HIP(hipMemcpy(p.d_toeplitz.ptr, recs.data(), recs.size() * sizeof(unsigned short),
                         hipMemcpyHostToDevice));
   }
   if (!p.hre.empty()) {
@@ -592,11 +648,13 @@ int filter_launch(ctsim_ctx* ctx, FilterPlan& p, int precision, const float* d_i
   const double* w = p.preweight.empty() ? nullptr : p.d_weight.as<double>();
   const bool f64 = precision == CTSIM_PREC_F64;
   if (p.method == METHOD_CONVOLUTION && !f64) {
-    const int nq = 2 * p.n - 1;
-    const unsigned short* g = p.d_toeplitz.as<unsigned short>();
+    const int tiles = ((p.n + kTN - 1) / kTN) * ((nView + kTM - 1) / kTM);
+    if (static_cast<long long>(kTM) * p.n * 4 > INT32_MAX)
+      return fail(CTSIM_ERR_UNSUPPORTED, "mixed convolution: detector count too large");
     const float* w32 = p.preweight.empty() ? nullptr : p.d_weight32.as<float>();
-    hipLaunchKernelGGL(toeplitz_mfma_kernel, dim3((p.n + kTN - 1) / kTN, (nView + kTM - 1) / kTM),
-                       dim3(256), 0, s, d_in, w32, g, g + nq, g + 2 * static_cast<size_t>(nq),
+    auto kern = w32 ? toeplitz_mfma_kernel<true> : toeplitz_mfma_kernel<false>;
+    hipLaunchKernelGGL(kern, dim3(8 * ((tiles + 7) / 8)), dim3(256), 0, s, d_in, w32,
+                       p.d_toeplitz.as<u16x8_t>(), toeplitz_records(p.n),
                        static_cast<float*>(d_out), nView, p.n, outStride);
     CTSIM_HIP(hipGetLastError());
     return CTSIM_OK;
