@@ -202,12 +202,18 @@ def main():
     # --- plain FP64 GPU path
     fbp(ctsim_amd.PREC_F64, f64, img64)
     torch.cuda.synchronize()
-    ctx.profile(True)
     t0 = time.perf_counter()
     for _ in range(args.fp64_steps):
         fbp(ctsim_amd.PREC_F64, f64, img64)
     torch.cuda.synchronize()
     t_f64 = (time.perf_counter() - t0) / args.fp64_steps
+    # kernel duration from HIP events recorded around the backprojection
+    # kernel on its stream, in one more step (event records are barrier
+    # packets that add a few microseconds between kernels, so they stay out
+    # of the timed steps)
+    ctx.profile(True)
+    fbp(ctsim_amd.PREC_F64, f64, img64)
+    torch.cuda.synchronize()
     k64_ms, k64_n = ctx.profile_read()
     ctx.profile(False)
     log(f"[bench] FP64 path: {t_f64 * 1e3:.2f} ms/slice (backprojection kernel {k64_ms / k64_n:.3f} ms)")
@@ -244,16 +250,24 @@ def main():
     for _ in range(args.warmup):
         bp.fbp(ps, sino, ang, img32, ctsim_amd.PREC_MIXED)
     torch.cuda.synchronize()
-    ctx.profile(True)
     t0 = time.perf_counter()
     for k in range(args.steps):
         bp.fbp(ps, sino, ang, img32, ctsim_amd.PREC_MIXED)
     torch.cuda.synchronize()
     t_mixed = (time.perf_counter() - t0) / args.steps
+    # the same K steps again with HIP events around every backprojection
+    # kernel (on the library's stream) for the kernel's mean duration
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        bp.fbp(ps, sino, ang, img32, ctsim_amd.PREC_MIXED)
+    torch.cuda.synchronize()
+    t_instr = (time.perf_counter() - t0) / args.steps
     kms, kn = ctx.profile_read()
     ctx.profile(False)
     bp_ms = kms / kn
-    log(f"[bench] mixed path: {t_mixed * 1e3:.3f} ms/slice (backprojection kernel {bp_ms:.3f} ms)")
+    log(f"[bench] mixed path: {t_mixed * 1e3:.3f} ms/slice (instrumented replay {t_instr * 1e3:.3f} "
+        f"ms/slice, backprojection kernel {bp_ms:.3f} ms)")
 
     d, r, e = ctsim_amd.image_stats(ctx, img64, img32)
     log(f"[bench] mixed vs FP64: d={d:.3e} r={r:.3e} e={e:.3e}")
@@ -318,6 +332,9 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic if shape[5] == 2 else None,
                      "kernel": KERNEL_BY_SYM[shape[5]], "kernel_ms": bp_ms,
+                     "kernel_timing": "HIP events around each backprojection launch on the "
+                                      "library's stream, over a replay of the K timed steps "
+                                      f"({t_instr * 1e3:.3f} ms/step with the events)",
                      "flops_per_pvu": FLOPS_PER_PVU_EQUIANGULAR_CUBIC_KAHAN,
                      "flops_counting": "algorithmic: one position, weight and cubic interpolation "
                                        "per update; the symmetric kernels evaluate a position "

@@ -83,8 +83,12 @@ int ctsim_ctx::activate() const {
 void ctsim_ctx::prof_begin_on(hipStream_t s) {
   if (!profOn) return;
   if (profEvents.size() < 2 * (profUsed + 1)) {
+    // timing-only events: without the system-scope fence a record is a
+    // timestamp, not a cache writeback and invalidate between the kernels
     hipEvent_t a = nullptr, b = nullptr;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    if (hipEventCreateWithFlags(&a, hipEventDisableSystemFence) != hipSuccess ||
+        hipEventCreateWithFlags(&b, hipEventDisableSystemFence) != hipSuccess)
+      return;
     profEvents.push_back(a);
     profEvents.push_back(b);
   }

@@ -257,8 +257,9 @@ __global__ __launch_bounds__(256) void fft_coef_kernel_c(
   auto ra = [z](int j) { return z[j].x; };
   auto rb = [z](int j) { return z[j].y; };
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    c[static_cast<size_t>(va) * n + j] = dev::interval_coef<INTERP, LAST_INCL>(ra, j, n);
-    if (vb >= 0) c[static_cast<size_t>(vb) * n + j] = dev::interval_coef<INTERP, LAST_INCL>(rb, j, n);
+    dev::store_stream(&c[static_cast<size_t>(va) * n + j], dev::interval_coef<INTERP, LAST_INCL>(ra, j, n));
+    if (vb >= 0)
+      dev::store_stream(&c[static_cast<size_t>(vb) * n + j], dev::interval_coef<INTERP, LAST_INCL>(rb, j, n));
   }
 }
 

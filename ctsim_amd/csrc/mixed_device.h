@@ -38,6 +38,21 @@ __device__ __forceinline__ int xcd_tile(int bid, int n) {
 template <int INTERP>
 using CoefT = typename std::conditional<INTERP == INTERP_CUBIC, float4, float2>::type;
 
+// Store with the non-temporal hint (streaming: the line is not kept for reuse
+// in this XCD's L2), for results only a later kernel reads.
+template <class T>
+__device__ __forceinline__ void store_stream(T* p, const T& v) {
+  if constexpr (sizeof(T) == 16) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    __builtin_nontemporal_store(__builtin_bit_cast(f4, v), reinterpret_cast<f4*>(p));
+  } else if constexpr (sizeof(T) == 8) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(__builtin_bit_cast(f2, v), reinterpret_cast<f2*>(p));
+  } else {
+    __builtin_nontemporal_store(v, p);
+  }
+}
+
 // Coefficients of detector interval i of one filtered row (n cells), read
 // through row(j) (any index the rules below use is in [0, n)).
 template <int INTERP, bool LAST_INCL, class Row>
@@ -776,7 +791,7 @@ __device__ __forceinline__ void bp_sym_tile_pipe(const MixedArgs& a, int tile, i
 #pragma unroll
       for (int m = 0; m < 4; m++) {
         const float v = sum[m][k] - comp[m][k];
-        if (a.split > 1) out[idx[m]] = v;
+        if (a.split > 1) store_stream(&out[idx[m]], v);
         else out[idx[m]] = a.overwrite ? a.endScale * v : fmaf(a.endScale, v, out[idx[m]]);
       }
     }
