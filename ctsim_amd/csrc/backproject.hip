@@ -667,7 +667,7 @@ bool bp_sym_ok(const BpPlan& p, const MixedShape& m, int nView) {
       p.cachedKind != 1)
     return false;
   const size_t elem = p.interp == INTERP_CUBIC ? sizeof(float4) : sizeof(float2);
-  if (m.sym == 2) return m.W <= 128;  // the pipelined body stages 4 S W cells, two per lane per row
+  if (m.sym == 2) return m.W <= dev::kPipeRow;  // the pipelined body stages rows of kPipeRow cells, two per lane
   return static_cast<size_t>(4 * sv.S) * m.W * elem <= sym_lds(m);
 }
 
@@ -762,7 +762,7 @@ int bp_mixed_tiles(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const v
     a.split = m.split < 1 ? 1 : (m.split > a.Q ? a.Q : m.split);
     a.viewsPerGroup = (a.Q + a.split - 1) / a.split;
     a.split = (a.Q + a.viewsPerGroup - 1) / a.viewsPerGroup;
-    a.capW = m.sym == 2 ? 4 * sv.S * a.W : static_cast<int>(sym_lds(m) / elem);
+    a.capW = m.sym == 2 ? 4 * sv.S * dev::kPipeRow : static_cast<int>(sym_lds(m) / elem);
   }
   const size_t lds = sym ? static_cast<size_t>(a.capW) * elem : static_cast<size_t>(a.chunk) * a.W * elem;
   if (lds > kMixedLdsMax) return fail(CTSIM_ERR_UNSUPPORTED, "backprojector: detector row too long for LDS");

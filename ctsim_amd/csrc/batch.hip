@@ -461,10 +461,11 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
     if (static_cast<size_t>(chunk) * Wv * elem > 62 * 1024)
       return fail(CTSIM_ERR_UNSUPPORTED, "batch: a slice's detector window does not fit in LDS");
     // the (pipelined) symmetric body when the views come in rotated copies
-    // and the windows hold at most 128 cells; it stages 4 S W cells
+    // and the windows hold at most kPipeRow cells; it stages 4 S rows of
+    // kPipeRow cells
     const SymViews sv = bp_sym_views(sl.angles.data(), p.nView);
-    const size_t symLds = static_cast<size_t>(4 * sv.S) * Wv * elem;
-    d.sym = b.sym > 0 && b.P <= 2 && p.symImage && sv.S > 0 && m.W > 0 && m.W <= 128;
+    const size_t symLds = static_cast<size_t>(4 * sv.S) * dev::kPipeRow * elem;
+    d.sym = b.sym > 0 && b.P <= 2 && p.symImage && sv.S > 0 && m.W > 0 && m.W <= dev::kPipeRow;
     d.symS = sv.S;
     d.symK = sv.K;
     maxBpLds = std::max(maxBpLds, d.sym ? symLds : static_cast<size_t>(chunk) * Wv * elem);
@@ -498,7 +499,7 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
       a.dirf = sv.dir;
       a.tilesX = (a.half + tx - 1) / tx;
       a.tilesY = (a.half + ty - 1) / ty;
-      a.capW = 4 * sv.S * Wv;
+      a.capW = 4 * sv.S * dev::kPipeRow;
       // view groups of at least 8 base views; each group's partial image
       // goes to the batch's partial buffer, added by batch_combine_kernel
       int G = std::max(1, std::min(b.sym, a.Q / 8));

@@ -14,6 +14,8 @@ namespace ctsim {
 namespace dev {
 
 constexpr int kMaxChunk = 64;
+// LDS row stride (cells) of the pipelined symmetric body: its largest window.
+constexpr int kPipeRow = 128;
 
 // Remap block ids so that neighbouring tiles run on the same XCD.  The
 // dispatcher deals workgroups round-robin over the 8 XCDs, so XCD k receives
@@ -642,15 +644,17 @@ __device__ __forceinline__ void member_window(const MixedArgs& a, int m, float c
 // view v + 1's windows are in flight into registers; they go to LDS after a
 // barrier.  Wave w stages member w's S rows (at most 128 cells each, two per
 // lane), so each LDS slot (member m, row s) sits at a fixed offset
-// (m S + s) W and needs no layout pass; windows and view constants are
-// computed for 64 base views at a time, one (view, member) per lane.  The
-// LDS holds 4 S W coefficients (capW >= 4 S W, W <= 128).
+// (m S + s) kPipeRow and needs no layout pass; windows and view constants
+// are computed for 64 base views at a time, one (view, member) per lane.
+// The row stride is the compile-time maximum window (128 cells), so every
+// LDS read of an update is one address plus an immediate offset; the LDS
+// holds 4 S kPipeRow coefficients (capW = 4 S kPipeRow, W <= kPipeRow).
 template <int GK, int INTERP, int P, int DEG, int S, int K>
 __device__ __forceinline__ void bp_sym_tile_pipe(const MixedArgs& a, int tile, int grp,
                                                  CoefT<INTERP>* s_w, float2* s_ab, SymWin* s_win) {
   using CT = CoefT<INTERP>;
   const int t = threadIdx.x;
-  const int h = a.half, n = 2 * a.half, W = a.W;
+  const int h = a.half, n = 2 * a.half;
   const int tx = 256 / a.lanesY, ty = a.lanesY * P;
   const int ix0 = (tile / a.tilesY) * tx, iy0 = (tile % a.tilesY) * ty;
   const int ix = ix0 + t / a.lanesY, iyb = iy0 + (t % a.lanesY) * P;
@@ -674,7 +678,7 @@ __device__ __forceinline__ void bp_sym_tile_pipe(const MixedArgs& a, int tile, i
   int pending = 0;
   const float dirf = a.dirf;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
-  CT* slot = s_w + wave * S * W;  // this wave's member rows
+  CT* slot = s_w + wave * S * kPipeRow;  // this wave's member rows
 
   // global loads of member `wave`'s S rows of base view v0 + vv
   auto fetch = [&](int vv, int v0, CT (&r)[S][2]) {
@@ -690,8 +694,8 @@ __device__ __forceinline__ void bp_sym_tile_pipe(const MixedArgs& a, int tile, i
     const int size = __builtin_amdgcn_readfirstlane(s_win[vv * 4 + wave].size);
 #pragma unroll
     for (int s = 0; s < S; s++) {
-      if (lane < size) slot[s * W + lane] = r[s][0];
-      if (lane + 64 < size) slot[s * W + lane + 64] = r[s][1];
+      if (lane < size) slot[s * kPipeRow + lane] = r[s][0];
+      if (lane + 64 < size) slot[s * kPipeRow + lane + 64] = r[s][1];
     }
   };
 
@@ -739,13 +743,13 @@ __device__ __forceinline__ void bp_sym_tile_pipe(const MixedArgs& a, int tile, i
           else if (m == 2) pos_member<GK, DEG, 2>(a, Am, Bm, r2mF2[k], w[m].cshift, pos, wg[m]);
           else pos_member<GK, DEG, 3>(a, Am, Bm, r2mF2[k], w[m].cshift, pos, wg[m]);
           u[m] = __builtin_amdgcn_fractf(pos);
-          j[m] = m * S * W + static_cast<int>(__builtin_amdgcn_fmed3f(pos, 0.f, w[m].sizeM1));
+          j[m] = m * S * kPipeRow + static_cast<int>(__builtin_amdgcn_fmed3f(pos, 0.f, w[m].sizeM1));
         }
 #pragma unroll
         for (int s = 0; s < S; s++) {
           CT q[4];
 #pragma unroll
-          for (int m = 0; m < 4; m++) q[m] = s_w[j[m] + s * W];
+          for (int m = 0; m < 4; m++) q[m] = s_w[j[m] + s * kPipeRow];
 #pragma unroll
           for (int m = 0; m < 4; m++) {
             float val;
