@@ -147,8 +147,9 @@ __device__ __forceinline__ void filter_item(const SliceDev& s, int pair, Cplx<fl
     dev::fft_filter_pair<float, R>(s.sino, s.w, s.hre, s.him, s.tw, va, vb, s.n, s.N, s.logN, z, tw);
   // interval coefficients of both rows straight from LDS
   const int n = s.n;
-  auto ra = [z](int j) { return z[j].x; };
-  auto rb = [z](int j) { return z[j].y; };
+  // the compile-time transforms leave cell j in slot fft_slot(j)
+  auto ra = [z](int j) { return z[R < 0 ? dev::fft_slot(j) : j].x; };
+  auto rb = [z](int j) { return z[R < 0 ? dev::fft_slot(j) : j].y; };
   if (s.interp == INTERP_CUBIC) {
     float4* c = static_cast<float4*>(const_cast<void*>(s.bp.coef));
     for (int j = threadIdx.x; j < n; j += blockDim.x) {

@@ -391,6 +391,14 @@ __device__ __forceinline__ void fft_filter_pair(const float* __restrict__ in, co
 // (N = 2^LOGN <= 2048, 256 lanes): the pass sequence unrolls, LDS offsets
 // become immediates, and no lane bound or direction select is evaluated.
 // Same passes, tables and arithmetic as stockham8.
+//
+// Element i of these transforms lives in LDS slot fft_slot(i): the low four
+// index bits XORed with bits 4-7.  The butterfly outputs of the early passes
+// go to indices R apart (R = 2, 4, 8), which in the plain layout fall on a
+// few of the banks; the XOR spreads each run of 16 such writes over all of
+// them.  Reads at t + r N/R keep their immediate offsets (N/R is a multiple
+// of 256 from 512 points up).  Callers read the result through fft_slot.
+__host__ __device__ constexpr int fft_slot(int i) { return i ^ ((i >> 4) & 15); }
 
 template <int DIR, class T>
 __device__ __forceinline__ void dft4c(Cplx<T>& c0, Cplx<T>& c1, Cplx<T>& c2, Cplx<T>& c3) {
@@ -446,7 +454,7 @@ __device__ __forceinline__ void radix_passc(Cplx<T>* z, const Cplx<T>* tw) {
     if (nb >= 256 || t < nb) {
       const int k = t & (Ls - 1);
 #pragma unroll
-      for (int r = 0; r < R; r++) y[b][r] = z[t + r * nb];
+      for (int r = 0; r < R; r++) y[b][r] = z[fft_slot(t + r * nb)];
       if constexpr (Ls > 1) {
 #pragma unroll
         for (int r = 1; r < R; r++) {
@@ -464,7 +472,7 @@ __device__ __forceinline__ void radix_passc(Cplx<T>* z, const Cplx<T>* tw) {
   for (int b = 0; b < NB; b++) {
     if (nb >= 256 || idx[b] >= 0) {
 #pragma unroll
-      for (int q = 0; q < R; q++) z[idx[b] + q * Ls] = y[b][q];
+      for (int q = 0; q < R; q++) z[fft_slot(idx[b] + q * Ls)] = y[b][q];
     }
   }
   __syncthreads();
@@ -493,7 +501,8 @@ __device__ __forceinline__ void stockham8c(Cplx<T>* z, const Cplx<T>* tw8) {
   }
 }
 
-// fft_filter_pair for FP32 with N = 2^LOGN <= 2048 at compile time.
+// fft_filter_pair for FP32 with N = 2^LOGN <= 2048 at compile time; filtered
+// cell j of the rows is z[fft_slot(j)].x (va) and .y (vb).
 template <int LOGN>
 __device__ __forceinline__ void fft_filter_pair_c(const float* __restrict__ in, const double* __restrict__ w,
                                                   const float* __restrict__ hre, const float* __restrict__ him,
@@ -535,7 +544,7 @@ __device__ __forceinline__ void fft_filter_pair_c(const float* __restrict__ in, 
         da *= wj[i];
         db *= wj[i];
       }
-      z[j] = {static_cast<float>(da), static_cast<float>(db)};
+      z[fft_slot(j)] = {static_cast<float>(da), static_cast<float>(db)};
     }
   }
   __syncthreads();
@@ -545,15 +554,15 @@ __device__ __forceinline__ void fft_filter_pair_c(const float* __restrict__ in, 
     const int k = threadIdx.x + i * 256;
     if (k > N / 2) break;
     const int kn = (N - k) & (N - 1);
-    const Cplx<float> zk = z[k], zn = z[kn];
+    const Cplx<float> zk = z[fft_slot(k)], zn = z[fft_slot(kn)];
     const float ar = (zk.x + zn.x) * 0.5f, ai = (zk.y - zn.y) * 0.5f;
     const float br = (zk.y + zn.y) * 0.5f, bi = (zn.x - zk.x) * 0.5f;
     const float hr = mr[i], hi = mi[i];
     const float yar = hr * ar, yai = hi * ai, ybr = hr * br, ybi = hi * bi;
     const Cplx<float> wk = {yar - ybi, yai + ybr};
     const Cplx<float> wn = {yar + ybi, ybr - yai};
-    z[k] = wk;
-    if (kn != k) z[kn] = wn;
+    z[fft_slot(k)] = wk;
+    if (kn != k) z[fft_slot(kn)] = wn;
   }
   __syncthreads();
   stockham8c<+1, LOGN>(z, tw);

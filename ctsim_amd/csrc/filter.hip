@@ -236,7 +236,7 @@ __global__ __launch_bounds__(256) void fft_filter_kernel_c(
   const int va = 2 * blockIdx.x, vb = va + 1 < nView ? va + 1 : -1;
   dev::fft_filter_pair_c<LOGN>(in, w, hre, him, twg, va, vb, n, z, tw);
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    const Cplx<float> r = z[j];
+    const Cplx<float> r = z[dev::fft_slot(j)];
     out[static_cast<size_t>(va) * outStride + j] = r.x;
     if (vb >= 0) out[static_cast<size_t>(vb) * outStride + j] = r.y;
   }
@@ -254,8 +254,8 @@ __global__ __launch_bounds__(256) void fft_coef_kernel_c(
   dev::fft_filter_pair_c<LOGN>(in, w, hre, him, twg, va, vb, n, z, tw);
   using CT = dev::CoefT<INTERP>;
   CT* c = static_cast<CT*>(coef);
-  auto ra = [z](int j) { return z[j].x; };
-  auto rb = [z](int j) { return z[j].y; };
+  auto ra = [z](int j) { return z[dev::fft_slot(j)].x; };
+  auto rb = [z](int j) { return z[dev::fft_slot(j)].y; };
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
     dev::store_stream(&c[static_cast<size_t>(va) * n + j], dev::interval_coef<INTERP, LAST_INCL>(ra, j, n));
     if (vb >= 0)
