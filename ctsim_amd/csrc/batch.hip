@@ -292,6 +292,63 @@ __global__ __launch_bounds__(256) void fbp_batch_bp_kernel(const SliceDev* __res
   fbp_batch_body<P, SYM, false, true>(slices, items, queueStart, counters, error);
 }
 
+// The two-phase backprojection launch of the symmetric slices of one class
+// (geometry, interpolation, fan polynomial degree, symmetry): one body at its
+// own register budget, the single-slice kernel's, where the kernel that
+// serves every class holds the largest body's.
+template <int P, int GK, int INTERP, int DEG, int S, int K>
+__global__ __launch_bounds__(256) void fbp_batch_sym_class_kernel(const SliceDev* __restrict__ slices,
+                                                                  const Item* __restrict__ items,
+                                                                  const int* __restrict__ queueStart,
+                                                                  int* counters) {
+  extern __shared__ float4 smem[];
+  __shared__ dev::SymWin s_win[256];
+  __shared__ float2 s_ab[64];
+  __shared__ int s_item;
+  const int q = blockIdx.x & 7;
+  const int qBegin = queueStart[q], qEnd = queueStart[q + 1];
+  while (true) {
+    if (threadIdx.x == 0) s_item = qBegin + atomicAdd(&counters[q], 1);
+    __syncthreads();
+    const int i = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (i >= qEnd) break;  // every workgroup ends when its queue is drained
+    const Item it = items[i];
+    const SliceDev& s = slices[it.slice];
+    dev::bp_sym_tile_pipe<GK, INTERP, P, DEG, S, K>(s.bp, it.a, it.pad,
+                                                    reinterpret_cast<CoefT<INTERP>*>(smem), s_ab, s_win);
+    __syncthreads();  // LDS is reused by the next item
+  }
+}
+
+template <int P, int GK, int INTERP, int DEG>
+const void* sym_class_kernel_sk(int S, int K) {
+  if (S == 4) return reinterpret_cast<const void*>(fbp_batch_sym_class_kernel<P, GK, INTERP, DEG, 4, 1>);
+  if (K == 1) return reinterpret_cast<const void*>(fbp_batch_sym_class_kernel<P, GK, INTERP, DEG, 2, 1>);
+  return reinterpret_cast<const void*>(fbp_batch_sym_class_kernel<P, GK, INTERP, DEG, 2, 2>);
+}
+
+template <int P, int INTERP>
+const void* sym_class_kernel_g(int gk, int deg, int S, int K) {
+  if (gk == 0) return sym_class_kernel_sk<P, 0, INTERP, 0>(S, K);
+  if (gk == 2) return sym_class_kernel_sk<P, 2, INTERP, 0>(S, K);
+  switch (deg) {
+    case 4: return sym_class_kernel_sk<P, 1, INTERP, 4>(S, K);
+    case 6: return sym_class_kernel_sk<P, 1, INTERP, 6>(S, K);
+    case 8: return sym_class_kernel_sk<P, 1, INTERP, 8>(S, K);
+    default: return sym_class_kernel_sk<P, 1, INTERP, 0>(S, K);
+  }
+}
+
+// The class kernel of symmetric slices (P 1 or 2).
+const void* sym_class_kernel(int P, int gk, int interp, int deg, int S, int K) {
+  if (P == 1)
+    return interp == INTERP_CUBIC ? sym_class_kernel_g<1, INTERP_CUBIC>(gk, deg, S, K)
+                                  : sym_class_kernel_g<1, INTERP_LINEAR>(gk, deg, S, K);
+  return interp == INTERP_CUBIC ? sym_class_kernel_g<2, INTERP_CUBIC>(gk, deg, S, K)
+                                : sym_class_kernel_g<2, INTERP_LINEAR>(gk, deg, S, K);
+}
+
 // ------------------------------------------------------------------ host
 
 void fft_host(std::vector<std::complex<double>>& a) {
@@ -344,6 +401,16 @@ struct ctsim_batch {
   };
   std::vector<FilterClass> fclass;
   DevBuf d_itemsFc, d_qFc;
+  // two-phase backprojection: the plain-body and non-symmetric items in one
+  // launch (d_itemsB), the symmetric slices' items in one launch per class
+  struct BpClass {
+    const void* kernel = nullptr;
+    int qOff = 0, grid = 0;
+    size_t lds = 0;
+  };
+  std::vector<BpClass> bclass;
+  DevBuf d_itemsBc, d_qBc;
+  int nPlainB = 0;  // items of the d_itemsB launch
   DevBuf d_partial;      // view-group partial images of the symmetric slices
   DevBuf d_combine;      // slices whose view groups are added after the backprojection
   int nCombine = 0;
@@ -583,6 +650,44 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
     fc.lds = static_cast<size_t>(maxN + dev::fft_tw_entries<float>(maxN)) * sizeof(Cplx<float>);
     b.fclass.push_back(fc);
   }
+  // the two-phase backprojection: symmetric slices by class (geometry,
+  // interpolation, polynomial degree, symmetry), each class's items in the
+  // per-XCD order above; the other items in the plain launch
+  auto bclassKey = [&](int k) {
+    const SliceDev& d = b.host[k];
+    return d.sym ? 1 + d.gk + 3 * (d.interp == INTERP_CUBIC ? 1 : 0) + 6 * d.deg + 60 * d.symS + 300 * d.symK
+                 : 0;
+  };
+  std::vector<int> keys;
+  for (int k = 0; k < S; k++)
+    if (bclassKey(k) && std::find(keys.begin(), keys.end(), bclassKey(k)) == keys.end())
+      keys.push_back(bclassKey(k));
+  std::vector<Item> itemsBc;
+  std::vector<int> qBc;
+  b.bclass.clear();
+  for (int key : keys) {
+    ctsim_batch::BpClass bc;
+    bc.qOff = static_cast<int>(qBc.size());
+    int rep = -1;
+    for (int x = 0; x < 8; x++) {
+      qBc.push_back(static_cast<int>(itemsBc.size()));
+      for (const Item& it : bpq[x])
+        if (bclassKey(it.slice) == key) {
+          itemsBc.push_back(it);
+          rep = it.slice;
+        }
+    }
+    qBc.push_back(static_cast<int>(itemsBc.size()));
+    const SliceDev& d = b.host[rep];
+    bc.kernel = sym_class_kernel(b.P, d.gk, d.interp, d.deg, d.symS, d.symK);
+    bc.lds = static_cast<size_t>(4 * d.symS) * dev::kPipeRow * (d.interp == INTERP_CUBIC ? 16 : 8);
+    b.bclass.push_back(bc);
+  }
+  size_t plainLds = 0;
+  for (int k = 0; k < S; k++)
+    if (!b.host[k].sym)
+      plainLds = std::max(plainLds, static_cast<size_t>(b.host[k].bp.chunk) * b.host[k].bp.W *
+                                        (b.host[k].interp == INTERP_CUBIC ? 16 : 8));
   std::vector<Item> itemsF, itemsB, itemsAll;
   std::vector<int> qF(9), qB(9), qAll(9);
   for (int x = 0; x < 8; x++) {
@@ -590,16 +695,18 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
     qB[x] = static_cast<int>(itemsB.size());
     qAll[x] = static_cast<int>(itemsAll.size());
     itemsF.insert(itemsF.end(), f[x].begin(), f[x].end());
-    itemsB.insert(itemsB.end(), bpq[x].begin(), bpq[x].end());
+    for (const Item& it : bpq[x])
+      if (!b.host[it.slice].sym) itemsB.push_back(it);
     itemsAll.insert(itemsAll.end(), f[x].begin(), f[x].end());
     itemsAll.insert(itemsAll.end(), bpq[x].begin(), bpq[x].end());
   }
   qF[8] = static_cast<int>(itemsF.size());
   qB[8] = static_cast<int>(itemsB.size());
   qAll[8] = static_cast<int>(itemsAll.size());
+  b.nPlainB = static_cast<int>(itemsB.size());
   auto up = [](DevBuf& d, const void* src, size_t n) -> int {
-    CTSIM_CHECK(d.reserve(n));
-    CTSIM_HIP(hipMemcpy(d.ptr, src, n, hipMemcpyHostToDevice));
+    CTSIM_CHECK(d.reserve(n > 0 ? n : 16));
+    if (n > 0) CTSIM_HIP(hipMemcpy(d.ptr, src, n, hipMemcpyHostToDevice));
     return CTSIM_OK;
   };
   CTSIM_CHECK(up(b.d_slices, b.host.data(), b.host.size() * sizeof(SliceDev)));
@@ -607,6 +714,10 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
   CTSIM_CHECK(up(b.d_itemsFc, itemsFc.data(), itemsFc.size() * sizeof(Item)));
   CTSIM_CHECK(up(b.d_qFc, qFc.data(), qFc.size() * sizeof(int)));
   CTSIM_CHECK(up(b.d_itemsB, itemsB.data(), itemsB.size() * sizeof(Item)));
+  if (!b.bclass.empty()) {
+    CTSIM_CHECK(up(b.d_itemsBc, itemsBc.data(), itemsBc.size() * sizeof(Item)));
+    CTSIM_CHECK(up(b.d_qBc, qBc.data(), qBc.size() * sizeof(int)));
+  }
   CTSIM_CHECK(up(b.d_itemsAll, itemsAll.data(), itemsAll.size() * sizeof(Item)));
   CTSIM_CHECK(up(b.d_qF, qF.data(), qF.size() * sizeof(int)));
   CTSIM_CHECK(up(b.d_qB, qB.data(), qB.size() * sizeof(int)));
@@ -615,7 +726,7 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
   if (!combine.empty()) CTSIM_CHECK(up(b.d_combine, combine.data(), combine.size() * sizeof(int)));
   CTSIM_CHECK(b.d_error.reserve(sizeof(int)));
   CTSIM_HIP(hipMemset(b.d_error.ptr, 0, sizeof(int)));
-  b.ldsB = maxBpLds;
+  b.ldsB = plainLds;
   b.ldsAll = std::max(maxBpLds, static_cast<size_t>(b.ldsFilter));
   const bool symKernel = batch_sym_kernel(b);
   auto pick = [&](auto k4, auto k1s, auto k1, auto k2s, auto k2) -> const void* {
@@ -626,13 +737,14 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
   const void* kAll = pick(fbp_batch_kernel<4, false, true, true>, fbp_batch_kernel<1, true, true, true>,
                           fbp_batch_kernel<1, false, true, true>, fbp_batch_kernel<2, true, true, true>,
                           fbp_batch_kernel<2, false, true, true>);
-  const void* kB = pick(fbp_batch_bp_kernel<4, false>, fbp_batch_bp_kernel<1, true>,
-                        fbp_batch_bp_kernel<1, false>, fbp_batch_bp_kernel<2, true>,
-                        fbp_batch_bp_kernel<2, false>);
+  const void* kB = b.P == 4   ? reinterpret_cast<const void*>(fbp_batch_bp_kernel<4, false>)
+                   : b.P == 1 ? reinterpret_cast<const void*>(fbp_batch_bp_kernel<1, false>)
+                              : reinterpret_cast<const void*>(fbp_batch_bp_kernel<2, false>);
   const void* kF = reinterpret_cast<const void*>(fbp_batch_kernel<1, false, true, false>);
   b.gridF = occupancy_grid(kF, b.ldsFilter, ctx->numCUs);
   for (auto& fc : b.fclass) fc.grid = occupancy_grid(filter_kernel_of(fc.logN), fc.lds, ctx->numCUs);
   b.gridB = occupancy_grid(kB, b.ldsB, ctx->numCUs);
+  for (auto& bc : b.bclass) bc.grid = occupancy_grid(bc.kernel, bc.lds, ctx->numCUs);
   b.gridAll = occupancy_grid(kAll, b.ldsAll, ctx->numCUs);
   b.shapeDirty = false;
   if (b.graph) {
@@ -657,10 +769,21 @@ void launch_batch(ctsim_batch& b, hipStream_t st) {
       if (c > 0) hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8);
       launch_filter_class(b, b.fclass[c], st);
     }
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8);
-    hipLaunchKernelGGL((fbp_batch_bp_kernel<P, SYM>), dim3(b.gridB), dim3(256), b.ldsB, st,
-                       b.d_slices.as<SliceDev>(), b.d_itemsB.as<Item>(), b.d_qB.as<int>(),
-                       b.d_counters.as<int>(), b.d_error.as<int>());
+    if (b.nPlainB > 0) {
+      hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8);
+      hipLaunchKernelGGL((fbp_batch_bp_kernel<P, false>), dim3(b.gridB), dim3(256), b.ldsB, st,
+                         b.d_slices.as<SliceDev>(), b.d_itemsB.as<Item>(), b.d_qB.as<int>(),
+                         b.d_counters.as<int>(), b.d_error.as<int>());
+    }
+    for (const auto& bc : b.bclass) {
+      hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8);
+      const SliceDev* sl = b.d_slices.as<SliceDev>();
+      const Item* it = b.d_itemsBc.as<Item>();
+      const int* q = b.d_qBc.as<int>() + bc.qOff;
+      int* c = b.d_counters.as<int>();
+      void* args[] = {&sl, &it, &q, &c};
+      (void)hipLaunchKernel(bc.kernel, dim3(bc.grid), dim3(256), args, bc.lds, st);
+    }
   }
   if (b.nCombine > 0)
     hipLaunchKernelGGL(batch_combine_kernel, dim3(256, b.nCombine), dim3(256), 0, st,

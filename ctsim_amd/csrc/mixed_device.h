@@ -107,6 +107,16 @@ struct MixedArgs {
   float dirf;
 };
 
+// A pointer known to be the same in every lane of the wave, moved to scalar
+// registers (the compiler cannot prove it for pointers loaded from memory).
+template <class T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  const unsigned long long u = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(u));
+  const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(u >> 32));
+  return reinterpret_cast<T*>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
 // Loads of one row of interval coefficients through a buffer resource: the
 // hardware range check returns zeros for cells outside [0, n) (a negative
 // cell's byte offset wraps to a large unsigned value), which is the zero
@@ -680,12 +690,18 @@ __device__ __forceinline__ void bp_sym_tile_pipe(const MixedArgs& a, int tile, i
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
   CT* slot = s_w + wave * S * kPipeRow;  // this wave's member rows
 
-  // global loads of member `wave`'s S rows of base view v0 + vv
+  // global loads of member `wave`'s S rows of base view v0 + vv; the row
+  // addresses are wave-uniform, and said so, so that the buffer descriptors
+  // stay in scalar registers when the arguments come from memory (a batch
+  // slice table) rather than from the kernel arguments
+  const CT* coefU = uniform_ptr(static_cast<const CT*>(a.coef));
+  const int nDetU = __builtin_amdgcn_readfirstlane(a.nDet), QU = __builtin_amdgcn_readfirstlane(a.Q);
   auto fetch = [&](int vv, int v0, CT (&r)[S][2]) {
     const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(a.center - s_win[vv * 4 + wave].cshift));
 #pragma unroll
     for (int s = 0; s < S; s++) {
-      const RowLoad<CT> row(a.coef, static_cast<size_t>(v0 + vv + s * a.Q) * a.nDet, a.nDet);
+      const int rowIdx = __builtin_amdgcn_readfirstlane(v0 + vv + s * QU);
+      const RowLoad<CT> row(coefU, static_cast<size_t>(rowIdx) * nDetU, nDetU);
       r[s][0] = row(lo + lane);
       r[s][1] = row(lo + lane + 64);
     }
