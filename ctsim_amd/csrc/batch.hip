@@ -62,6 +62,7 @@ struct SliceDev {
   // backprojection
   int gk, interp, lastIncl, deg;
   int sym, symS, symK;  // rotationally symmetric body (tiles of the quadrant)
+  int P;                // pixels per lane of this slice's tiles
   MixedArgs bp;  // bp.coef, bp.image, bp.vc set; overwrite = 1, split = 1
 };
 
@@ -515,12 +516,24 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
     d.interp = p.interp;
     d.lastIncl = p.kind == KIND_IDIFF;
     d.deg = d.gk == 1 ? p.atanDeg : 0;
-    // backprojection shape (P is the batch kernel's template parameter)
+    // backprojection shape (P is the batch kernel's template parameter).
+    // The (pipelined) symmetric body runs when the views come in rotated
+    // copies and the windows hold at most kPipeRow cells (it stages 4 S rows
+    // of kPipeRow cells) at P <= 2; in two-phase mode the symmetric classes
+    // have launches of their own, so with P = 4 their slices take P = 2.
+    const SymViews sv = bp_sym_views(sl.angles.data(), p.nView);
+    const bool symViews = b.sym > 0 && p.symImage && sv.S > 0;
     MixedShape m;
     m.lanesY = b.lanesY;
-    m.P = b.P;
     m.split = 1;
+    m.P = symViews && b.P == 4 && !b.fused ? 2 : b.P;
     m.W = bp_window_bound(p, 256 / m.lanesY, m.lanesY * m.P);
+    d.sym = symViews && m.P <= 2 && m.W > 0 && m.W <= dev::kPipeRow;
+    if (!d.sym && m.P != b.P) {  // the plain body at the batch's P after all
+      m.P = b.P;
+      m.W = bp_window_bound(p, 256 / m.lanesY, m.lanesY * m.P);
+    }
+    d.P = m.P;
     const int Wv = m.W > 0 ? m.W : p.nDet + 4;
     const size_t elem = p.interp == INTERP_CUBIC ? 16 : 8;
     int chunk = static_cast<int>(b.chunkBudget / (Wv * elem));
@@ -528,12 +541,7 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
     m.chunk = chunk;
     if (static_cast<size_t>(chunk) * Wv * elem > 62 * 1024)
       return fail(CTSIM_ERR_UNSUPPORTED, "batch: a slice's detector window does not fit in LDS");
-    // the (pipelined) symmetric body when the views come in rotated copies
-    // and the windows hold at most kPipeRow cells; it stages 4 S rows of
-    // kPipeRow cells
-    const SymViews sv = bp_sym_views(sl.angles.data(), p.nView);
     const size_t symLds = static_cast<size_t>(4 * sv.S) * dev::kPipeRow * elem;
-    d.sym = b.sym > 0 && b.P <= 2 && p.symImage && sv.S > 0 && m.W > 0 && m.W <= dev::kPipeRow;
     d.symS = sv.S;
     d.symK = sv.K;
     maxBpLds = std::max(maxBpLds, d.sym ? symLds : static_cast<size_t>(chunk) * Wv * elem);
@@ -655,7 +663,8 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
   // per-XCD order above; the other items in the plain launch
   auto bclassKey = [&](int k) {
     const SliceDev& d = b.host[k];
-    return d.sym ? 1 + d.gk + 3 * (d.interp == INTERP_CUBIC ? 1 : 0) + 6 * d.deg + 60 * d.symS + 300 * d.symK
+    return d.sym ? 1 + d.gk + 3 * (d.interp == INTERP_CUBIC ? 1 : 0) + 6 * d.deg + 60 * d.symS +
+                       300 * d.symK + 900 * d.P
                  : 0;
   };
   std::vector<int> keys;
@@ -679,7 +688,7 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
     }
     qBc.push_back(static_cast<int>(itemsBc.size()));
     const SliceDev& d = b.host[rep];
-    bc.kernel = sym_class_kernel(b.P, d.gk, d.interp, d.deg, d.symS, d.symK);
+    bc.kernel = sym_class_kernel(d.P, d.gk, d.interp, d.deg, d.symS, d.symK);
     bc.lds = static_cast<size_t>(4 * d.symS) * dev::kPipeRow * (d.interp == INTERP_CUBIC ? 16 : 8);
     b.bclass.push_back(bc);
   }
@@ -986,14 +995,15 @@ int ctsim_batch_autotune(ctsim_ctx* ctx, ctsim_batch* b, const float* const* d_s
   float best = 1e30f;
   int bP = b->P, bL = b->lanesY, bC = b->chunkBudget, bF = b->fused, bS = b->sym;
   int rc = CTSIM_OK;
-  // P = 1 only with the symmetric body, P = 4 only without (it has none)
+  // P = 1 only with the symmetric body; P = 4 with it only in two-phase
+  // mode (its symmetric slices then take P = 2 in launches of their own)
   for (int P : {1, 2, 4})
     for (int sym : {1, 4, 0})
     for (int lanesY : {8, 16, 32})
       for (int chunk : {16 * 1024, 32 * 1024})
         for (int fused : {1, 0}) {
           if (rc != CTSIM_OK) break;
-          if ((P == 1 && !sym) || (P == 4 && sym)) continue;
+          if ((P == 1 && !sym) || (P == 4 && sym && fused)) continue;
           if ((rc = ctsim_batch_set_shape(ctx, b, P, lanesY, chunk, fused, sym)) != CTSIM_OK) {
             if (rc == CTSIM_ERR_UNSUPPORTED) { rc = CTSIM_OK; continue; }
             break;

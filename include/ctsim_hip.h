@@ -253,8 +253,10 @@ int ctsim_batch_status(ctsim_ctx* ctx, ctsim_batch* b);
 /* Launch shape: P rows per lane (1, 2, 4), lanesY (8, 16, 32), LDS bytes per
  * view chunk, fused (1) or two-phase (0), sym (0; 1, 2 or 4: slices with a
  * square centred image whose views come in 2 or 4 rotated copies run the
- * rotationally symmetric body, when P <= 2, with each quadrant tile's views
- * in that many groups whose partial images the tile's last group adds). */
+ * rotationally symmetric body, when P <= 2 or in two-phase mode, where they
+ * take P = 2 beside a P = 4 launch of the others, with each quadrant tile's
+ * views in that many groups whose partial images are added after the
+ * backprojection). */
 int ctsim_batch_set_shape(ctsim_ctx* ctx, ctsim_batch* b, int P, int lanesY, int chunkBytes,
                           int fused, int sym);
 /* Times candidate shapes on these inputs and keeps the fastest; shape5 gets

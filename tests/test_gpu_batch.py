@@ -69,14 +69,16 @@ def test_batch_matches_single_slice(gpu_ctx, batch_case):
 
 
 # sym: slices whose views come in rotated copies (here all but the 181- and
-# 97-view ones) run the rotationally symmetric body when P <= 2, with each
-# quadrant tile's views in sym groups (1, 2, 4) combined by the last group
+# 97-view ones) run the rotationally symmetric body when P <= 2 (in two-phase
+# mode with P = 4 those slices take P = 2 in their own launches), with each
+# quadrant tile's views in sym groups (1, 2, 4) added after the launch
 @pytest.mark.parametrize("P,lanesY,chunk,fused,sym",
                          [(2, 16, 32768, True, 1), (4, 8, 16384, True, 0),
                           (2, 32, 16384, False, 1), (4, 16, 32768, False, 0),
                           (1, 16, 8192, True, 1), (2, 16, 32768, True, 0),
                           (1, 8, 16384, False, 1), (2, 8, 16384, True, 4),
-                          (1, 16, 16384, False, 2), (2, 8, 32768, False, 4)])
+                          (1, 16, 16384, False, 2), (2, 8, 32768, False, 4),
+                          (4, 8, 16384, False, 2)])
 def test_batch_shapes(gpu_ctx, batch_case, P, lanesY, chunk, fused, sym):
     import torch
     specs, b, sinos, images, ref64, refmx = batch_case
