@@ -218,6 +218,7 @@ __device__ __forceinline__ void fbp_batch_body(const SliceDev* __restrict__ slic
           case 9: filter_item<-9>(s, it.a, z, tw); break;
           case 10: filter_item<-10>(s, it.a, z, tw); break;
           case 11: filter_item<-11>(s, it.a, z, tw); break;
+          case 12: filter_item<-12>(s, it.a, z, tw); break;
           default:
             switch (s.R) {
               case 1: filter_item<1>(s, it.a, z, tw); break;
@@ -444,6 +445,7 @@ const void* filter_kernel_of(int logN) {
     case 9: return reinterpret_cast<const void*>(fbp_batch_filter_kernel<9>);
     case 10: return reinterpret_cast<const void*>(fbp_batch_filter_kernel<10>);
     case 11: return reinterpret_cast<const void*>(fbp_batch_filter_kernel<11>);
+    case 12: return reinterpret_cast<const void*>(fbp_batch_filter_kernel<12>);
     default: return reinterpret_cast<const void*>(fbp_batch_kernel<1, false, true, false>);
   }
 }
@@ -462,6 +464,7 @@ void launch_filter_class(const ctsim_batch& b, const ctsim_batch::FilterClass& f
     case 9: hipLaunchKernelGGL(fbp_batch_filter_kernel<9>, g, blk, fc.lds, st, sl, it, q, c, e); break;
     case 10: hipLaunchKernelGGL(fbp_batch_filter_kernel<10>, g, blk, fc.lds, st, sl, it, q, c, e); break;
     case 11: hipLaunchKernelGGL(fbp_batch_filter_kernel<11>, g, blk, fc.lds, st, sl, it, q, c, e); break;
+    case 12: hipLaunchKernelGGL(fbp_batch_filter_kernel<12>, g, blk, fc.lds, st, sl, it, q, c, e); break;
     default:
       hipLaunchKernelGGL((fbp_batch_kernel<1, false, true, false>), g, blk, fc.lds, st, sl, it, q, c, e);
   }
@@ -635,8 +638,8 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
   std::vector<Item> itemsFc;
   std::vector<int> qFc;
   b.fclass.clear();
-  auto fclassOf = [&](int k) { return b.host[k].logN >= 6 && b.host[k].logN <= 11 ? b.host[k].logN : 0; };
-  for (int cls : {6, 7, 8, 9, 10, 11, 0}) {
+  auto fclassOf = [&](int k) { return dev::fft_c_length(b.host[k].logN) ? b.host[k].logN : 0; };
+  for (int cls : {6, 7, 8, 9, 10, 11, 12, 0}) {
     ctsim_batch::FilterClass fc;
     fc.logN = cls;
     fc.qOff = static_cast<int>(qFc.size());
@@ -655,7 +658,8 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
       qFc.resize(fc.qOff);
       continue;
     }
-    fc.lds = static_cast<size_t>(maxN + dev::fft_tw_entries<float>(maxN)) * sizeof(Cplx<float>);
+    fc.lds = cls ? dev::fft_c_lds(maxN)
+                 : static_cast<size_t>(maxN + dev::fft_tw_entries<float>(maxN)) * sizeof(Cplx<float>);
     b.fclass.push_back(fc);
   }
   // the two-phase backprojection: symmetric slices by class (geometry,
@@ -916,7 +920,13 @@ int ctsim_batch_create(ctsim_ctx* ctx, ctsim_batch** out, const ctsim_slice_desc
     b->pvu += pvu;
   }
   if (rc == CTSIM_OK) {
-    b->ldsFilter = static_cast<int>((maxN + dev::fft_tw_entries<float>(maxN)) * sizeof(Cplx<float>));
+    // the fused launch's filter items: compile-time lengths hold N twiddles
+    size_t ldsF = 0;
+    for (const auto& sl : b->s)
+      ldsF = std::max(ldsF, dev::fft_c_length(sl.logN)
+                                ? dev::fft_c_lds(sl.N)
+                                : static_cast<size_t>(sl.N + dev::fft_tw_entries<float>(sl.N)) * sizeof(Cplx<float>));
+    b->ldsFilter = static_cast<int>(ldsF);
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->evIn, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->evOut, hipEventDisableTiming) != hipSuccess)

@@ -156,20 +156,21 @@ __host__ __device__ constexpr int fft8_first_radix(int logN) {
   return logN % 3 == 1 ? 2 : (logN % 3 == 2 ? 4 : 8);
 }
 
-template <class T>
+template <class T, int PER_LANE = 8>
 __device__ __forceinline__ void pass_twiddles8(const Cplx<T>* __restrict__ twg, Cplx<T>* tw8, int N,
                                                int logN) {
   // the radix-8 passes with twiddles start at Ls = the first radix (a
   // radix-8 first pass has Ls = 1 and needs none, the next has Ls = 8); at
-  // most N - 4 <= 2044 entries, so 8 per lane, all gathered before the stores
+  // most N - 4 entries, PER_LANE per lane (8 up to 2048 points, 16 for
+  // 4096), all gathered before the stores
   const int half = N >> 1;
   const int l0 = fft8_first_radix(logN) == 2 ? 1 : (fft8_first_radix(logN) == 4 ? 2 : 3);
   int total = 0;
   for (int lg = l0; lg < logN; lg += 3) total += 7 << lg;
-  Cplx<T> v[8];
-  int dst[8];
+  Cplx<T> v[PER_LANE];
+  int dst[PER_LANE];
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
+  for (int i = 0; i < PER_LANE; i++) {
     const int e = threadIdx.x + i * blockDim.x;
     dst[i] = -1;
     if (e < total) {
@@ -184,7 +185,7 @@ __device__ __forceinline__ void pass_twiddles8(const Cplx<T>* __restrict__ twg, 
     }
   }
 #pragma unroll
-  for (int i = 0; i < 8; i++)
+  for (int i = 0; i < PER_LANE; i++)
     if (dst[i] >= 0) tw8[dst[i]] = v[i];
 }
 
@@ -400,6 +401,11 @@ __device__ __forceinline__ void fft_filter_pair(const float* __restrict__ in, co
 // of 256 from 512 points up).  Callers read the result through fft_slot.
 __host__ __device__ constexpr int fft_slot(int i) { return i ^ ((i >> 4) & 15); }
 
+// Lengths 2^6 ... 2^12 take these transforms; their LDS holds the N points and
+// the N-entry per-pass twiddle table (64 KB at 4096 points).
+__host__ __device__ constexpr bool fft_c_length(int logN) { return logN >= 6 && logN <= 12; }
+__host__ __device__ constexpr size_t fft_c_lds(int N) { return 2 * static_cast<size_t>(N) * sizeof(Cplx<float>); }
+
 template <int DIR, class T>
 __device__ __forceinline__ void dft4c(Cplx<T>& c0, Cplx<T>& c1, Cplx<T>& c2, Cplx<T>& c3) {
   const Cplx<T> s02 = {c0.x + c2.x, c0.y + c2.y}, d02 = {c0.x - c2.x, c0.y - c2.y};
@@ -501,14 +507,14 @@ __device__ __forceinline__ void stockham8c(Cplx<T>* z, const Cplx<T>* tw8) {
   }
 }
 
-// fft_filter_pair for FP32 with N = 2^LOGN <= 2048 at compile time; filtered
-// cell j of the rows is z[fft_slot(j)].x (va) and .y (vb).
+// fft_filter_pair for FP32 with N = 2^LOGN <= 4096 at compile time (tw holds
+// N entries); filtered cell j of the rows is z[fft_slot(j)].x (va) and .y (vb).
 template <int LOGN>
 __device__ __forceinline__ void fft_filter_pair_c(const float* __restrict__ in, const double* __restrict__ w,
                                                   const float* __restrict__ hre, const float* __restrict__ him,
                                                   const Cplx<float>* __restrict__ twg, int va, int vb, int n,
                                                   Cplx<float>* z, Cplx<float>* tw) {
-  static_assert(LOGN >= 1 && LOGN <= 11, "FP32 compile-time transforms up to 2048 points");
+  static_assert(LOGN >= 1 && LOGN <= 12, "FP32 compile-time transforms up to 4096 points");
   constexpr int N = 1 << LOGN;
   constexpr int IT = N >= 256 ? N / 256 : 1, HT = (N / 2 + 1 + 255) / 256;
   float xa[IT], xb[IT];
@@ -534,7 +540,7 @@ __device__ __forceinline__ void fft_filter_pair_c(const float* __restrict__ in, 
       mi[i] = him[k];
     }
   }
-  pass_twiddles8(twg, tw, N, LOGN);
+  pass_twiddles8<float, (N > 2048 ? 16 : 8)>(twg, tw, N, LOGN);
 #pragma unroll
   for (int i = 0; i < IT; i++) {
     const int j = threadIdx.x + i * 256;

@@ -447,10 +447,11 @@ int launch_fft(ctsim_ctx* ctx, const FilterPlan& p, const float* d_in, int nView
   const int grid = (nView + 1) / 2;
   const int R = (N / 2 + 255) / 256;
   if constexpr (sizeof(T) == 4 && sizeof(TO) == 4) {
-    if (logN >= 6 && logN <= 11) {
+    if (dev::fft_c_length(logN)) {
+      const size_t ldsC = dev::fft_c_lds(N);
 #define CTSIM_FFT_C(L)                                                                         \
   case L:                                                                                      \
-    hipLaunchKernelGGL((fft_filter_kernel_c<L>), dim3(grid), dim3(256), lds, ctx->stream, d_in, w, \
+    hipLaunchKernelGGL((fft_filter_kernel_c<L>), dim3(grid), dim3(256), ldsC, ctx->stream, d_in, w, \
                        hre, him, tw, reinterpret_cast<float*>(d_out), nView, p.n, outStride);  \
     break;
       switch (logN) {
@@ -460,6 +461,7 @@ int launch_fft(ctsim_ctx* ctx, const FilterPlan& p, const float* d_in, int nView
         CTSIM_FFT_C(9)
         CTSIM_FFT_C(10)
         CTSIM_FFT_C(11)
+        CTSIM_FFT_C(12)
       }
 #undef CTSIM_FFT_C
       CTSIM_HIP(hipGetLastError());
@@ -597,10 +599,10 @@ int filter_launch_coef(ctsim_ctx* ctx, FilterPlan& p, const float* d_in, int nVi
   const double* w = p.preweight.empty() ? nullptr : p.d_weight.as<double>();
   const int grid = (nView + 1) / 2;
   const int R = (N / 2 + 255) / 256;
-  if (logN >= 6 && logN <= 11) {
+  if (dev::fft_c_length(logN)) {
     auto goc = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, ctx->stream, d_in, w, hre, him, tw, d_coef,
-                         nView, p.n);
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(256), dev::fft_c_lds(N), ctx->stream, d_in, w, hre, him,
+                         tw, d_coef, nView, p.n);
     };
 #define CTSIM_COEF_C(L)                                                                       \
   case L:                                                                                     \
@@ -615,6 +617,7 @@ int filter_launch_coef(ctsim_ctx* ctx, FilterPlan& p, const float* d_in, int nVi
       CTSIM_COEF_C(9)
       CTSIM_COEF_C(10)
       CTSIM_COEF_C(11)
+      CTSIM_COEF_C(12)
     }
 #undef CTSIM_COEF_C
     CTSIM_HIP(hipGetLastError());

@@ -131,7 +131,9 @@ def test_batch_all_symmetric(gpu_ctx, P, fused, sym):
     rows = [("equiangular", 512, 360, 256, "cubic", "fft", "abs_hanning"),
             ("parallel", 300, 200, 128, "linear", "fft", "abs_bandlimit"),
             ("equilinear", 367, 300, 256, "linear", "convolution", "shepp"),
-            ("equiangular", 400, 240, 128, "cubic", "fft", "abs_hanning")]
+            ("equiangular", 400, 240, 128, "cubic", "fft", "abs_hanning"),
+            # a 4096-point FFT convolution (2 nDet - 1 > 2048)
+            ("parallel", 1100, 200, 256, "linear", "convolution", "abs_bandlimit")]
     specs = [W.SliceSpec(elems=W.seeded_elements(20 + k, 0xC7511 + 950 + k), geometry=g, nDet=nd,
                          nView=nv, rotLen=np.pi if g == "parallel" else 2 * np.pi, filter=filt,
                          method=meth, interp=interp, backproj="idiff", filterParam=-1.0, nx=nx, ny=nx)
