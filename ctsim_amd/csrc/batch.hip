@@ -175,7 +175,8 @@ __device__ __forceinline__ void filter_item(const SliceDev& s, int pair, Cplx<fl
 // SYM instantiates the symmetric body (P <= 2); the plain kernel leaves it
 // out and keeps its own register budget.  DO_F / DO_B instantiate the filter
 // and the backprojection items: the fused launch takes both, the two-phase
-// launches one each, so each phase runs at its own register budget.
+// filter launches only the first (the two-phase backprojection runs in the
+// class kernels below, one body each).
 template <int P, bool SYM, bool DO_F, bool DO_B, int FL = 0>
 __device__ __forceinline__ void fbp_batch_body(const SliceDev* __restrict__ slices,
                                                const Item* __restrict__ items,
@@ -267,7 +268,7 @@ __device__ __forceinline__ void fbp_batch_body(const SliceDev* __restrict__ slic
   }
 }
 
-// The fused and the filter launches, and the backprojection launch.
+// The fused launch and the generic filter launch.
 template <int P, bool SYM, bool DO_F, bool DO_B>
 __global__ __launch_bounds__(256) void fbp_batch_kernel(const SliceDev* __restrict__ slices,
                                                         const Item* __restrict__ items,
@@ -286,12 +287,18 @@ __global__ __launch_bounds__(256) void fbp_batch_filter_kernel(const SliceDev* _
   fbp_batch_body<1, false, true, false, FL>(slices, items, queueStart, counters, error);
 }
 
-template <int P, bool SYM>
+
+// The two-phase backprojection launch of the plain-body items of a batch
+// whose slices span several classes: one launch for all of them, dealt over
+// the XCD queues as a whole (per-class launches each wait for their most
+// loaded queue, which on a mixed batch cost more than the register budget
+// of the largest body).
+template <int P>
 __global__ __launch_bounds__(256) void fbp_batch_bp_kernel(const SliceDev* __restrict__ slices,
-                                                              const Item* __restrict__ items,
-                                                              const int* __restrict__ queueStart,
-                                                              int* counters, int* error) {
-  fbp_batch_body<P, SYM, false, true>(slices, items, queueStart, counters, error);
+                                                           const Item* __restrict__ items,
+                                                           const int* __restrict__ queueStart,
+                                                           int* counters, int* error) {
+  fbp_batch_body<P, false, false, true>(slices, items, queueStart, counters, error);
 }
 
 // The two-phase backprojection launch of the symmetric slices of one class
@@ -302,7 +309,7 @@ template <int P, int GK, int INTERP, int DEG, int S, int K>
 __global__ __launch_bounds__(256) void fbp_batch_sym_class_kernel(const SliceDev* __restrict__ slices,
                                                                   const Item* __restrict__ items,
                                                                   const int* __restrict__ queueStart,
-                                                                  int* counters) {
+                                                                  int* counters, int* /*error: no waits here*/) {
   extern __shared__ float4 smem[];
   __shared__ dev::SymWin s_win[256];
   __shared__ float2 s_ab[64];
@@ -340,6 +347,53 @@ const void* sym_class_kernel_g(int gk, int deg, int S, int K) {
     case 8: return sym_class_kernel_sk<P, 1, INTERP, 8>(S, K);
     default: return sym_class_kernel_sk<P, 1, INTERP, 0>(S, K);
   }
+}
+
+// The two-phase backprojection launch of the plain-body items of one class
+// (geometry, interpolation, fan polynomial degree) at that body's register
+// budget.
+template <int P, int GK, int INTERP, int DEG>
+__global__ __launch_bounds__(256) void fbp_batch_plain_class_kernel(const SliceDev* __restrict__ slices,
+                                                                    const Item* __restrict__ items,
+                                                                    const int* __restrict__ queueStart,
+                                                                    int* counters, int* /*error: no waits here*/) {
+  extern __shared__ float4 smem[];
+  __shared__ float4 s_v[dev::kMaxChunk + 2];
+  __shared__ int s_lo[dev::kMaxChunk];
+  __shared__ int s_item;
+  const int q = blockIdx.x & 7;
+  const int qBegin = queueStart[q], qEnd = queueStart[q + 1];
+  while (true) {
+    if (threadIdx.x == 0) s_item = qBegin + atomicAdd(&counters[q], 1);
+    __syncthreads();
+    const int i = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (i >= qEnd) break;  // every workgroup ends when its queue is drained
+    const Item it = items[i];
+    const SliceDev& s = slices[it.slice];
+    dev::bp_mixed_tile<GK, INTERP, P, DEG>(s.bp, it.a, 0, reinterpret_cast<CoefT<INTERP>*>(smem), s_v, s_lo);
+    __syncthreads();  // LDS is reused by the next item
+  }
+}
+
+template <int P, int INTERP>
+const void* plain_class_kernel_g(int gk, int deg) {
+  if (gk == 0) return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 0, INTERP, 0>);
+  if (gk == 2) return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 2, INTERP, 0>);
+  switch (deg) {
+    case 4: return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 1, INTERP, 4>);
+    case 6: return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 1, INTERP, 6>);
+    case 8: return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 1, INTERP, 8>);
+    default: return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 1, INTERP, 0>);
+  }
+}
+
+// The class kernel of plain-body slices (P 1, 2 or 4).
+const void* plain_class_kernel(int P, int gk, int interp, int deg) {
+  const bool cubic = interp == INTERP_CUBIC;
+  if (P == 1) return cubic ? plain_class_kernel_g<1, INTERP_CUBIC>(gk, deg) : plain_class_kernel_g<1, INTERP_LINEAR>(gk, deg);
+  if (P == 2) return cubic ? plain_class_kernel_g<2, INTERP_CUBIC>(gk, deg) : plain_class_kernel_g<2, INTERP_LINEAR>(gk, deg);
+  return cubic ? plain_class_kernel_g<4, INTERP_CUBIC>(gk, deg) : plain_class_kernel_g<4, INTERP_LINEAR>(gk, deg);
 }
 
 // The class kernel of symmetric slices (P 1 or 2).
@@ -394,7 +448,7 @@ struct ctsim_batch {
   int sym = 1;  // 0: plain body; 1, 2, 4: symmetric body with that many view groups per tile
   int ldsFilter = 0;
   std::vector<SliceDev> host;
-  DevBuf d_slices, d_itemsF, d_itemsB, d_itemsAll, d_qF, d_qB, d_qAll, d_counters, d_error;
+  DevBuf d_slices, d_itemsF, d_itemsAll, d_qF, d_qAll, d_counters, d_error;
   // two-phase filter launches, one per transform length 2^6 ... 2^11
   // (specialised kernels) and one for the other lengths (logN 0)
   struct FilterClass {
@@ -403,21 +457,23 @@ struct ctsim_batch {
   };
   std::vector<FilterClass> fclass;
   DevBuf d_itemsFc, d_qFc;
-  // two-phase backprojection: the plain-body and non-symmetric items in one
-  // launch (d_itemsB), the symmetric slices' items in one launch per class
+  // two-phase backprojection: one launch per class of symmetric slices (body,
+  // geometry, interpolation, fan polynomial degree, symmetry, P), and the
+  // plain-body slices in one launch of their class kernel when they share a
+  // class, else in one generic launch
   struct BpClass {
     const void* kernel = nullptr;
     int qOff = 0, grid = 0;
     size_t lds = 0;
+    bool generic = false;  // fbp_batch_bp_kernel (takes the error flag too)
   };
   std::vector<BpClass> bclass;
   DevBuf d_itemsBc, d_qBc;
-  int nPlainB = 0;  // items of the d_itemsB launch
   DevBuf d_partial;      // view-group partial images of the symmetric slices
   DevBuf d_combine;      // slices whose view groups are added after the backprojection
   int nCombine = 0;
-  int gridF = 0, gridB = 0, gridAll = 0;
-  size_t ldsB = 0, ldsAll = 0;
+  int gridF = 0, gridAll = 0;
+  size_t ldsAll = 0;
   std::vector<const float*> sinos;
   std::vector<float*> images;
   hipStream_t stream = nullptr;
@@ -450,11 +506,13 @@ const void* filter_kernel_of(int logN) {
   }
 }
 
-void launch_filter_class(const ctsim_batch& b, const ctsim_batch::FilterClass& fc, hipStream_t st) {
+// heads: this launch's 8 queue heads (the filter-only body never touches the
+// per-slice done counts that follow the fused launch's heads)
+void launch_filter_class(const ctsim_batch& b, const ctsim_batch::FilterClass& fc, int* heads, hipStream_t st) {
   const SliceDev* sl = b.d_slices.as<SliceDev>();
   const Item* it = b.d_itemsFc.as<Item>();
   const int* q = b.d_qFc.as<int>() + fc.qOff;
-  int* c = b.d_counters.as<int>();
+  int* c = heads;
   int* e = b.d_error.as<int>();
   const dim3 g(fc.grid), blk(256);
   switch (fc.logN) {
@@ -468,6 +526,13 @@ void launch_filter_class(const ctsim_batch& b, const ctsim_batch::FilterClass& f
     default:
       hipLaunchKernelGGL((fbp_batch_kernel<1, false, true, false>), g, blk, fc.lds, st, sl, it, q, c, e);
   }
+}
+
+// counters: [0, 8) the fused launch's queue heads, [8, 8 + nSlices) its filter
+// items done per slice, then 8 queue heads for each two-phase launch (filter
+// classes, then backprojection classes), all zeroed by one reset kernel.
+size_t batch_counters(const ctsim_batch& b) {
+  return 8 + b.s.size() + 8 * (b.fclass.size() + b.bclass.size());
 }
 
 // The symmetric instantiation runs when some slice takes the symmetric body.
@@ -662,19 +727,28 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
                  : static_cast<size_t>(maxN + dev::fft_tw_entries<float>(maxN)) * sizeof(Cplx<float>);
     b.fclass.push_back(fc);
   }
-  // the two-phase backprojection: symmetric slices by class (geometry,
-  // interpolation, polynomial degree, symmetry), each class's items in the
-  // per-XCD order above; the other items in the plain launch
+  // the two-phase backprojection by class (body, geometry, interpolation,
+  // polynomial degree, symmetry, P), each class's items in the per-XCD order
+  // above
+  // plain-body slices of several classes share one generic launch (key -1)
+  auto plainKey = [&](int k) {
+    const SliceDev& d = b.host[k];
+    return d.gk + 3 * (d.interp == INTERP_CUBIC ? 1 : 0) + 6 * d.deg + 900 * d.P;
+  };
+  std::vector<int> plainKeys;
+  for (int k = 0; k < S; k++)
+    if (!b.host[k].sym && std::find(plainKeys.begin(), plainKeys.end(), plainKey(k)) == plainKeys.end())
+      plainKeys.push_back(plainKey(k));
+  const bool genericPlain = plainKeys.size() > 1;
   auto bclassKey = [&](int k) {
     const SliceDev& d = b.host[k];
-    return d.sym ? 1 + d.gk + 3 * (d.interp == INTERP_CUBIC ? 1 : 0) + 6 * d.deg + 60 * d.symS +
-                       300 * d.symK + 900 * d.P
-                 : 0;
+    if (!d.sym) return genericPlain ? -1 : plainKey(k);
+    return d.gk + 3 * (d.interp == INTERP_CUBIC ? 1 : 0) + 6 * d.deg + 60 * d.symS + 300 * d.symK +
+           900 * d.P + 9000;
   };
   std::vector<int> keys;
   for (int k = 0; k < S; k++)
-    if (bclassKey(k) && std::find(keys.begin(), keys.end(), bclassKey(k)) == keys.end())
-      keys.push_back(bclassKey(k));
+    if (std::find(keys.begin(), keys.end(), bclassKey(k)) == keys.end()) keys.push_back(bclassKey(k));
   std::vector<Item> itemsBc;
   std::vector<int> qBc;
   b.bclass.clear();
@@ -692,31 +766,34 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
     }
     qBc.push_back(static_cast<int>(itemsBc.size()));
     const SliceDev& d = b.host[rep];
-    bc.kernel = sym_class_kernel(d.P, d.gk, d.interp, d.deg, d.symS, d.symK);
-    bc.lds = static_cast<size_t>(4 * d.symS) * dev::kPipeRow * (d.interp == INTERP_CUBIC ? 16 : 8);
+    const size_t elem = d.interp == INTERP_CUBIC ? 16 : 8;
+    if (d.sym) {
+      bc.kernel = sym_class_kernel(d.P, d.gk, d.interp, d.deg, d.symS, d.symK);
+      bc.lds = static_cast<size_t>(4 * d.symS) * dev::kPipeRow * elem;
+    } else {
+      bc.generic = key < 0;
+      bc.kernel = !bc.generic ? plain_class_kernel(d.P, d.gk, d.interp, d.deg)
+                  : d.P == 4  ? reinterpret_cast<const void*>(fbp_batch_bp_kernel<4>)
+                  : d.P == 1  ? reinterpret_cast<const void*>(fbp_batch_bp_kernel<1>)
+                              : reinterpret_cast<const void*>(fbp_batch_bp_kernel<2>);
+      for (int k = 0; k < S; k++)  // the largest staging of the launch's slices
+        if (bclassKey(k) == key)
+          bc.lds = std::max(bc.lds, static_cast<size_t>(b.host[k].bp.chunk) * b.host[k].bp.W *
+                                        (b.host[k].interp == INTERP_CUBIC ? 16 : 8));
+    }
     b.bclass.push_back(bc);
   }
-  size_t plainLds = 0;
-  for (int k = 0; k < S; k++)
-    if (!b.host[k].sym)
-      plainLds = std::max(plainLds, static_cast<size_t>(b.host[k].bp.chunk) * b.host[k].bp.W *
-                                        (b.host[k].interp == INTERP_CUBIC ? 16 : 8));
-  std::vector<Item> itemsF, itemsB, itemsAll;
-  std::vector<int> qF(9), qB(9), qAll(9);
+  std::vector<Item> itemsF, itemsAll;
+  std::vector<int> qF(9), qAll(9);
   for (int x = 0; x < 8; x++) {
     qF[x] = static_cast<int>(itemsF.size());
-    qB[x] = static_cast<int>(itemsB.size());
     qAll[x] = static_cast<int>(itemsAll.size());
     itemsF.insert(itemsF.end(), f[x].begin(), f[x].end());
-    for (const Item& it : bpq[x])
-      if (!b.host[it.slice].sym) itemsB.push_back(it);
     itemsAll.insert(itemsAll.end(), f[x].begin(), f[x].end());
     itemsAll.insert(itemsAll.end(), bpq[x].begin(), bpq[x].end());
   }
   qF[8] = static_cast<int>(itemsF.size());
-  qB[8] = static_cast<int>(itemsB.size());
   qAll[8] = static_cast<int>(itemsAll.size());
-  b.nPlainB = static_cast<int>(itemsB.size());
   auto up = [](DevBuf& d, const void* src, size_t n) -> int {
     CTSIM_CHECK(d.reserve(n > 0 ? n : 16));
     if (n > 0) CTSIM_HIP(hipMemcpy(d.ptr, src, n, hipMemcpyHostToDevice));
@@ -726,20 +803,17 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
   CTSIM_CHECK(up(b.d_itemsF, itemsF.data(), itemsF.size() * sizeof(Item)));
   CTSIM_CHECK(up(b.d_itemsFc, itemsFc.data(), itemsFc.size() * sizeof(Item)));
   CTSIM_CHECK(up(b.d_qFc, qFc.data(), qFc.size() * sizeof(int)));
-  CTSIM_CHECK(up(b.d_itemsB, itemsB.data(), itemsB.size() * sizeof(Item)));
   if (!b.bclass.empty()) {
     CTSIM_CHECK(up(b.d_itemsBc, itemsBc.data(), itemsBc.size() * sizeof(Item)));
     CTSIM_CHECK(up(b.d_qBc, qBc.data(), qBc.size() * sizeof(int)));
   }
   CTSIM_CHECK(up(b.d_itemsAll, itemsAll.data(), itemsAll.size() * sizeof(Item)));
   CTSIM_CHECK(up(b.d_qF, qF.data(), qF.size() * sizeof(int)));
-  CTSIM_CHECK(up(b.d_qB, qB.data(), qB.size() * sizeof(int)));
   CTSIM_CHECK(up(b.d_qAll, qAll.data(), qAll.size() * sizeof(int)));
-  CTSIM_CHECK(b.d_counters.reserve((8 + S) * sizeof(int)));
+  CTSIM_CHECK(b.d_counters.reserve(batch_counters(b) * sizeof(int)));
   if (!combine.empty()) CTSIM_CHECK(up(b.d_combine, combine.data(), combine.size() * sizeof(int)));
   CTSIM_CHECK(b.d_error.reserve(sizeof(int)));
   CTSIM_HIP(hipMemset(b.d_error.ptr, 0, sizeof(int)));
-  b.ldsB = plainLds;
   b.ldsAll = std::max(maxBpLds, static_cast<size_t>(b.ldsFilter));
   const bool symKernel = batch_sym_kernel(b);
   auto pick = [&](auto k4, auto k1s, auto k1, auto k2s, auto k2) -> const void* {
@@ -750,13 +824,9 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
   const void* kAll = pick(fbp_batch_kernel<4, false, true, true>, fbp_batch_kernel<1, true, true, true>,
                           fbp_batch_kernel<1, false, true, true>, fbp_batch_kernel<2, true, true, true>,
                           fbp_batch_kernel<2, false, true, true>);
-  const void* kB = b.P == 4   ? reinterpret_cast<const void*>(fbp_batch_bp_kernel<4, false>)
-                   : b.P == 1 ? reinterpret_cast<const void*>(fbp_batch_bp_kernel<1, false>)
-                              : reinterpret_cast<const void*>(fbp_batch_bp_kernel<2, false>);
   const void* kF = reinterpret_cast<const void*>(fbp_batch_kernel<1, false, true, false>);
   b.gridF = occupancy_grid(kF, b.ldsFilter, ctx->numCUs);
   for (auto& fc : b.fclass) fc.grid = occupancy_grid(filter_kernel_of(fc.logN), fc.lds, ctx->numCUs);
-  b.gridB = occupancy_grid(kB, b.ldsB, ctx->numCUs);
   for (auto& bc : b.bclass) bc.grid = occupancy_grid(bc.kernel, bc.lds, ctx->numCUs);
   b.gridAll = occupancy_grid(kAll, b.ldsAll, ctx->numCUs);
   b.shapeDirty = false;
@@ -770,32 +840,26 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
 template <int P, bool SYM>
 void launch_batch(ctsim_batch& b, hipStream_t st) {
   const int S = static_cast<int>(b.s.size());
-  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8 + S);
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(),
+                     static_cast<int>(batch_counters(b)));
   if (b.fused) {
     hipLaunchKernelGGL((fbp_batch_kernel<P, SYM, true, true>), dim3(b.gridAll), dim3(256), b.ldsAll, st,
                        b.d_slices.as<SliceDev>(), b.d_itemsAll.as<Item>(), b.d_qAll.as<int>(),
                        b.d_counters.as<int>(), b.d_error.as<int>());
   } else {
-    // one filter launch per transform length class (queue heads reset in
-    // between; the per-slice done counts carry over), then the backprojection
-    for (size_t c = 0; c < b.fclass.size(); c++) {
-      if (c > 0) hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8);
-      launch_filter_class(b, b.fclass[c], st);
-    }
-    if (b.nPlainB > 0) {
-      hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8);
-      hipLaunchKernelGGL((fbp_batch_bp_kernel<P, false>), dim3(b.gridB), dim3(256), b.ldsB, st,
-                         b.d_slices.as<SliceDev>(), b.d_itemsB.as<Item>(), b.d_qB.as<int>(),
-                         b.d_counters.as<int>(), b.d_error.as<int>());
-    }
+    // one filter launch per transform length class, then one backprojection
+    // launch per class, each with queue heads of its own
+    int* heads = b.d_counters.as<int>() + 8 + S;
+    for (size_t c = 0; c < b.fclass.size(); c++, heads += 8) launch_filter_class(b, b.fclass[c], heads, st);
     for (const auto& bc : b.bclass) {
-      hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(), 8);
       const SliceDev* sl = b.d_slices.as<SliceDev>();
       const Item* it = b.d_itemsBc.as<Item>();
       const int* q = b.d_qBc.as<int>() + bc.qOff;
-      int* c = b.d_counters.as<int>();
-      void* args[] = {&sl, &it, &q, &c};
+      int* c = heads;
+      int* e = b.d_error.as<int>();
+      void* args[] = {&sl, &it, &q, &c, &e};
       (void)hipLaunchKernel(bc.kernel, dim3(bc.grid), dim3(256), args, bc.lds, st);
+      heads += 8;
     }
   }
   if (b.nCombine > 0)

@@ -152,3 +152,31 @@ def test_batch_all_symmetric(gpu_ctx, P, fused, sym):
             d = ctsim_amd.image_stats(gpu_ctx, r64, img)[0]
             print(f"  rep {rep} {s.geometry} {s.nDet}x{s.nView} {s.nx}^2: d={d:.2e}", flush=True)
             assert d <= 1e-5
+
+
+@pytest.mark.parametrize("P", [1, 4])
+def test_batch_one_plain_class(gpu_ctx, P):
+    """A batch whose slices share one plain-body class (no rotated view copies:
+    odd view counts) runs that class's own kernel in two-phase mode; each
+    slice against its FP64 single-slice image."""
+    import torch
+    import ctsim_amd
+    from ctsim_amd import fbp
+    from ctsim_amd import workloads as W
+    specs = [W.SliceSpec(elems=W.seeded_elements(18 + k, 0xC7511 + 970 + k), geometry="equilinear",
+                         nDet=300 + 40 * k, nView=181 + 20 * k, rotLen=2 * np.pi, filter="abs_hanning",
+                         method="fft", interp="linear", backproj="idiff", nx=128 + 64 * k, ny=128 + 64 * k)
+             for k in range(3)]
+    print(f"case one-plain-class batch P={P}", flush=True)
+    sims = [fbp.simulate(gpu_ctx, s) for s in specs]
+    ref64 = [fbp.reconstruct(gpu_ctx, s, sino, ang, info, ctsim_amd.PREC_F64)
+             for s, (sino, ang, info) in zip(specs, sims)]
+    b = ctsim_amd.Batch(gpu_ctx, [fbp.slice_desc(s, info, ang) for s, (_, ang, info) in zip(specs, sims)])
+    b.set_shape(P, 16, 16384, False, 1)
+    images = [torch.full((s.nx, s.ny), 7.0, dtype=torch.float32, device="cuda") for s in specs]
+    b.run([x[0] for x in sims], images)
+    b.status()
+    for s, img, r64 in zip(specs, images, ref64):
+        d = ctsim_amd.image_stats(gpu_ctx, r64, img)[0]
+        print(f"  {s.nDet}x{s.nView} {s.nx}^2: d={d:.2e}", flush=True)
+        assert d <= 1e-5
