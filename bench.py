@@ -55,15 +55,19 @@ KERNEL_BY_SYM = {0: "bp_mixed_kernel<equiangular, cubic>",
 def measured_traffic():
     """HBM bytes per launch of the pipelined symmetric backprojection kernel
     on this workload, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary (copy-calibrated), or None when the summary is absent."""
+    summary (copy-calibrated), and the launch shape that run pinned
+    (lanesY,P,split,chunk,sym,KB); (None, None) when the summary is absent."""
+    traffic, shape = None, None
     try:
         with open(TRAFFIC_SUMMARY) as fh:
             for line in fh:
                 if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+                    traffic = float(line.split(":", 1)[1])
+                elif "--shape " in line and shape is None:
+                    shape = line.split("--shape ", 1)[1].split()[0]
     except OSError:
         pass
-    return None
+    return traffic, shape
 
 
 def cpu_model():
@@ -314,7 +318,7 @@ def main():
     flops = FLOPS_PER_PVU_EQUIANGULAR_CUBIC_KAHAN * pvu
     achieved = flops / (bp_ms * 1e-3) / 1e12
     hbm_bytes = 4 * spec.nView * spec.nDet + 2 * 4 * spec.nx * spec.ny
-    traffic = measured_traffic()
+    traffic, traffic_shape = measured_traffic()
 
     out = {
         "metric": METRIC,
@@ -340,6 +344,9 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic if shape[5] == 2 else None,
+                     # the partial images of the view groups scale the writes:
+                     # the profile's shape may differ from the autotuned one
+                     "traffic_launch_shape": traffic_shape if shape[5] == 2 else None,
                      "kernel": KERNEL_BY_SYM[shape[5]], "kernel_ms": bp_ms,
                      "kernel_timing": "HIP events around each backprojection launch on the "
                                       "library's stream, over a replay of the K timed steps "
