@@ -65,9 +65,11 @@ void* ctsim_ctx_stream(const ctsim_ctx* ctx);
 int ctsim_ctx_synchronize(ctsim_ctx* ctx);
 /* No counterpart in the reference (measurement).  While enabled, the main
  * backprojection kernel of every launch on this context is bracketed by HIP
- * events on the context's stream; ctsim_ctx_profile_read waits for them and
- * returns the summed kernel time and the number of launches since the last
- * read (or since enabling). */
+ * events on the context's stream (timing-only events, without the
+ * system-scope fence; each record still adds a few microseconds between
+ * kernels, so time throughput with profiling off); ctsim_ctx_profile_read
+ * waits for them and returns the summed kernel time and the number of
+ * launches since the last read (or since enabling). */
 int ctsim_ctx_profile(ctsim_ctx* ctx, int enable);
 int ctsim_ctx_profile_read(ctsim_ctx* ctx, double* totalMs, int* count);
 /* Message of the last failure on this thread (reference: fail()/failMessage(),
