@@ -42,6 +42,7 @@ SIGNATURES = {
     "ctsim_phantom_destroy": (None, [_vp]),
     "ctsim_phantom_add_element": (_i, [_vp, _cp, _d, _d, _d, _d, _d, _d]),
     "ctsim_phantom_extent": (_i, [_vp, _vp]),
+    "ctsim_phantom_element": (_i, [_vp, _i, _vp]),
     "ctsim_scanner_create": (_i, [_pp, _vp, _cp, _i, _i, _i, _i, _d, _d, _d, _d, _d]),
     "ctsim_scanner_destroy": (None, [_vp]),
     "ctsim_scanner_info": (_i, [_vp, _vp]),

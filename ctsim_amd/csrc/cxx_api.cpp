@@ -40,7 +40,9 @@ void now_fields(int* f) {
   f[3] = lt->tm_hour; f[4] = lt->tm_min; f[5] = lt->tm_sec;
 }
 
-void warn(const std::string& msg) { std::cerr << "WARNING ERROR: " << msg << "\n"; }
+// sys_error(ERR_WARNING, ...) prints to standard output and goes on
+// (libctsupport/syserror.cpp:45-77).
+void warn(const std::string& msg) { std::cout << "WARNING ERROR: " << msg << "\n"; }
 
 }  // namespace
 
@@ -76,6 +78,181 @@ int Trace::convertTraceNameToID(const char* name) {  // libctsim/trace.cpp:41-61
   return TRACE_INVALID;
 }
 
+// ------------------------------------------------------------------ Timer
+void Timer::timerReport(const char* msg) const {
+  std::cout << msg << ": " << m_timeElapsed << " seconds" << std::endl;
+}
+
+// ------------------------------------------------------- Array2dFileLabel
+// array2dfile.cpp:44-158: a new label carries the local date and time.
+Array2dFileLabel::Array2dFileLabel() {
+  int f[6];
+  now_fields(f);
+  setDateTime(f[0], f[1], f[2], f[3], f[4], f[5]);
+}
+
+Array2dFileLabel::Array2dFileLabel(const char* str, double ctime) : Array2dFileLabel() {
+  m_strLabel = str ? str : "";
+  m_labelType = L_USER;
+  m_calcTime = ctime;
+}
+
+Array2dFileLabel::Array2dFileLabel(int type, const char* str, double ctime) : Array2dFileLabel() {
+  m_strLabel = str ? str : "";
+  m_labelType = type;
+  m_calcTime = ctime;
+}
+
+void Array2dFileLabel::setDateTime(int year, int month, int day, int hour, int minute, int second) {
+  m_year = year; m_month = month; m_day = day;
+  m_hour = hour; m_minute = minute; m_second = second;
+}
+
+void Array2dFileLabel::getDateTime(int& year, int& month, int& day, int& hour, int& minute,
+                                   int& second) const {
+  year = m_year; month = m_month; day = m_day;
+  hour = m_hour; minute = m_minute; second = m_second;
+}
+
+// "M/DD/YYYY HH:MM:SS" with tm-style year and month fields.
+const std::string& Array2dFileLabel::getDateString() const {
+  char buf[128];
+  snprintf(buf, sizeof buf, "%2d/%02d/%4d %02d:%02d:%02d", m_month + 1, m_day, m_year + 1900,
+           m_hour, m_minute, m_second);
+  m_strDate = buf;
+  return m_strDate;
+}
+
+// The reference prints a literal "%s" before the user and unknown labels'
+// timestamps (array2dfile.cpp:125-131); kept so the text matches.
+void Array2dFileLabel::print(std::ostream& os) const {
+  if (m_labelType == L_HISTORY) {
+    os << "History: " << std::endl << "  " << m_strLabel << std::endl;
+    os << "  calc time = " << m_calcTime << " secs" << std::endl;
+    os << "  Timestamp = " << getDateString() << std::endl;
+  } else {
+    if (m_labelType == L_USER) os << "Note: " << m_strLabel << std::endl;
+    else os << "Unknown (" << m_labelType << "): " << m_strLabel << std::endl;
+    os << "  Timestamp = %s" << getDateString() << std::endl;
+  }
+}
+
+void Array2dFileLabel::printBrief(std::ostream& os) const {
+  if (m_labelType == L_HISTORY) {
+    os << "History (";
+    if (m_calcTime > 0) os << m_calcTime << " secs, ";
+    os << getDateString() << "): " << m_strLabel << std::endl;
+  } else if (m_labelType == L_USER) {
+    os << "Note (" << getDateString() << "): " << m_strLabel << std::endl;
+  } else {
+    os << "Unknown (" << getDateString() << "): " << m_strLabel << std::endl;
+  }
+}
+
+// ------------------------------------------------------ static name tables
+namespace {
+
+// Index of name in the table (case-insensitive), or -1.
+int lookup(const char* const* table, int n, const char* name) {
+  for (int i = 0; i < n; i++)
+    if (name && !strcasecmp(name, table[i])) return i;
+  return -1;
+}
+
+const char* entry(const char* const* table, int n, int id) {
+  return id >= 0 && id < n ? table[id] : "";
+}
+
+// libctsim/filter.cpp:46-75 (names 4 and 5 swapped against their IDs)
+const char* const kFilterNames[] = {"abs_bandlimit", "abs_hamming", "abs_hanning", "abs_cosine",
+                                    "shepp", "abs_sinc", "bandlimit", "sinc", "hamming",
+                                    "hanning", "cosine", "triangle"};
+const char* const kFilterTitles[] = {"Abs(w) * Bandlimit", "Abs(w) * Hamming", "Abs(w) * Hanning",
+                                     "Abs(w) * Cosine", "Shepp", "Abs(w) * Sinc", "Bandlimit",
+                                     "Sinc", "Hamming", "Hanning", "Cosine", "Triangle"};
+const char* const kDomainNames[] = {"frequency", "spatial"};
+const char* const kDomainTitles[] = {"Frequency", "Spatial"};
+// libctsim/procsignal.cpp:41-73 (with the FFTW entries)
+const char* const kMethodNames[] = {"convolution", "fourier", "fouier-table", "fft", "fftw",
+                                    "rfftw"};
+const char* const kMethodTitles[] = {"Convolution", "Fourier", "Fouier Trigometric Table", "FFT",
+                                     "FFTW", "Real/Half-Complex FFTW"};
+const char* const kGenerationNames[] = {"direct", "inverse-fourier"};
+const char* const kGenerationTitles[] = {"Direct", "Inverse Fourier"};
+// libctsim/backprojectors.cpp:34-97
+const char* const kBackprojNames[] = {"trig", "table", "diff", "idiff"};
+const char* const kBackprojTitles[] = {"Direct Trigometric", "Trigometric Table",
+                                       "Difference Iteration", "Integer Difference Iteration"};
+const char* const kInterpNames[] = {"nearest", "linear", "cubic"};
+const char* const kInterpTitles[] = {"Nearest", "Linear", "Cubic"};
+// libctsim/scanner.cpp:35-49
+const char* const kGeometryNames[] = {"parallel", "equiangular", "equilinear", "linogram"};
+const char* const kGeometryTitles[] = {"Parallel", "Equiangular", "Equilinear", "Linogram"};
+// libctsim/phantom.cpp:37-49
+const char* const kPhantomNames[] = {"herman", "shepp-logan", "unit-pulse"};
+const char* const kPhantomTitles[] = {"Herman Head", "Shepp-Logan", "Unit Pulse"};
+
+template <size_t N>
+constexpr int count(const char* const (&)[N]) { return static_cast<int>(N); }
+
+}  // namespace
+
+int SignalFilter::getFilterCount() { return count(kFilterNames); }
+int SignalFilter::getReconstructFilterCount() { return 4; }  // filter.cpp:44
+const char* const* SignalFilter::getFilterNameArray() { return kFilterNames; }
+const char* const* SignalFilter::getFilterTitleArray() { return kFilterTitles; }
+int SignalFilter::convertFilterNameToID(const char* n) { return lookup(kFilterNames, count(kFilterNames), n); }
+const char* SignalFilter::convertFilterIDToName(int id) { return entry(kFilterNames, count(kFilterNames), id); }
+const char* SignalFilter::convertFilterIDToTitle(int id) { return entry(kFilterTitles, count(kFilterTitles), id); }
+int SignalFilter::getDomainCount() { return count(kDomainNames); }
+const char* const* SignalFilter::getDomainNameArray() { return kDomainNames; }
+const char* const* SignalFilter::getDomainTitleArray() { return kDomainTitles; }
+int SignalFilter::convertDomainNameToID(const char* n) { return lookup(kDomainNames, count(kDomainNames), n); }
+const char* SignalFilter::convertDomainIDToName(int id) { return entry(kDomainNames, count(kDomainNames), id); }
+const char* SignalFilter::convertDomainIDToTitle(int id) { return entry(kDomainTitles, count(kDomainTitles), id); }
+
+int ProcessSignal::getFilterMethodCount() { return count(kMethodNames); }
+const char* const* ProcessSignal::getFilterMethodNameArray() { return kMethodNames; }
+const char* const* ProcessSignal::getFilterMethodTitleArray() { return kMethodTitles; }
+int ProcessSignal::convertFilterMethodNameToID(const char* n) { return lookup(kMethodNames, count(kMethodNames), n); }
+const char* ProcessSignal::convertFilterMethodIDToName(int id) { return entry(kMethodNames, count(kMethodNames), id); }
+const char* ProcessSignal::convertFilterMethodIDToTitle(int id) { return entry(kMethodTitles, count(kMethodTitles), id); }
+int ProcessSignal::getFilterGenerationCount() { return count(kGenerationNames); }
+const char* const* ProcessSignal::getFilterGenerationNameArray() { return kGenerationNames; }
+const char* const* ProcessSignal::getFilterGenerationTitleArray() { return kGenerationTitles; }
+int ProcessSignal::convertFilterGenerationNameToID(const char* n) { return lookup(kGenerationNames, count(kGenerationNames), n); }
+const char* ProcessSignal::convertFilterGenerationIDToName(int id) { return entry(kGenerationNames, count(kGenerationNames), id); }
+const char* ProcessSignal::convertFilterGenerationIDToTitle(int id) { return entry(kGenerationTitles, count(kGenerationTitles), id); }
+
+int Backprojector::getBackprojectCount() { return count(kBackprojNames); }
+const char* const* Backprojector::getBackprojectNameArray() { return kBackprojNames; }
+const char* const* Backprojector::getBackprojectTitleArray() { return kBackprojTitles; }
+int Backprojector::convertBackprojectNameToID(const char* n) { return lookup(kBackprojNames, count(kBackprojNames), n); }
+const char* Backprojector::convertBackprojectIDToName(int id) { return entry(kBackprojNames, count(kBackprojNames), id); }
+const char* Backprojector::convertBackprojectIDToTitle(int id) { return entry(kBackprojTitles, count(kBackprojTitles), id); }
+int Backprojector::getInterpCount() { return count(kInterpNames); }
+const char* const* Backprojector::getInterpNameArray() { return kInterpNames; }
+const char* const* Backprojector::getInterpTitleArray() { return kInterpTitles; }
+int Backprojector::convertInterpNameToID(const char* n) { return lookup(kInterpNames, count(kInterpNames), n); }
+const char* Backprojector::convertInterpIDToName(int id) { return entry(kInterpNames, count(kInterpNames), id); }
+const char* Backprojector::convertInterpIDToTitle(int id) { return entry(kInterpTitles, count(kInterpTitles), id); }
+
+int Scanner::getGeometryCount() { return count(kGeometryNames); }
+const char* const* Scanner::getGeometryNameArray() { return kGeometryNames; }
+const char* const* Scanner::getGeometryTitleArray() { return kGeometryTitles; }
+int Scanner::convertGeometryNameToID(const char* n) { return lookup(kGeometryNames, count(kGeometryNames), n); }
+const char* Scanner::convertGeometryIDToName(int id) { return entry(kGeometryNames, count(kGeometryNames), id); }
+// The reference returns the name here, not the title (scanner.cpp:242-251).
+const char* Scanner::convertGeometryIDToTitle(int id) { return entry(kGeometryNames, count(kGeometryNames), id); }
+
+int Phantom::getPhantomCount() { return count(kPhantomNames); }
+const char* const* Phantom::getPhantomNameArray() { return kPhantomNames; }
+const char* const* Phantom::getPhantomTitleArray() { return kPhantomTitles; }
+int Phantom::convertNameToPhantomID(const char* n) { return lookup(kPhantomNames, count(kPhantomNames), n); }
+const char* Phantom::convertPhantomIDToName(int id) { return entry(kPhantomNames, count(kPhantomNames), id); }
+// The reference returns the name here, not the title (phantom.cpp:103-110).
+const char* Phantom::convertPhantomIDToTitle(int id) { return entry(kPhantomNames, count(kPhantomNames), id); }
+
 // ---------------------------------------------------------------- Phantom
 Phantom::Phantom() { ctsim_phantom_create(&m_hip, nullptr); }
 
@@ -87,40 +264,108 @@ Phantom::Phantom(const char* phmName) {
 Phantom::~Phantom() { ctsim_phantom_destroy(m_hip); }
 
 bool Phantom::createFromPhantom(const char* phmName) {
-  ctsim_phantom* p = nullptr;
-  if (ctsim_phantom_create(&p, phmName) != CTSIM_OK) {
+  const int id = convertNameToPhantomID(phmName);
+  if (id == PHM_INVALID) {
     m_fail = true;
     m_failMessage = std::string("Invalid phantom name ") + (phmName ? phmName : "");
     return false;
   }
-  ctsim_phantom_destroy(m_hip);
-  m_hip = p;
   m_name = phmName;
-  double ext[4];
-  m_nPElem = ctsim_phantom_extent(m_hip, ext);
+  return createFromPhantom(id);
+}
+
+// phantom.cpp:136-170: the built-in's elements are added to this phantom
+// (an empty phantom takes the built-in's plan as it is, unit-pulse flag
+// included).
+bool Phantom::createFromPhantom(int phmid) {
+  if (phmid < 0 || phmid >= getPhantomCount()) {
+    m_fail = true;
+    m_failMessage = "Illegal phantom id " + std::to_string(phmid);
+    return false;
+  }
+  ctsim_phantom* p = nullptr;
+  if (ctsim_phantom_create(&p, kPhantomNames[phmid]) != CTSIM_OK) {
+    m_fail = true;
+    m_failMessage = ctsim_last_error();
+    return false;
+  }
+  double e[13];
+  const int n = ctsim_phantom_extent(p, e);
+  if (m_nPElem == 0) {
+    ctsim_phantom_destroy(m_hip);
+    m_hip = p;
+    m_nPElem = n;
+  } else {
+    static const char* const types[] = {"", "rectangle", "triangle", "ellipse", "sector", "segment"};
+    for (int i = n - 1; i >= 0; i--) {  // definition order (the list is newest-first)
+      ctsim_phantom_element(p, i, e);
+      addPElem(types[static_cast<int>(e[0])], e[1], e[2], e[3], e[4], e[12], e[6]);
+    }
+    ctsim_phantom_destroy(p);
+  }
+  m_id = phmid;
   return true;
 }
 
-// Phantom::createFromFile, phantom.cpp:184-213: "type cx cy u v rot atten" lines.
+// Phantom::createFromFile, phantom.cpp:184-213.  A record with fewer than
+// seven fields warns, makes the result false and is still added; reading
+// then resumes at the next token.  The reference reads indeterminate locals
+// for the fields such a record lacks; here they keep the previous record's
+// values (zero before the first record).
 bool Phantom::createFromFile(const char* fname) {
   FILE* fp = fopen(fname, "r");
   if (!fp) return false;
   m_name = fname;
   bool good = true;
+  double cx = 0, cy = 0, u = 0, v = 0, rot = 0, dens = 0;
   while (true) {
-    double cx, cy, u, v, rot, dens;
-    char type[80];
+    char type[80] = "";
     const int status = fscanf(fp, "%79s %lf %lf %lf %lf %lf %lf", type, &cx, &cy, &u, &v, &rot, &dens);
     if (status == EOF) break;
     if (status != 7) {
-      warn(std::string("Insufficient fields reading phantom file ") + fname);
+      warn(std::string("Insufficient fields reading phantom file ") + fname +
+           " [Phantom::createFromFile]");
       good = false;
-      break;
     }
     addPElem(type, cx, cy, u, v, rot, dens);
   }
   fclose(fp);
   return good;
+}
+
+// Phantom::fileWrite, phantom.cpp:215-223: the definitions as text.
+bool Phantom::fileWrite(const char* fname) {
+  std::ofstream fs(fname);
+  if (fs) printDefinitions(fs);
+  return static_cast<bool>(fs);
+}
+
+namespace {
+const char* elem_type_name(int t) {
+  static const char* const names[] = {"Unknown", "rectangle", "triangle", "ellipse", "sector", "segment"};
+  return t >= 1 && t <= 5 ? names[t] : names[0];
+}
+}  // namespace
+
+void Phantom::print(std::ostream& os) const {
+  double e[13];
+  os << "Number of PElements: " << m_nPElem << "\n";
+  os << "Limits: xmin=" << xmin() << ", ymin=" << ymin() << ", xmax=" << xmax() << ", ymax=" << ymax() << "\n";
+  for (int i = 0; i < m_nPElem; i++) {
+    ctsim_phantom_element(m_hip, i, e);
+    os << "PhantomElement: nPoints=" << static_cast<int>(e[11]);
+    os << ", atten=" << e[6] << " rot=" << e[5] * (180. / 3.14159265358979323846) << "\n";
+    os << "xmin=" << e[7] << ", ymin=" << e[8] << ", xmax=" << e[9] << ", ymax=" << e[10] << "\n";
+  }
+}
+
+void Phantom::printDefinitions(std::ostream& os) const {
+  double e[13];
+  for (int i = 0; i < m_nPElem; i++) {
+    ctsim_phantom_element(m_hip, i, e);
+    os << elem_type_name(static_cast<int>(e[0])) << " " << e[1] << " " << e[2] << " " << e[3]
+       << " " << e[4] << " " << e[5] * (180. / 3.14159265358979323846) << " " << e[6] << "\n";
+  }
 }
 
 void Phantom::addPElem(const char* type, double cx, double cy, double u, double v, double rot,
@@ -164,18 +409,6 @@ void Phantom::convertToImagefile(ImageFile& im, double dViewRatio, int nsample, 
 }
 
 // -------------------------------------------------------------- Scanner
-int Scanner::convertGeometryNameToID(const char* name) {
-  static const char* const names[] = {"parallel", "equiangular", "equilinear", "linogram"};
-  for (int i = 0; i < 4; i++)
-    if (name && !strcasecmp(name, names[i])) return i;
-  return GEOMETRY_INVALID;
-}
-
-const char* Scanner::convertGeometryIDToName(int id) {
-  static const char* const names[] = {"parallel", "equiangular", "equilinear", "linogram"};
-  return id >= 0 && id < 4 ? names[id] : "";
-}
-
 Scanner::Scanner(const Phantom& phm, const char* geometryName, int nDet, int nView, int iOffsetView,
                  int nSample, double rot_anglen, double dFocalLengthRatio,
                  double dCenterDetectorRatio, double dViewRatio, double dScanRatio)
@@ -192,6 +425,9 @@ Scanner::Scanner(const Phantom& phm, const char* geometryName, int nDet, int nVi
 }
 
 Scanner::~Scanner() { ctsim_scanner_destroy(m_hip); }
+
+void Scanner::setNView(int nView) { m_nView = nView; }
+void Scanner::setOffsetView(int iOffsetView) { m_offsetView = iOffsetView; }
 
 void Scanner::collectProjections(Projections& proj, const Phantom& phm, int trace, SGP* pSGP) {
   collectProjections(proj, phm, 0, proj.nView(), m_offsetView, true, trace, pSGP);
@@ -244,6 +480,7 @@ Projections::Projections(int nView, int nDet) { init(nView, nDet); }
 Projections::Projections(const Scanner& s) { initFromScanner(s); }
 
 void Projections::init(int nView, int nDet) {
+  m_label.setLabelType(Array2dFileLabel::L_HISTORY);  // projections.cpp:128-143
   m_nView = nView;
   m_nDet = nDet;
   m_data.assign(static_cast<size_t>(nView) * nDet, 0.f);
@@ -272,6 +509,24 @@ void Projections::initFromScanner(const Scanner& s) {
 }
 
 double Projections::phmLen() const { return m_dViewDiameter / kSqrt2; }
+double Projections::viewLen() const { return m_dViewDiameter / kSqrt2; }
+
+void Projections::setNView(int nView) { init(nView, m_nDet); }
+
+void Projections::printScanInfo(std::ostringstream& os) const {
+  os << "Number of detectors: " << m_nDet << "\n";
+  os << "Number of views: " << m_nView << "\n";
+  os << "Description: " << m_remark.c_str() << "\n";
+  os << "Geometry: " << Scanner::convertGeometryIDToName(m_geometry) << "\n";
+  os << "Focal Length: " << m_dFocalLength << "\n";
+  os << "Source Detector Length: " << m_dSourceDetectorLength << "\n";
+  os << "View Diameter: " << m_dViewDiameter << "\n";
+  os << "Fan Beam Angle: " << m_dFanBeamAngle * (180. / 3.14159265358979323846) << "\n";
+  os << "detStart: " << m_detStart << "\n";
+  os << "detInc: " << m_detInc << "\n";
+  os << "rotStart: " << m_rotStart << "\n";
+  os << "rotInc: " << m_rotInc << "\n";
+}
 void Projections::setPhmLen(double p) { m_dViewDiameter = p * kSqrt2; }
 
 // .pj layout, projections.cpp:485-546 (header) and 816-889 (views).
@@ -328,6 +583,12 @@ bool Projections::read(const char* fname) {
   m_dViewDiameter = d[5]; m_dFocalLength = d[6]; m_dSourceDetectorLength = d[7]; m_dFanBeamAngle = d[8];
   m_year = f16[0]; m_month = f16[1]; m_day = f16[2]; m_hour = f16[3]; m_minute = f16[4]; m_second = f16[5];
   m_remark = remark;
+  m_filename = fname;
+  // projections.cpp:628-634
+  m_label.setLabelType(Array2dFileLabel::L_HISTORY);
+  m_label.setLabelString(m_remark);
+  m_label.setCalcTime(m_calcTime);
+  m_label.setDateTime(m_year, m_month, m_day, m_hour, m_minute, m_second);
   for (int v = 0; v < m_nView; v++) {
     double a;
     uint32_t nd;
@@ -441,6 +702,9 @@ Projections* Projections::interpolateToParallel() const {
   p->m_dFanBeamAngle = m_dFanBeamAngle;
   p->m_calcTime = 0;
   p->m_remark = m_remark + "; Interpolate to Parallel";
+  p->m_label.setLabelString(p->m_remark);  // projections.cpp:1264-1270
+  p->m_label.setCalcTime(0);
+  p->m_label.setDateTime(p->m_year, p->m_month, p->m_day, p->m_hour, p->m_minute, p->m_second);
   p->m_rotStart = info[0];
   p->m_rotInc = info[1];
   p->m_detStart = info[2];
@@ -482,13 +746,14 @@ void ImageFile::setAxisExtent(double minX, double maxX, double minY, double maxY
 }
 
 void ImageFile::labelAdd(const char* label, double calcTime) {
-  Label l;
-  l.type = 1;  // L_HISTORY
-  l.calcTime = calcTime;
-  l.text = label ? label : "";
-  now_fields(l.date);
-  m_labels.push_back(l);
+  labelAdd(Array2dFileLabel::L_HISTORY, label, calcTime);
 }
+
+void ImageFile::labelAdd(int type, const char* label, double calcTime) {
+  labelAdd(Array2dFileLabel(type, label, calcTime));
+}
+
+void ImageFile::labelAdd(const Array2dFileLabel& label) { m_labels.push_back(label); }
 
 // .if layout, array2dfile.cpp:443-520 (header, column-major data) and
 // 601-633 (labels); pixel format 7 = float32, data type 1 = real.
@@ -510,12 +775,15 @@ bool ImageFile::fileWrite(const char* fname) {
   put<uint16_t>(fs, m_axisExtentKnown ? 1 : 0);
   for (double d : {m_minX, m_maxX, m_minY, m_maxY, 0.0, 1.0}) put<double>(fs, d);
   fs.write(reinterpret_cast<const char*>(m_data.data()), sizeof(float) * m_data.size());
-  for (const Label& l : m_labels) {
-    put<uint16_t>(fs, l.type);
-    for (int k = 0; k < 6; k++) put<uint16_t>(fs, l.date[k]);
-    put<double>(fs, l.calcTime);
-    put<uint16_t>(fs, static_cast<uint16_t>(l.text.size()));
-    fs.write(l.text.data(), l.text.size());
+  for (const Array2dFileLabel& l : m_labels) {
+    int f[6];
+    l.getDateTime(f[0], f[1], f[2], f[3], f[4], f[5]);
+    put<uint16_t>(fs, l.getLabelType());
+    for (int k = 0; k < 6; k++) put<uint16_t>(fs, f[k]);
+    put<double>(fs, l.getCalcTime());
+    const std::string& text = l.getLabelString();
+    put<uint16_t>(fs, static_cast<uint16_t>(text.size()));
+    fs.write(text.data(), text.size());
   }
   return static_cast<bool>(fs);
 }
@@ -544,16 +812,16 @@ bool ImageFile::fileRead(const char* fname) {
   m_minX = ext[0]; m_maxX = ext[1]; m_minY = ext[2]; m_maxY = ext[3];
   m_labels.clear();
   for (int i = 0; i < nlab && fs; i++) {
-    Label l;
     uint16_t t, dt[6], len;
+    double calcTime = 0;
     get(fs, t);
     for (uint16_t& x : dt) get(fs, x);
-    get(fs, l.calcTime);
+    get(fs, calcTime);
     get(fs, len);
-    l.type = t;
-    for (int k = 0; k < 6; k++) l.date[k] = dt[k];
-    l.text.assign(len, '\0');
-    fs.read(&l.text[0], len);
+    std::string text(len, '\0');
+    fs.read(&text[0], len);
+    Array2dFileLabel l(t, text.c_str(), calcTime);
+    l.setDateTime(dt[0], dt[1], dt[2], dt[3], dt[4], dt[5]);
     m_labels.push_back(l);
   }
   return static_cast<bool>(fs) || fs.eof();
@@ -644,20 +912,6 @@ void ProcessSignal::filterSignal(const float input[], double output[]) const {
 }
 
 // -------------------------------------------------------- Backprojector
-int Backprojector::convertBackprojectNameToID(const char* name) {
-  static const char* const names[] = {"trig", "table", "diff", "idiff"};
-  for (int i = 0; i < 4; i++)
-    if (name && !strcasecmp(name, names[i])) return i;
-  return -1;
-}
-
-int Backprojector::convertInterpNameToID(const char* name) {
-  static const char* const names[] = {"nearest", "linear", "cubic"};
-  for (int i = 0; i < 3; i++)
-    if (name && !strcasecmp(name, names[i])) return i;
-  return -1;
-}
-
 Backprojector::Backprojector(const Projections& proj, ImageFile& im, const char* backprojName,
                              const char* interpName, int interpFactor,
                              const ReconstructionROI* pROI)

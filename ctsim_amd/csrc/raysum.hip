@@ -455,6 +455,20 @@ int ctsim_phantom_extent(const ctsim_phantom* p, double* ext4) {
   return static_cast<int>(p->plan.elems.size());
 }
 
+int ctsim_phantom_element(const ctsim_phantom* p, int index, double* out13) {
+  if (!p || !out13) return fail(CTSIM_ERR_INVALID, "ctsim_phantom_element: NULL argument");
+  const int n = static_cast<int>(p->plan.elems.size());
+  if (index < 0 || index >= n) return fail(CTSIM_ERR_INVALID, "ctsim_phantom_element: index out of range");
+  const int k = n - 1 - index;  // the reference's list is newest-first
+  const ElemInfo& in = p->plan.infos[k];
+  const ElemGPU& e = p->plan.elems[k];
+  const double v[13] = {static_cast<double>(in.type), in.cx, in.cy, in.u, in.v, in.rotRad, in.atten,
+                        e.lim[0], e.lim[1], e.lim[2], e.lim[3], static_cast<double>(in.nOutline),
+                        in.rotDeg};
+  for (int i = 0; i < 13; i++) out13[i] = v[i];
+  return CTSIM_OK;
+}
+
 int ctsim_scanner_create(ctsim_scanner** out, const ctsim_phantom* phm, const char* geometry,
                          int nDet, int nView, int offsetView, int nSample, double rotLen,
                          double focalLengthRatio, double centerDetectorRatio, double viewRatio,

@@ -167,6 +167,7 @@ int phantom_add(PhantomPlan& p, const char* typeName, double cx, double cy, doub
   xmn -= xf; ymn -= yf; xmx += xf; ymx += yf;
   e.lim[0] = xmn; e.lim[1] = ymn; e.lim[2] = xmx; e.lim[3] = ymx;
   p.elems.push_back(e);
+  p.infos.push_back(ElemInfo{type, n, cx, cy, u, v, rr, atten, rotDeg});
   if (p.xmin > xmn) p.xmin = xmn;
   if (p.xmax < xmx) p.xmax = xmx;
   if (p.ymin > ymn) p.ymin = ymn;

@@ -21,8 +21,18 @@ struct ElemGPU {
   int pad;
 };
 
+// An element's definition as PhantomElement keeps it (phantom.h:41-128):
+// what Phantom::print and printDefinitions report.
+struct ElemInfo {
+  int type;
+  int nOutline;  // outline points (PhantomElement::nOutlinePoints)
+  double cx, cy, u, v, rotRad, atten;
+  double rotDeg;  // the rotation as defined
+};
+
 struct PhantomPlan {
   std::vector<ElemGPU> elems;  // definition order (the reference's list is newest-first)
+  std::vector<ElemInfo> infos; // same order as elems
   double xmin = 1E30, xmax = -1E30, ymin = 1E30, ymax = -1E30;
   bool unitPulse = false;
   DevBuf d_elems;

@@ -142,10 +142,9 @@ int main(int argc, char* const argv[]) {
     usage(argv[0]);
     return 1;
   }
-  if (!phmFile.empty() && !phm.createFromFile(phmFile.c_str())) {
-    std::cerr << "Unable to read phantom file " << phmFile << "\n";
-    return 1;
-  }
+  // The reference ignores createFromFile's result (phm2pj.cpp:305); a
+  // file that cannot be opened leaves the phantom empty and the Scanner fails.
+  if (!phmFile.empty()) phm.createFromFile(phmFile.c_str());
   // The reference multiplies the turn count by 2*pi in double (phm2pj.cpp:332).
   const double twoPi = 6.28318530717958647692;
   Scanner scanner(phm, geometry.c_str(), nDet, nView, offsetView, nray, rotAngle * twoPi, focal,
@@ -163,6 +162,14 @@ int main(int argc, char* const argv[]) {
     std::cerr << "Unable to write " << outFile << "\n";
     return 1;
   }
-  if (verbose) std::cout << "  Remark: " << pj.remark() << "\nRun time: " << secs << " seconds\n";
+  if (verbose) {  // phm2pj.cpp:381-389
+    phm.print(std::cout);
+    std::cout << std::endl;
+    std::ostringstream os;
+    pj.printScanInfo(os);
+    std::cout << os.str() << std::endl;
+    std::cout << "  Remark: " << pj.remark() << std::endl;
+    std::cout << "Run time: " << pj.calcTime() << " seconds\n";
+  }
   return 0;
 }

@@ -132,6 +132,11 @@ int main(int argc, char* const argv[]) {
     fprintf(stderr, "Unable to read projectfile file %s\n", pjName);
     return 1;
   }
+  if (verbose) {
+    std::ostringstream os;
+    proj.printScanInfo(os);
+    std::cout << os.str();
+  }
   ImageFile im(nx, ny);
   {
     Reconstructor rec(proj, im, filter.c_str(), filterParam, method.c_str(), zeropad,
@@ -143,8 +148,8 @@ int main(int argc, char* const argv[]) {
     rec.reconstructAllViews();
   }
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  im.labelAdd(proj.remark(), proj.calcTime());
-  im.labelAdd(label.str().c_str(), secs);
+  im.labelAdd(proj.getLabel());  // pjrec.cpp:355-356
+  im.labelAdd(Array2dFileLabel::L_HISTORY, label.str().c_str(), secs);
   if (!im.fileWrite(imName)) {
     fprintf(stderr, "Unable to write image file %s\n", imName);
     return 1;

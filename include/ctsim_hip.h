@@ -186,6 +186,14 @@ int ctsim_phantom_add_element(ctsim_phantom* phm, const char* type, double cx, d
 /* xmin(), xmax(), ymin(), ymax() (include/phantom.h:196-199) into ext4;
  * returns the number of elements. */
 int ctsim_phantom_extent(const ctsim_phantom* phm, double* ext4);
+/* Element `index` of Phantom::listPElem() (the reference's newest-first list
+ * order, phantom.cpp:247), as PhantomElement reports it (include/phantom.h:
+ * 41-128): type (1 rectangle, 2 triangle, 3 ellipse, 4 sector, 5 segment),
+ * cx, cy, u, v, rot (radians), atten, xmin, ymin, xmax, ymax (outline
+ * bounding box), nOutlinePoints, and the rotation in degrees as it was
+ * defined (13 doubles).  Phantom::print and printDefinitions
+ * (phantom.cpp:272-322) read it. */
+int ctsim_phantom_element(const ctsim_phantom* phm, int index, double* out13);
 
 /* Replaces Scanner::Scanner (libctsim/scanner.cpp:86-224; include/scanner.h:75-77),
  * rotLen in radians.  nSample < 1 is rejected (the reference divides by
