@@ -17,7 +17,7 @@ __version__ = "0.1"
 
 _API = {"Context", "Backprojector", "ProcessSignal", "Phantom", "Scanner", "CtsimError",
         "PREC_F64", "PREC_MIXED", "image_stats", "helical180li",
-        "halfscan_feather", "Batch", "SliceDesc", "Rebin", "check"}
+        "halfscan_feather", "Batch", "SliceDesc", "Rebin", "check", "crmath_eval"}
 
 
 def __getattr__(name):

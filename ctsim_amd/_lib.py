@@ -66,6 +66,7 @@ SIGNATURES = {
     "ctsim_ctx_profile_read": (_i, [_vp, _vp, _vp]),
     "ctsim_backprojector_autotune": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp]),
     "ctsim_backprojector_set_shape": (_i, [_vp, _vp]),
+    "ctsim_crmath_eval": (_i, [_vp, _i, _i, _vp, _vp, _vp]),
 }
 
 for _name, (_res, _args) in SIGNATURES.items():

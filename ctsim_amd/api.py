@@ -89,6 +89,20 @@ def image_stats(ctx: Context, ref, cmp) -> tuple[float, float, float]:
     return float(out[0]), float(out[1]), float(out[2])
 
 
+CRMATH_FN = {"cos": 0, "sin": 1, "atan": 2, "atan2": 3}
+
+
+def crmath_eval(ctx: Context, fn: str, x, y=None):
+    """The FP64 kernels' correctly rounded cos/sin/atan/atan2 evaluated on the
+    device over float64 tensors (atan2 takes y, x)."""
+    import torch
+    out = torch.empty_like(x)
+    check(lib.ctsim_crmath_eval(ctx.handle, CRMATH_FN[fn], x.numel(), C.c_void_p(_dptr(x)),
+                                C.c_void_p(_dptr(y)) if y is not None else None,
+                                C.c_void_p(_dptr(out))), "ctsim_crmath_eval")
+    return out
+
+
 def helical180li(ctx: Context, geometry, sino, rotInc: float, detInc: float, fanBeamAngle: float,
                  sourceDetectorLength: float, interpView: int = -1):
     """Projections::Helical180LI (libctsim/projections.cpp:200-358) of a device

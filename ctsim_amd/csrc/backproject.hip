@@ -21,6 +21,7 @@
 
 #include "bp_plan.h"
 #include "filter_plan.h"
+#include "crmath.h"
 #include "mixed_device.h"
 
 // FP64 parity kernels keep the reference's rounding: no contraction.
@@ -106,13 +107,13 @@ __device__ __forceinline__ double pos_f64(const BpArgs& a, const ViewConstF64& c
     const size_t k = static_cast<size_t>(ix) * a.ny + iy;
     const double r = a.r[k], phi = a.phi[k];
     if constexpr (KIND == KIND_TRIG || KIND == KIND_TABLE) {
-      return a.iDetCenter + (r * cos(c.theta - phi)) / a.detInc;
+      return a.iDetCenter + (r * crm_cos(c.theta - phi)) / a.detInc;
     } else {
       double s, co;
-      sincos(c.theta - phi, &s, &co);
+      crm_sincos(c.theta - phi, &s, &co);
       const double rc = r * co, rs = r * s;
       if constexpr (KIND == KIND_EQUIANGULAR) {
-        return a.iDetCenter + atan(rc / (a.F + rs)) / a.detInc;
+        return a.iDetCenter + crm_atan(rc / (a.F + rs)) / a.detInc;
       } else {
         const double U = (a.F + rs) / a.F;
         return a.iDetCenter + (rc / U) * (a.sdl / a.F) / a.detInc;
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(kBlock) void bp_f64_kernel(BpArgs a) {
         } else if constexpr (KIND == KIND_TRIG || KIND == KIND_TABLE) {
           // BackprojectTrig / BackprojectTable::BackprojectView,
           // backprojectors.cpp:365-404, 444-481
-          const double L = r * cos(c.theta - phi);
+          const double L = r * crm_cos(c.theta - phi);
           bool hit = false;
           double cval = 0;
           if constexpr (INTERP == INTERP_NEAREST) {
@@ -235,12 +236,12 @@ __global__ __launch_bounds__(kBlock) void bp_f64_kernel(BpArgs a) {
           // BackprojectEquiangular / BackprojectEquilinear::BackprojectView,
           // backprojectors.cpp:656-743
           double s, co;
-          sincos(c.theta - phi, &s, &co);
+          crm_sincos(c.theta - phi, &s, &co);
           const double rc = r * co, rs = r * s;
           double pos, wgt;
           if constexpr (KIND == KIND_EQUIANGULAR) {
             const double fl = a.F + rs;
-            const double gamma = atan(rc / fl);
+            const double gamma = crm_atan(rc / fl);
             pos = gamma / a.detInc;
             wgt = fl * fl + (rc * rc);
           } else {

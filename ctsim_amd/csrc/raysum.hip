@@ -20,6 +20,7 @@
 //               reference's clip.
 #include <hip/hip_runtime.h>
 
+#include "crmath.h"
 #include "scan_plan.h"
 
 #pragma clang fp contract(off)
@@ -130,8 +131,12 @@ __device__ bool clip_circle(double& x1, double& y1, double& x2, double& y2, doub
   double xc1 = x1, yc1 = y1, xc2 = x2, yc2 = y2, ccx = cx, ccy = cy;
   const double xtrans = -xc1, ytrans = -yc1;
   xc1 += xtrans; yc1 += ytrans; xc2 += xtrans; yc2 += ytrans; ccx += xtrans; ccy += ytrans;
-  const double theta = -atan2(yc2, xc2);
-  double c = cos(theta), s = sin(theta);
+  const double theta = -crm_atan2(yc2, xc2);
+  // correctly rounded cos and sin are exactly even and odd, so rotating back
+  // by -theta below reuses them as the reference's cos(-theta), sin(-theta)
+  double c, s;
+  crm_sincos(theta, &s, &c);
+  const double cb = c, sb = -s;
   {  // xform by rot(theta): x' = x c + y (-s) + 0, y' = x s + y c + 0
     double xt = xc2 * c + yc2 * -s + 0.0, yt = xc2 * s + yc2 * c + 0.0;
     xc2 = xt; yc2 = yt;
@@ -157,8 +162,8 @@ __device__ bool clip_circle(double& x1, double& y1, double& x2, double& y2, doub
       if (xc1 < xcmin) xc1 = xcmin;
     if (xc2 > xcmax) xc2 = xcmax;
   }
-  c = cos(-theta);
-  s = sin(-theta);
+  c = cb;
+  s = sb;
   {
     double xt = xc1 * c + yc1 * -s + 0.0, yt = xc1 * s + yc1 * c + 0.0;
     xc1 = xt; yc1 = yt;
@@ -368,8 +373,10 @@ __global__ __launch_bounds__(256) void raysum_kernel(RaysumArgs a) {
   double sum = 0.0;
   for (unsigned i = 0; i < a.nSample; i++) {
     if (a.geom == GEOM_EQUIANGULAR) {
-      xd = a.centerDetLength * cos(ang);
-      yd = a.centerDetLength * sin(ang);
+      double sa, ca;
+      crm_sincos(ang, &sa, &ca);
+      xd = a.centerDetLength * ca;
+      yd = a.centerDetLength * sa;
     }
     // projectSingleLine: the element list is newest-first (phantom.cpp:247)
     double rsum = 0.0;

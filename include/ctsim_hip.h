@@ -332,6 +332,16 @@ int ctsim_phantom_rasterize(ctsim_ctx* ctx, ctsim_phantom* phm, int nx, int ny, 
 int ctsim_image_stats(ctsim_ctx* ctx, const float* d_ref, const float* d_cmp, int nx, int ny,
                       double* h_dre);
 
+/* -------------------------------------------------- FP64 elementary functions */
+
+/* No counterpart in the reference, which calls the C library: evaluates the
+ * correctly rounded FP64 routines the FP64 kernels use in place of the C
+ * library's (fn 0 cos, 1 sin, 2 atan, 3 atan2(d_y[i], d_x[i])) on n device
+ * doubles into d_out, for checking the device build of the routines against
+ * their host build and a higher-precision reference. */
+int ctsim_crmath_eval(ctsim_ctx* ctx, int fn, int n, const double* d_x, const double* d_y,
+                      double* d_out);
+
 #ifdef __cplusplus
 }
 #endif

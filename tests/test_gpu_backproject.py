@@ -5,13 +5,12 @@ Golden images come from the reference compiled from source
 detector count, backprojected by every backprojector x interpolation.
 
 Bars
-  F64, IntDiff and Diff     bit-exact (integer indexing, host-tabulated
-                            per-view constants, no FMA contraction)
-  F64, Trig/Table/fan       per-pixel trig runs on the device; the image may
-                            differ from the reference where device and glibc
-                            cos/sin/atan round differently and flip a float32
-                            rounding: at most 1% of pixels, each within
-                            4 float32 ulps of max|image|
+  F64, every family         bit-exact (integer indexing, host-tabulated
+                            per-view constants, no FMA contraction; Trig,
+                            Table and the fan families evaluate cos, sin and
+                            atan per pixel with correctly rounded device
+                            routines, which agree with the C library wherever
+                            the latter is correctly rounded)
   MIXED                     Herman normalised distance d <= 1e-4 against the
                             FP64 reference image (nearest interpolation and
                             Diff take their detector index in FP64 in the
@@ -22,7 +21,7 @@ import itertools
 import numpy as np
 import pytest
 
-from conftest import load_golden
+from conftest import load_golden, record_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -74,15 +73,8 @@ def test_f64_matches_reference(gpu_ctx, geom, nDet, bp, interp):
     g, b = _setup(gpu_ctx, geom, nDet, bp, interp)
     out = _run(gpu_ctx, b, g, ctsim_amd.PREC_F64)
     ref = g[f"image_{bp}_{interp}"]
-    exact = geom == "parallel" and bp in ("idiff", "diff")
-    diff = out.view(np.uint32) != ref.view(np.uint32)
-    print(f"  pixels differing from the reference: {int(diff.sum())}")
-    if exact:
-        assert diff.sum() == 0, f"{diff.sum()} pixels differ"
-    else:
-        ulp = np.spacing(np.float32(np.abs(ref).max()))
-        assert diff.mean() <= 0.01, f"{diff.sum()} pixels differ"
-        assert np.abs(out - ref).max() <= 4 * ulp
+    n = record_parity(f"golden backproject {geom} {nDet} {bp} {interp}", out, ref)
+    assert n == 0, f"{n} pixels differ"
 
 
 @pytest.mark.parametrize("geom,nDet,bp,interp", list(_cases()))
@@ -103,18 +95,14 @@ def test_f64_unscaled_and_roi(gpu_ctx):
         g, b = _setup(gpu_ctx, geom, nDet, "idiff", "linear")
         out = _run(gpu_ctx, b, g, ctsim_amd.PREC_F64, post=False)
         ref = g["image_idiff_linear_nopost"]
-        if geom == "parallel":
-            assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+        assert record_parity(f"golden unscaled {geom} {nDet}", out, ref) == 0
         # golden roi is (xmin, xmax, ymin, ymax); the ABI takes ReconstructionROI
         # member order (xMin, yMin, xMax, yMax)
         x0, x1, y0, y1 = g["roi"]
         g2, b2 = _setup(gpu_ctx, geom, nDet, "idiff", "cubic", 40, 36, roi=[x0, y0, x1, y1])
         out2 = _run(gpu_ctx, b2, g2, ctsim_amd.PREC_F64, 40, 36)
         ref2 = g["image_idiff_cubic_roi"]
-        if geom == "parallel":
-            assert np.array_equal(out2.view(np.uint32), ref2.view(np.uint32))
-        else:
-            assert np.abs(out2 - ref2).max() <= 4 * np.spacing(np.float32(np.abs(ref2).max()))
+        assert record_parity(f"golden roi {geom} {nDet}", out2, ref2) == 0
 
 
 def test_view_chunks_accumulate_in_order(gpu_ctx):

@@ -18,7 +18,7 @@ import itertools
 import numpy as np
 import pytest
 
-from conftest import load_golden
+from conftest import load_golden, record_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -146,10 +146,7 @@ def test_fourier_hanning_cubic_end_to_end(gpu_ctx, geom, nDet):
                              "cubic", 48, 48, ctsim_amd.PREC_F64)
     assert np.array_equal(filt.view(np.uint64), g["filtered_fourier_hanning"].view(np.uint64))
     ref = g["image_fourier_hanning_cubic"]
-    if geom == "parallel":
-        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
-    else:
-        assert np.abs(img - ref).max() <= 4 * np.spacing(np.float32(np.abs(ref).max()))
+    assert record_parity(f"golden fourier hanning cubic {geom} {nDet}", img, ref) == 0
 
 
 @pytest.mark.parametrize("n", [33, 130, 367, 1024, 2048])

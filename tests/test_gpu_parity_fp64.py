@@ -1,0 +1,79 @@
+"""FP64 parity beyond the golden grids: ray sums and every backprojector
+family of the FP64 GPU path against the oracle (a plain-C restatement of the
+reference, checked bit for bit against the reference itself in
+tests/test_oracle.py), on a 367/368-detector, 360-view scan of a phantom with
+every element type, reconstructed at 128 x 128.  Each case is bit-exact;
+every comparison is recorded (tests/conftest.py:record_parity) with its
+count of differing values.
+
+The trig-based families (Trig, Table, Equilinear, Equiangular) evaluate cos,
+sin and atan per pixel-view update and the ray sums atan2, cos and sin per
+ray-element hit; these cases run 5.9 M updates and 0.2-0.4 M hits each.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import record_parity
+from ctsim_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+NX = 128
+SCANS = [("parallel", 367, np.pi), ("parallel", 368, np.pi), ("equilinear", 367, 2 * np.pi),
+         ("equilinear", 368, 2 * np.pi), ("equiangular", 367, 2 * np.pi),
+         ("equiangular", 368, 2 * np.pi)]
+
+
+@pytest.fixture(scope="module")
+def scans(oracle_lib):
+    O = oracle_lib
+    phm = O.Phantom(None, W.mixed_elements(31) + W.seeded_elements(24, 32))
+    out = {}
+    for geom, nDet, rot in SCANS:
+        scan = O.ScanSpec(geom, nDet, 360, 0, 1, rot)
+        out[(geom, nDet)] = (phm, scan, O.pipeline("oracle", phm, scan, O.ReconSpec(nx=NX, ny=NX)))
+    return out
+
+
+@pytest.mark.parametrize("geom,nDet", [(g, n) for g, n, _ in SCANS])
+def test_raysums_match_oracle(gpu_ctx, scans, geom, nDet):
+    import torch
+    import ctsim_amd
+    print(f"case parity raysum {geom} {nDet}", flush=True)
+    phm_o, scan, ref = scans[(geom, nDet)]
+    phm = ctsim_amd.Phantom(None, phm_o.elems)
+    sc = ctsim_amd.Scanner(phm, geom, nDet, scan.nView, 0, 1, scan.rotLen)
+    sino = torch.zeros((scan.nView, nDet), dtype=torch.float32, device="cuda")
+    ang = sc.collect(gpu_ctx, sino, ctsim_amd.PREC_F64)
+    torch.cuda.synchronize()
+    assert np.array_equal(ang.view(np.uint64), ref["angles"].view(np.uint64))
+    assert record_parity(f"oracle raysum {geom} {nDet}x360", sino.cpu().numpy(), ref["sino"]) == 0
+
+
+def _families(geom):
+    # fan geometries select their own backprojector whatever is asked
+    return ["trig", "table", "diff", "idiff"] if geom == "parallel" else ["idiff"]
+
+
+@pytest.mark.parametrize("geom,nDet,bp,interp",
+                         [(g, n, b, i) for g, n, _ in SCANS for b in _families(g)
+                          for i in ("nearest", "linear", "cubic")])
+def test_backprojection_matches_oracle(gpu_ctx, scans, oracle_lib, geom, nDet, bp, interp):
+    import torch
+    import ctsim_amd
+    print(f"case parity backproject {geom} {nDet} {bp} {interp}", flush=True)
+    O = oracle_lib
+    phm_o, scan, ref = scans[(geom, nDet)]
+    want = O.backproject("oracle", phm_o, scan, bp, interp, NX, NX, ref["filtered"], ref["angles"])
+    info = ref["info"]
+    b = ctsim_amd.Backprojector(gpu_ctx, geom, nDet, float(info[0]), scan.nView, float(info[3]),
+                                float(info[4]), float(info[5]), bp, interp, NX, NX)
+    f = torch.from_numpy(ref["filtered"]).to("cuda", torch.float64)
+    img = torch.zeros((NX, NX), dtype=torch.float32, device="cuda")
+    b.backproject(f, ref["angles"], img, ctsim_amd.PREC_F64)
+    b.postprocess(img)
+    torch.cuda.synchronize()
+    case = f"oracle backproject {geom} {nDet}x360 {bp} {interp} {NX}^2"
+    assert record_parity(case, img.cpu().numpy(), want) == 0
