@@ -663,10 +663,12 @@ int bp_launch(ctsim_ctx* ctx, const BpPlan& p, int precision, const void* d_filt
 // nView), P <= 2 and windows small enough that one base view's 4*S windows
 // fit the LDS it stages into.
 bool bp_sym_ok(const BpPlan& p, const MixedShape& m, int nView) {
-  const SymViews& sv = p.cachedSym;
-  if (!p.symImage || sv.S == 0 || m.P > 2 || m.W <= 0 || static_cast<int>(p.cachedAngles.size()) != nView ||
-      p.cachedKind != 1)
-    return false;
+  if (static_cast<int>(p.cachedAngles.size()) != nView || p.cachedKind != 1) return false;
+  return bp_sym_ok_for(p, m, p.cachedSym);
+}
+
+bool bp_sym_ok_for(const BpPlan& p, const MixedShape& m, const SymViews& sv) {
+  if (!p.symImage || sv.S == 0 || m.P > 2 || m.W <= 0) return false;
   const size_t elem = p.interp == INTERP_CUBIC ? sizeof(float4) : sizeof(float2);
   if (m.sym == 2) return m.W <= dev::kPipeRow;  // the pipelined body stages rows of kPipeRow cells, two per lane
   return static_cast<size_t>(4 * sv.S) * m.W * elem <= sym_lds(m);
@@ -694,23 +696,28 @@ int bp_launch_mixed(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const 
 
 // Interval coefficients of nView filtered float rows into ctx->scratch2.
 int bp_mixed_coef(ctsim_ctx* ctx, const BpPlan& p, const float* d_filtered, int nView) {
+  const size_t elem = p.interp == INTERP_CUBIC ? sizeof(float4) : sizeof(float2);
+  CTSIM_CHECK(ctx->scratch2.reserve(static_cast<size_t>(nView) * p.nDet * elem));
+  return bp_mixed_coef_into(ctx, p, d_filtered, nView, ctx->scratch2.ptr);
+}
+
+int bp_mixed_coef_into(ctsim_ctx* ctx, const BpPlan& p, const float* d_filtered, int nView,
+                       void* d_coef) {
   const hipStream_t s = ctx->stream;
   const bool cubic = p.interp == INTERP_CUBIC;
   const bool lastIncl = p.kind == KIND_IDIFF;
-  const size_t elem = cubic ? sizeof(float4) : sizeof(float2);
-  CTSIM_CHECK(ctx->scratch2.reserve(static_cast<size_t>(nView) * p.nDet * elem));
   const int perRow = (p.nDet + 255) / 256;
   const int cgrid = nView * perRow;
   const int fstride = p.nDet * p.interpFactor;
   if (cubic)
     hipLaunchKernelGGL((coef_kernel<INTERP_CUBIC, false>), dim3(cgrid), dim3(256), 0, s, d_filtered,
-                       fstride, p.nDet, nView, perRow, ctx->scratch2.ptr);
+                       fstride, p.nDet, nView, perRow, d_coef);
   else if (lastIncl)
     hipLaunchKernelGGL((coef_kernel<INTERP_LINEAR, true>), dim3(cgrid), dim3(256), 0, s, d_filtered,
-                       fstride, p.nDet, nView, perRow, ctx->scratch2.ptr);
+                       fstride, p.nDet, nView, perRow, d_coef);
   else
     hipLaunchKernelGGL((coef_kernel<INTERP_LINEAR, false>), dim3(cgrid), dim3(256), 0, s, d_filtered,
-                       fstride, p.nDet, nView, perRow, ctx->scratch2.ptr);
+                       fstride, p.nDet, nView, perRow, d_coef);
   CTSIM_HIP(hipGetLastError());
   return CTSIM_OK;
 }
@@ -720,7 +727,7 @@ int bp_mixed_coef(ctsim_ctx* ctx, const BpPlan& p, const float* d_filtered, int 
 // when overwrite.
 int bp_mixed_tiles(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const void* coef,
                    const ViewConstF32* d_vc, int nView, float* d_image, double endScale,
-                   bool overwrite) {
+                   bool overwrite, const SymViews* views, float* partial) {
   const hipStream_t s = ctx->stream;
   const bool cubic = p.interp == INTERP_CUBIC;
   const size_t elem = cubic ? sizeof(float4) : sizeof(float2);
@@ -751,8 +758,8 @@ int bp_mixed_tiles(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const v
   a.coef = coef;
   a.image = d_image;
   const size_t npix = static_cast<size_t>(p.nx) * p.ny;
-  const SymViews sv = p.cachedSym;
-  const bool sym = m.sym && bp_sym_ok(p, m, nView);
+  const SymViews sv = views ? *views : p.cachedSym;
+  const bool sym = m.sym && (views ? bp_sym_ok_for(p, m, sv) : bp_sym_ok(p, m, nView));
   if (sym) {
     // tiles of the quadrant; groups and chunks of base views
     a.half = p.nx / 2;
@@ -768,8 +775,12 @@ int bp_mixed_tiles(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const v
   const size_t lds = sym ? static_cast<size_t>(a.capW) * elem : static_cast<size_t>(a.chunk) * a.W * elem;
   if (lds > kMixedLdsMax) return fail(CTSIM_ERR_UNSUPPORTED, "backprojector: detector row too long for LDS");
   if (a.split > 1) {
-    CTSIM_CHECK(ctx->scratch.reserve(static_cast<size_t>(a.split) * npix * sizeof(float)));
-    a.partial = ctx->scratch.as<float>();
+    if (partial) {
+      a.partial = partial;
+    } else {
+      CTSIM_CHECK(ctx->scratch.reserve(static_cast<size_t>(a.split) * npix * sizeof(float)));
+      a.partial = ctx->scratch.as<float>();
+    }
   }
   const int grid = a.tilesX * a.tilesY * a.split;
   ctx->prof_begin();

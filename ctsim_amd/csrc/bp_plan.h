@@ -100,15 +100,22 @@ int bp_launch_post(ctsim_ctx* ctx, const BpPlan& p, float* d_image);
 int bp_launch_mixed(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const float* d_filtered,
                     const ViewConstF32* d_vc, int nView, float* d_image);
 int bp_mixed_coef(ctsim_ctx* ctx, const BpPlan& p, const float* d_filtered, int nView);
+// The same into a caller buffer of nView * nDet coefficient records.
+int bp_mixed_coef_into(ctsim_ctx* ctx, const BpPlan& p, const float* d_filtered, int nView,
+                       void* d_coef);
+// views: the symmetry of these angles (NULL: the plan's cached one);
+// partial: view-group partial images, split * nx * ny floats (NULL: the
+// context's scratch).
 int bp_mixed_tiles(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const void* coef,
                    const ViewConstF32* d_vc, int nView, float* d_image, double endScale,
-                   bool overwrite);
+                   bool overwrite, const SymViews* views = nullptr, float* partial = nullptr);
 int bp_view_consts_mixed(ctsim_ctx* ctx, const BpPlan& p, const double* h_angles, int nView,
                          const ViewConstF32** out);
 // True when the mixed precision of this plan runs the tiled coefficient
 // kernel (not the FP64 index path) and one view's staging fits LDS.
 bool bp_mixed_tiled(const BpPlan& p);
 bool bp_sym_ok(const BpPlan& p, const MixedShape& m, int nView);
+bool bp_sym_ok_for(const BpPlan& p, const MixedShape& m, const SymViews& sv);
 size_t sym_lds(const MixedShape& m);  // LDS bytes the symmetric kernel stages into
 
 }  // namespace ctsim

@@ -124,6 +124,9 @@ int ctsim_ctx_create(ctsim_ctx** out, int device, void* stream) {
   c->device = device;
   int rc = c->activate();
   if (rc) { delete c; return rc; }
+  // the device address of the no-allocation entry point's static scratch;
+  // a failure here only disables that entry point, which then says so
+  (void)static_scratch_setup(device);
   // NULL is HIP's null stream, so work is ordered with other null-stream work
   c->stream = static_cast<hipStream_t>(stream);
   hipDeviceProp_t prop;

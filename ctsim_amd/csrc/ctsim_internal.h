@@ -49,6 +49,10 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
     if (rc_ != CTSIM_OK) return rc_; \
   } while (0)
 
+// Takes the device address of the static scratch array of
+// ctsim_fbp_slice_static on `device` (the current device), once per device.
+int static_scratch_setup(int device);
+
 // ------------------------------------------------------------ device buffer
 // A growable device allocation owned by a context or a plan.
 struct DevBuf {

@@ -125,6 +125,25 @@ int ctsim_fbp_slice(ctsim_ctx* ctx, ctsim_procsignal* ps, const ctsim_backprojec
                     int precision, const float* d_sino, const double* h_angles, int nView,
                     float* d_image);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ctsim_fbp_slice at CTSIM_PREC_MIXED with every scratch
+ * buffer (per-view constants, interval coefficients, filtered rows of
+ * convolution filters, view-group partial images) in one static device array
+ * of CTSIM_STATIC_SCRATCH_BYTES per device, whose address is taken when a
+ * context is created on the device.  Calls no hipMalloc/hipFree or other
+ * allocating routine; the caller passes the inputs and the output.  Same
+ * result as ctsim_fbp_slice(..., CTSIM_PREC_MIXED, ...), bit for bit.
+ * Needs the tiled mixed path (linear or cubic interpolation, no
+ * preinterpolation) and at most 16384 views; a slice whose scratch exceeds
+ * the array is rejected with CTSIM_ERR_INVALID before any launch.  Every
+ * call on one device shares that device's array: calls that overlap on a
+ * device race. */
+#define CTSIM_STATIC_SCRATCH_BYTES (64u << 20)
+int ctsim_fbp_slice_static(ctsim_ctx* ctx, ctsim_procsignal* ps, const ctsim_backprojector* bp,
+                           const float* d_sino, const double* h_angles, int nView,
+                           float* d_image);
+
 /* Geometry the backprojector derived: xMin, xMax, yMin, yMax, xInc, yInc,
  * rotScale, iDetCenter (8 doubles). */
 int ctsim_backprojector_info(const ctsim_backprojector* bp, double* out8);
