@@ -24,6 +24,7 @@ import os
 import platform
 import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -502,18 +503,15 @@ def bench_config5(ctx, args):
     fan, rotInc = probe.info["fanBeamAngle"], 2 * np.pi / h.viewsPerRotation
     L = int((np.pi + fan) / rotInc)
     nTotal = 2 * L + 1 + (h.positions - 1) * h.stride() + 2
-    sc = ctsim_amd.Scanner(phm0, h.geometry, h.nDet, nTotal, 0, 1, rotInc * nTotal, h.focalRatio,
-                           h.centerRatio)
-    info = sc.info
     log(f"[bench] config 5: {nTotal} views ({nTotal / h.viewsPerRotation:.2f} rotations), "
         f"{h.positions} positions, L={L}")
-    sino = torch.empty((nTotal, h.nDet), dtype=torch.float32, device="cuda")
     t0 = time.perf_counter()
-    for v in range(nTotal):  # one phantom per view, as phm2helix.cpp:299-317
-        phm = ctsim_amd.Phantom(None, h.elements(v, base))
-        sc.collect(ctx, sino, ctsim_amd.PREC_F64, iStartView=v, iNumViews=1, phantom=phm)
+    # the scan, one phantom per view, by the phm2helix drop-in (C++)
+    with tempfile.TemporaryDirectory() as wd:
+        sino, _, info = fbp.helical_scan(ctx, h, wd, nTotal, ctsim_amd.PREC_F64)
     torch.cuda.synchronize()
-    log(f"[bench] time-varying scan simulated in {time.perf_counter() - t0:.1f} s")
+    log(f"[bench] time-varying scan simulated by phm2helix in {time.perf_counter() - t0:.1f} s")
+    fan = info["fanBeamAngle"]
     spec = W.SliceSpec(geometry=h.geometry, nDet=h.nDet, nView=L + 1, filter=h.filter,
                        method=h.method, interp=h.interp, backproj=h.backproj, nx=h.nx, ny=h.ny)
     ivs = [L + 1 + p * h.stride() for p in range(h.positions)]

@@ -424,7 +424,10 @@ Scanner::Scanner(const Phantom& phm, const char* geometryName, int nDet, int nVi
   m_geometry = static_cast<int>(m_info[11]);
 }
 
-Scanner::~Scanner() { ctsim_scanner_destroy(m_hip); }
+Scanner::~Scanner() {
+  if (m_dSino) (void)hipFree(m_dSino);
+  ctsim_scanner_destroy(m_hip);
+}
 
 void Scanner::setNView(int nView) { m_nView = nView; }
 void Scanner::setOffsetView(int iOffsetView) { m_offsetView = iOffsetView; }
@@ -450,11 +453,20 @@ void Scanner::collectProjections(Projections& proj, const Phantom& phm, int iSta
     return;
   }
   ctsim_ctx* ctx = ctsim_amd::context();
-  float* d = nullptr;
-  if (hipMalloc(&d, sizeof(float) * static_cast<size_t>(rows) * m_nDet) != hipSuccess) {
-    warn("collectProjections: device allocation failed");
-    return;
+  // one device buffer of the projections' rows, kept for later calls (a
+  // time-varying scan collects one view per call)
+  if (m_dRows < static_cast<size_t>(rows)) {
+    if (m_dSino) (void)hipFree(m_dSino);
+    m_dSino = nullptr;
+    m_dRows = 0;
+    if (hipMalloc(&m_dSino, sizeof(float) * static_cast<size_t>(rows) * m_nDet) != hipSuccess) {
+      m_dSino = nullptr;
+      warn("collectProjections: device allocation failed");
+      return;
+    }
+    m_dRows = rows;
   }
+  float* d = m_dSino;
   std::vector<double> ang(rows);
   const int rc = ctsim_collect_projections(ctx, m_hip, phm.handle(), ctsim_amd::precision(),
                                            iStartView, iNumViews, iOffsetView, iStorageOffset, d,
@@ -471,7 +483,6 @@ void Scanner::collectProjections(Projections& proj, const Phantom& phm, int iSta
   } else {
     warn(ctsim_last_error());
   }
-  (void)hipFree(d);
 }
 
 // ---------------------------------------------------------- Projections

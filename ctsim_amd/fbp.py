@@ -4,10 +4,16 @@ sinogram on the GPU, reconstruct it on the FP64 or mixed single-slice path,
 or describe it for a batch.  All compute runs in libctsim_amd.so."""
 from __future__ import annotations
 
+import os
+import subprocess
+
 import numpy as np
 
 from . import api
-from .workloads import SliceSpec
+from .workloads import HelicalSpec, SliceSpec
+
+BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "bin")
+TWO_PI = 6.28318530717958647692  # the reference's TWOPI (include/ctsupport.h)
 
 
 def simulate(ctx: api.Context, spec: SliceSpec, precision: int = api.PREC_F64):
@@ -74,3 +80,48 @@ def slice_desc(spec: SliceSpec, info: dict, angles) -> dict:
             "generation": spec.generation, "zeropad": spec.zeropad, "backproj": spec.backproj,
             "interp": spec.interp, "nx": spec.nx, "ny": spec.ny,
             "angles": np.asarray(angles, np.float64)}
+
+
+def read_pj(path: str):
+    """A .pj projection file's views and view angles (the layout of
+    Projections::write, projections.cpp:485-546, 816-889): (float32
+    [nView, nDet], float64 [nView])."""
+    b = open(path, "rb").read()
+    hsize = int(np.frombuffer(b, "<u2", 1, 0)[0])
+    nView, nDet = (int(x) for x in np.frombuffer(b, "<u4", 2, 4))
+    data = np.zeros((nView, nDet), np.float32)
+    ang = np.zeros(nView, np.float64)
+    off = hsize
+    for v in range(nView):
+        ang[v] = np.frombuffer(b, "<f8", 1, off)[0]
+        data[v] = np.frombuffer(b, "<f4", nDet, off + 12)
+        off += 12 + 4 * nDet
+    return data, ang
+
+
+def helical_scan(ctx: api.Context, h: HelicalSpec, workdir: str, nTotal: int,
+                 precision: int = api.PREC_F64):
+    """The time-varying scan of ``h`` over ``nTotal`` views by the phm2helix
+    tool (one phantom per view from the helixphm program, as
+    tools/phm2helix.cpp:287-315 runs dynphm); returns (device sinogram
+    float32 [nTotal, nDet], angles, info of the Scanner the tool builds)."""
+    import torch
+    base = os.path.join(workdir, "helix_base.txt")
+    h.write_program_base(base)
+    pj = os.path.join(workdir, "helix.pj")
+    turns = repr(nTotal / h.viewsPerRotation)
+    prog = (f"{os.path.join(BIN, 'helixphm')} --base {base} --period {h.viewsPerRotation} "
+            f"--drift {h.drift!r}")
+    subprocess.run([os.path.join(BIN, "phm2helix"), pj, str(h.nDet), str(nTotal), prog,
+                    "--geometry", h.geometry, "--rotangle", turns, "--focal-length",
+                    repr(h.focalRatio), "--center-detector-length", repr(h.centerRatio),
+                    "--phmfile", os.path.join(workdir, "helix_phm.txt"), "--precision",
+                    "f64" if precision == api.PREC_F64 else "mixed", "--device", str(ctx.device)],
+                   check=True, capture_output=True, timeout=600)
+    data, ang = read_pj(pj)
+    # the tool's Scanner: the view-0 phantom, rotation turns * TWOPI
+    phm0 = api.Phantom(None, h.elements(0, h.base()))
+    sc = api.Scanner(phm0, h.geometry, h.nDet, nTotal, 0, 1, float(turns) * TWO_PI,
+                     h.focalRatio, h.centerRatio)
+    sino = torch.from_numpy(data).to(f"cuda:{ctx.device}")
+    return sino, ang, sc.info

@@ -8,6 +8,7 @@ in degrees).
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -155,16 +156,27 @@ class HelicalSpec:
 
     def elements(self, view: int, base=None):
         """The phantom at acquisition view ``view`` (one rotation per period):
-        centre and semi-axes drift sinusoidally by ``drift``."""
+        centre and semi-axes drift sinusoidally by ``drift``.  The C
+        library's sin (math.sin), as the phantom program helixphm computes it."""
         elems, phases = base if base is not None else self.base()
         t = 2 * np.pi * view / self.viewsPerRotation
         out = []
         for (kind, cx, cy, u, v, rot, att), ph in zip(elems, phases):
             a = self.drift
-            out.append((kind, cx + a * np.sin(t + ph[0]), cy + a * np.sin(t + ph[1]),
-                        max(0.01, u + a * np.sin(t + ph[2])), max(0.01, v + a * np.sin(t + ph[3])),
-                        rot, att))
+            out.append((kind, cx + a * math.sin(t + ph[0]), cy + a * math.sin(t + ph[1]),
+                        max(0.01, u + a * math.sin(t + ph[2])),
+                        max(0.01, v + a * math.sin(t + ph[3])), rot, att))
         return out
+
+    def write_program_base(self, path: str) -> None:
+        """The input of the phantom program helixphm (ctsim_amd/tools/
+        helixphm.cpp): one element per line, the phantom file's fields then
+        the four drift phases, with 17 significant digits."""
+        elems, phases = self.base()
+        with open(path, "w") as f:
+            for (kind, cx, cy, u, v, rot, att), ph in zip(elems, phases):
+                vals = (cx, cy, u, v, rot, att, *ph)
+                f.write(kind + " " + " ".join(f"{float(x):.17g}" for x in vals) + "\n")
 
     def stride(self) -> int:
         return self.viewsPerRotation // self.positions

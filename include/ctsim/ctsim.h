@@ -335,6 +335,8 @@ class Scanner {
   std::string m_failMessage;
   int m_nDet = 0, m_nView = 0, m_offsetView = 0, m_geometry = -1;
   double m_info[12] = {};
+  float* m_dSino = nullptr;  // device rows, kept across collectProjections calls
+  size_t m_dRows = 0;
 };
 
 class ImageFile {
