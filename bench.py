@@ -71,6 +71,31 @@ def measured_traffic():
     return traffic, shape
 
 
+COUNTER_SUMMARY = os.path.join(ROOT, "profiles", "bp_sym_pipe_equiangular_cubic_512_counters.md")
+
+
+def measured_issue():
+    """Executed-instruction figures of the pipelined symmetric kernel on this
+    workload from the committed rocprofv3 counter summary: VALU issue
+    utilisation (wave VALU instructions x 4 cycles over all SIMD cycles) and
+    lane instructions per pixel-view update; None where absent."""
+    out = {"valu_issue_utilisation": None, "lane_instructions_per_pvu": None,
+           "source": os.path.relpath(COUNTER_SUMMARY, ROOT)}
+    try:
+        with open(COUNTER_SUMMARY) as fh:
+            for line in fh:
+                cells = [c.strip().strip("*") for c in line.split("|")]
+                if len(cells) < 3:
+                    continue
+                if cells[1].startswith("VALU issue utilisation"):
+                    out["valu_issue_utilisation"] = float(cells[2])
+                elif cells[1].startswith("lane instructions per pixel-view update"):
+                    out["lane_instructions_per_pvu"] = float(cells[2])
+    except (OSError, ValueError):
+        pass
+    return out
+
+
 def cpu_model():
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
@@ -84,7 +109,11 @@ def cpu_model():
 
 def cpu_baseline(spec, sino_h, ang_h, seconds):
     """The reference's own CPU path (compiled from source into oracle/_ref when
-    present, else the C restatement) on the first views of the same sinogram."""
+    present, else the C restatement) on the first views of the same sinogram:
+    one thread (Reconstructor::reconstructView view by view), then all host
+    cores as the reference's GUI reconstructs (src/backgroundsupr.cpp:111-152,
+    src/threadrecon.cpp:175-260: a Reconstructor per thread over contiguous
+    view blocks, the partial images summed in float)."""
     import numpy as np
     import oracle as O
     kind = "reference" if O.have_ref() else "port"
@@ -98,23 +127,40 @@ def cpu_baseline(spec, sino_h, ang_h, seconds):
                 backproj=spec.backproj, nx=spec.nx, ny=spec.ny, sino=sino_h, angles=ang_h)
     _, tf1, tb1 = O.fbp_sample(lib_kind, phm, nUse=1, **args)
     per_view = max(tf1 + tb1, 1e-6)
-    n = int(max(1, min(spec.nView, seconds / per_view)))
+    n = int(max(1, min(spec.nView, 0.5 * seconds / per_view)))
     _, tf, tb = O.fbp_sample(lib_kind, phm, nUse=n, **args)
     per_view_f, per_view_b = tf / n, tb / n
-    slices_per_s = 1.0 / ((per_view_f + per_view_b) * spec.nView)
-    return {
-        "value": slices_per_s,
+    one_thread = 1.0 / ((per_view_f + per_view_b) * spec.nView)
+    out = {
+        "value": one_thread,
         "unit": "slices/s",
         "cores": 1,
         "kind": kind,
-        "sample": (f"first {n} of {spec.nView} views of the same sinogram, filter "
-                   f"'fourier' (direct DFT; the reference's fft needs FFTW) + equiangular "
-                   f"cubic backprojection, single thread, scaled to one slice"),
-        "filter_s_per_view": per_view_f,
-        "backproject_s_per_view": per_view_b,
-        "backproject_only_slices_per_s": 1.0 / (per_view_b * spec.nView),
+        "sample": (f"first {n} of {spec.nView} views of the same sinogram, single thread, "
+                   f"scaled to one slice"),
+        "filter_method": ("fourier: the reference's fft needs FFTW, absent here, so the CPU "
+                          "path filters by its direct O(N^2) DFT of the same filter "
+                          "(procsignal.cpp:160-168, 565-581); the filter then dominates the "
+                          "CPU time, backproject_only_slices_per_s isolates the comparable stage"),
+        "single_thread": {"slices_per_s": one_thread, "views": n,
+                          "filter_s_per_view": per_view_f,
+                          "backproject_s_per_view": per_view_b,
+                          "backproject_only_slices_per_s": 1.0 / (per_view_b * spec.nView)},
         "cpu_model": cpu_model(),
     }
+    if kind == "reference":
+        threads = max(1, min(16, os.cpu_count() or 1))
+        nt = int(max(threads, min(spec.nView, 0.5 * seconds * threads * 0.8 / per_view)))
+        _, wall = O.fbp_threads(phm, nUse=nt, nThreads=threads,
+                                **{k: v for k, v in args.items()})
+        all_cores = nt / wall / spec.nView
+        out.update(value=all_cores, cores=threads,
+                   sample=(f"first {nt} of {spec.nView} views of the same sinogram on {threads} "
+                           f"threads (contiguous view blocks, a Reconstructor per thread, float "
+                           f"sum of the partial images), wall time scaled to one slice"),
+                   all_cores={"slices_per_s": all_cores, "threads": threads, "views": nt,
+                              "wall_s": wall})
+    return out
 
 
 def main():
@@ -318,7 +364,9 @@ def main():
     pvu = spec.nx * spec.ny * spec.nView
     flops = FLOPS_PER_PVU_EQUIANGULAR_CUBIC_KAHAN * pvu
     achieved = flops / (bp_ms * 1e-3) / 1e12
-    hbm_bytes = 4 * spec.nView * spec.nDet + 2 * 4 * spec.nx * spec.ny
+    # algorithmic HBM bytes of one slice: the float sinogram read once and the
+    # float image written once
+    hbm_bytes = 4 * spec.nView * spec.nDet + 4 * spec.nx * spec.ny
     traffic, traffic_shape = measured_traffic()
 
     out = {
@@ -356,6 +404,7 @@ def main():
                      "flops_counting": "algorithmic: one position, weight and cubic interpolation "
                                        "per update; the symmetric kernels evaluate a position "
                                        "once per 4 updates and issue fewer instructions",
+                     "executed": measured_issue() if shape[5] == 2 else None,
                      "pvu_per_launch": pvu,
                      "hbm_algorithmic_bytes": hbm_bytes,
                      "hbm_frac": hbm_bytes / (bp_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
@@ -368,7 +417,7 @@ def main():
         "batched": batch,
     }
     if not args.no_cpu:
-        log(f"[bench] CPU baseline (~{args.cpu_seconds:.0f} s of single-thread work) ...")
+        log(f"[bench] CPU baseline (~{args.cpu_seconds:.0f} s: one thread, then all cores) ...")
         sino_h = sino.cpu().numpy()
         out["cpu_baseline"] = cpu_baseline(spec, sino_h, ang, args.cpu_seconds)
         log(f"[bench] CPU: {out['cpu_baseline']['value']:.3e} slices/s")

@@ -302,6 +302,31 @@ def fbp_sample(kind: str, phm: Phantom, geometry: str, nDet: int, nView: int, nS
     return img, tim[0], tim[1]
 
 
+def fbp_threads(phm: Phantom, geometry: str, nDet: int, nView: int, nSample: int, rotLen: float,
+                focalRatio: float, centerRatio: float, filter: str, method: str, zeropad: int,
+                interp: str, backproj: str, nx: int, ny: int, sino: np.ndarray,
+                angles: np.ndarray, nUse: int, nThreads: int):
+    """The reference's all-cores reconstruction (per-thread Reconstructor over
+    contiguous view blocks of views [0, nUse), float sum of the images);
+    returns (image, wall seconds)."""
+    L = lib("ref")
+    f = L.ref_fbp_threads
+    c, i, d, vp = C.c_char_p, C.c_int, C.c_double, C.c_void_p
+    f.restype = i
+    f.argtypes = [vp, vp, i, c, c, i, i, i, d, d, d, c, c, i, c, c, i, i, vp, vp, i, i, vp, vp]
+    t, p, n = phm.arrays()
+    img = np.zeros((nx, ny), np.float32)
+    sec = np.zeros(1, np.float64)
+    sino = np.ascontiguousarray(sino, np.float32)
+    angles = np.ascontiguousarray(angles, np.float64)
+    rc = f(_ptr(t), _ptr(p), n, _b(phm.builtin), _b(geometry), nDet, nView, nSample, rotLen,
+           focalRatio, centerRatio, _b(filter), _b(method), zeropad, _b(interp), _b(backproj), nx,
+           ny, _ptr(sino), _ptr(angles), nUse, nThreads, _ptr(img), _ptr(sec))
+    if rc:
+        raise RuntimeError(f"ref fbp_threads failed ({rc}): {L.ref_last_error().decode()}")
+    return img, float(sec[0])
+
+
 def ref_write_pj(path: str, builtin: str, geometry: str, nDet: int, nView: int, nSample: int,
                  rotLen: float, focalRatio: float, centerRatio: float, data: np.ndarray,
                  angles: np.ndarray, calcTime: float, remark: str) -> None:

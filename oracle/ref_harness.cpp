@@ -17,6 +17,8 @@
 //   ref_stats         ~ ImageFile::comparativeStatistics (imagefile.cpp:185-240)
 #include "ct.h"
 #include <chrono>
+#include <thread>
+#include <vector>
 
 static std::string g_err;
 
@@ -228,6 +230,66 @@ int ref_fbp_sample(const int* types, const double* params, int nElem, const char
   copyImage(im, image);
   timings[0] = tf;
   timings[1] = tb;
+  return 0;
+}
+
+// All host cores as the reference's GUI reconstructs
+// (src/backgroundsupr.cpp:111-152, src/threadrecon.cpp:175-260): nThreads
+// workers over contiguous blocks of views [0, nUse) (nUse/nThreads each,
+// the first nUse%nThreads one more), each with its own Reconstructor over
+// the whole Projections writing its own image (reconstructView one view at a
+// time, then postProcessing), the images then summed in float.  seconds[0]
+// is the wall time of the workers and the sum.
+int ref_fbp_threads(const int* types, const double* params, int nElem, const char* builtin,
+                    const char* geometry, int nDet, int nView, int nSample, double rotLen,
+                    double focalRatio, double centerRatio, const char* filterName,
+                    const char* methodName, int zeropad, const char* interpName,
+                    const char* backprojName, int nx, int ny, const float* sino,
+                    const double* angles, int nUse, int nThreads, float* image,
+                    double* seconds) {
+  Phantom phm;
+  buildPhantom(phm, types, params, nElem, builtin);
+  Scanner scanner(phm, geometry, nDet, nView, 0, nSample, rotLen, focalRatio, centerRatio, 1.0, 1.0);
+  if (scanner.fail()) { g_err = scanner.failMessage(); return 2; }
+  if (nThreads < 1 || nUse < 1 || nUse > nView) { g_err = "bad thread or view count"; return 3; }
+  Projections pj(scanner);
+  for (int v = 0; v < nView; v++) {
+    DetectorArray& d = pj.getDetectorArray(v);
+    d.setViewAngle(angles[v]);
+    for (int i = 0; i < nDet; i++) d.detValues()[i] = sino[(size_t)v * nDet + i];
+  }
+  std::vector<ImageFile*> child(nThreads);
+  for (int t = 0; t < nThreads; t++) child[t] = new ImageFile(nx, ny);
+  std::vector<int> fails(nThreads, 0);
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<std::thread> workers;
+  const int base = nUse / nThreads, extra = nUse % nThreads;
+  int start = 0;
+  for (int t = 0; t < nThreads; t++) {
+    const int count = base + (t < extra ? 1 : 0);
+    workers.emplace_back([&, t, start, count]() {
+      Reconstructor rec(pj, *child[t], filterName, -1.0, methodName, zeropad, "direct", interpName,
+                        1, backprojName, Trace::TRACE_NONE, NULL, false);
+      if (rec.fail()) { fails[t] = 1; return; }
+      for (int u = 0; u < count; u++) rec.reconstructView(start + u, 1);
+      rec.postProcessing();
+    });
+    start += count;
+  }
+  for (std::thread& w : workers) w.join();
+  ImageFile sum(nx, ny);
+  sum.arrayDataClear();
+  ImageFileArray a = sum.getArray();
+  for (int t = 0; t < nThreads; t++) {
+    ImageFileArrayConst c = child[t]->getArray();
+    for (int ix = 0; ix < nx; ix++)
+      for (int iy = 0; iy < ny; iy++) a[ix][iy] += c[ix][iy];
+  }
+  seconds[0] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int t = 0; t < nThreads; t++) delete child[t];
+  for (int f : fails)
+    if (f) { g_err = "Reconstructor failed"; return 4; }
+  copyImage(sum, image);
   return 0;
 }
 

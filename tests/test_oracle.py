@@ -531,3 +531,29 @@ def test_raster_disc_area(oracle_lib):
     side = 2 * r * 1.0  # view diameter sqrt2 * 1.2, half length = 0.6 -> image spans [-0.6, 0.6]
     area = img.sum() * (side / 200) ** 2
     assert abs(area - np.pi * r * r) < 2e-3
+
+
+def test_reference_all_cores_mode():
+    """The bench's all-cores CPU baseline (oracle/ref_harness.cpp:
+    ref_fbp_threads, the reference GUI's per-thread Reconstructor over view
+    blocks): with one thread it is the sequential reference reconstruction
+    bit for bit; with three, the float sum of the partial images stays within
+    summation noise of it."""
+    import oracle as O
+    if not O.have_ref():
+        pytest.skip("reference library not built")
+    from ctsim_amd import workloads as W
+    phm = O.Phantom(None, W.seeded_elements(12, 77))
+    scan = O.ScanSpec("equiangular", 96, 60, 0, 1, 2 * np.pi)
+    r = O.pipeline("ref", phm, scan)
+    args = dict(geometry="equiangular", nDet=96, nView=60, nSample=1, rotLen=2 * np.pi,
+                focalRatio=2.0, centerRatio=2.0, filter="abs_hanning", method="fourier",
+                zeropad=1, interp="cubic", backproj="idiff", nx=48, ny=48, sino=r["sino"],
+                angles=r["angles"])
+    seq, _, _ = O.fbp_sample("ref", phm, nUse=60, **args)
+    one, _ = O.fbp_threads(phm, nUse=60, nThreads=1, **args)
+    assert np.array_equal(one.view(np.uint32), seq.view(np.uint32))
+    three, _ = O.fbp_threads(phm, nUse=60, nThreads=3, **args)
+    s = seq.astype(np.float64)
+    d = np.sqrt(((three - s) ** 2).sum() / ((s - s.mean()) ** 2).sum())
+    assert 0 < d < 1e-6
