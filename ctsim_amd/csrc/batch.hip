@@ -70,10 +70,12 @@ struct Item {
   int kind, slice, a, pad;
 };
 
-// Zero the queue heads and per-slice counters (a kernel, not a memset, so the
-// graph holds no memset node).
-__global__ void batch_reset_kernel(int* counters, int n) {
+// Zero the queue heads, the per-slice counters and the error word (a
+// kernel, not a memset, so the graph holds no memset node): every run
+// starts clean and reports only its own timeouts.
+__global__ void batch_reset_kernel(int* counters, int n, int* error) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) counters[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *error = 0;
 }
 
 // The view groups of the symmetric slices added after the backprojection
@@ -552,8 +554,16 @@ int occupancy_grid(const void* kernel, size_t lds, int numCUs) {
 }
 
 // Build the per-slice device descriptors and the item queues for the
-// current shape (P, lanesY, chunk budget).
+// current shape (P, lanesY, chunk budget).  Until it succeeds the batch is
+// marked dirty and holds no graph: a failed layout (a shape that does not
+// fit, a failed allocation) leaves nothing to replay, and run and status
+// refuse the batch until a shape is laid out.
 int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
+  b.shapeDirty = true;
+  if (b.graph) {
+    (void)hipGraphExecDestroy(b.graph);
+    b.graph = nullptr;
+  }
   const int S = static_cast<int>(b.s.size());
   b.host.assign(S, SliceDev{});
   size_t maxBpLds = 0;
@@ -604,6 +614,9 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
     d.P = m.P;
     const int Wv = m.W > 0 ? m.W : p.nDet + 4;
     const size_t elem = p.interp == INTERP_CUBIC ? 16 : 8;
+    // the plain body stages chunks of whole view windows within the budget
+    if (!d.sym && static_cast<size_t>(Wv) * elem > static_cast<size_t>(b.chunkBudget))
+      return fail(CTSIM_ERR_UNSUPPORTED, "batch: a view's detector window exceeds the chunk budget");
     int chunk = static_cast<int>(b.chunkBudget / (Wv * elem));
     chunk = chunk < 1 ? 1 : (chunk > dev::kMaxChunk ? dev::kMaxChunk : chunk);
     m.chunk = chunk;
@@ -830,10 +843,6 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
   for (auto& bc : b.bclass) bc.grid = occupancy_grid(bc.kernel, bc.lds, ctx->numCUs);
   b.gridAll = occupancy_grid(kAll, b.ldsAll, ctx->numCUs);
   b.shapeDirty = false;
-  if (b.graph) {
-    (void)hipGraphExecDestroy(b.graph);
-    b.graph = nullptr;
-  }
   return CTSIM_OK;
 }
 
@@ -841,7 +850,7 @@ template <int P, bool SYM>
 void launch_batch(ctsim_batch& b, hipStream_t st) {
   const int S = static_cast<int>(b.s.size());
   hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, st, b.d_counters.as<int>(),
-                     static_cast<int>(batch_counters(b)));
+                     static_cast<int>(batch_counters(b)), b.d_error.as<int>());
   if (b.fused) {
     hipLaunchKernelGGL((fbp_batch_kernel<P, SYM, true, true>), dim3(b.gridAll), dim3(256), b.ldsAll, st,
                        b.d_slices.as<SliceDev>(), b.d_itemsAll.as<Item>(), b.d_qAll.as<int>(),
@@ -997,6 +1006,11 @@ int ctsim_batch_create(ctsim_ctx* ctx, ctsim_batch** out, const ctsim_slice_desc
       rc = fail(CTSIM_ERR_HIP, "batch: stream or event creation failed");
   }
   if (rc == CTSIM_OK) rc = batch_layout(ctx, *b);
+  if (rc == CTSIM_ERR_UNSUPPORTED && b->chunkBudget < 62 * 1024) {
+    // a detector window wider than the default chunk budget: the largest
+    b->chunkBudget = 62 * 1024;
+    rc = batch_layout(ctx, *b);
+  }
   if (rc != CTSIM_OK) {
     delete b;
     return rc;
@@ -1033,6 +1047,8 @@ int ctsim_batch_run(ctsim_ctx* ctx, ctsim_batch* b, const float* const* d_sinos,
   for (int k = 0; k < S; k++)
     if (!d_sinos[k] || !d_images[k]) return fail(CTSIM_ERR_INVALID, "ctsim_batch_run: NULL slice pointer");
   CTSIM_CHECK(ctx->activate());
+  if (b->shapeDirty)
+    return fail(CTSIM_ERR_INVALID, "ctsim_batch_run: the batch's last layout failed; set a supported shape");
   const bool same = static_cast<int>(b->sinos.size()) == S &&
                     std::equal(b->sinos.begin(), b->sinos.end(), d_sinos) &&
                     std::equal(b->images.begin(), b->images.end(), d_images);
@@ -1052,6 +1068,8 @@ int ctsim_batch_run(ctsim_ctx* ctx, ctsim_batch* b, const float* const* d_sinos,
 int ctsim_batch_status(ctsim_ctx* ctx, ctsim_batch* b) {
   if (!ctx || !b) return fail(CTSIM_ERR_INVALID, "ctsim_batch_status: NULL argument");
   CTSIM_CHECK(ctx->activate());
+  if (b->shapeDirty)
+    return fail(CTSIM_ERR_INVALID, "ctsim_batch_status: the batch's last layout failed; set a supported shape");
   CTSIM_HIP(hipStreamSynchronize(b->stream));
   int flag = 0;
   CTSIM_HIP(hipMemcpy(&flag, b->d_error.ptr, sizeof(int), hipMemcpyDeviceToHost));
@@ -1082,22 +1100,27 @@ int ctsim_batch_autotune(ctsim_ctx* ctx, ctsim_batch* b, const float* const* d_s
             if (rc == CTSIM_ERR_UNSUPPORTED) { rc = CTSIM_OK; continue; }
             break;
           }
+          // a warm-up run, then two timed runs; a shape whose runs time out
+          // waiting for filter items (the status check) is skipped
           float tot = 0;
-          for (int rep = 0; rep < 3 && rc == CTSIM_OK; rep++) {
+          bool timedOut = false;
+          for (int rep = 0; rep < 3 && rc == CTSIM_OK && !timedOut; rep++) {
+            if (rep > 0) (void)hipEventRecord(e0, ctx->stream);
             if ((rc = ctsim_batch_run(ctx, b, d_sinos, d_images)) != CTSIM_OK) break;
-            if (rep == 0) {
-              if ((rc = ctsim_batch_status(ctx, b)) != CTSIM_OK) break;
-              continue;
+            if (rep > 0) {
+              (void)hipEventRecord(e1, ctx->stream);
+              (void)hipEventSynchronize(e1);
             }
-            (void)hipEventRecord(e0, ctx->stream);
-            if ((rc = ctsim_batch_run(ctx, b, d_sinos, d_images)) != CTSIM_OK) break;
-            (void)hipEventRecord(e1, ctx->stream);
-            (void)hipEventSynchronize(e1);
+            if (ctsim_batch_status(ctx, b) != CTSIM_OK) {
+              timedOut = true;
+              break;
+            }
+            if (rep == 0) continue;
             float ms = 0;
             (void)hipEventElapsedTime(&ms, e0, e1);
             tot += ms;
           }
-          if (rc == CTSIM_OK && tot / 2 < best) {
+          if (rc == CTSIM_OK && !timedOut && tot / 2 < best) {
             best = tot / 2;
             bP = P; bL = lanesY; bC = chunk; bF = fused; bS = sym;
           }

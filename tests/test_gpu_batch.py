@@ -180,3 +180,26 @@ def test_batch_one_plain_class(gpu_ctx, P):
         d = ctsim_amd.image_stats(gpu_ctx, r64, img)[0]
         print(f"  {s.nDet}x{s.nView} {s.nx}^2: d={d:.2e}", flush=True)
         assert d <= 1e-5
+
+
+def test_failed_layout_leaves_nothing_to_run(gpu_ctx, batch_case):
+    """A shape whose layout fails (here a 1 KiB chunk budget, narrower than
+    the view window of the 401-detector 128^2 cubic slice) leaves the batch
+    with nothing to replay: run and status report the failed layout and no
+    kernel touches the images; a supported shape then runs normally."""
+    import torch
+    import ctsim_amd
+    specs, b, sinos, images, ref64, refmx = batch_case
+    with pytest.raises(ctsim_amd.CtsimError, match="exceeds the chunk budget"):
+        b.set_shape(2, 16, 1024, False, 0)
+    imgs = [torch.full_like(i, 5.0) for i in images]
+    with pytest.raises(ctsim_amd.CtsimError, match="last layout failed"):
+        b.run(sinos, imgs)
+    with pytest.raises(ctsim_amd.CtsimError, match="last layout failed"):
+        b.status()
+    torch.cuda.synchronize()
+    assert all(float(i.min()) == 5.0 and float(i.max()) == 5.0 for i in imgs)
+    b.set_shape(2, 16, 32768, True, 1)
+    b.run(sinos, imgs)
+    b.status()
+    _check(gpu_ctx, specs, imgs, ref64, refmx)
