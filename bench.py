@@ -50,7 +50,8 @@ TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "bp_sym_pipe_equiangular_cubic_
 # symmetric, 2 pipelined symmetric)
 KERNEL_BY_SYM = {0: "bp_mixed_kernel<equiangular, cubic>",
                  1: "bp_sym_kernel<equiangular, cubic, 4 quarter turns>",
-                 2: "bp_sym_pipe_kernel<equiangular, cubic, 4 quarter turns>"}
+                 2: "bp_sym_pipe_kernel<equiangular, cubic, 4 quarter turns>",
+                 3: "bp_mirror_kernel<equiangular, cubic, mirror members>"}
 
 
 def measured_traffic():

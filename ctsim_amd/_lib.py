@@ -33,6 +33,7 @@ SIGNATURES = {
     "ctsim_fbp_slice": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _i, _vp]),
     "ctsim_fbp_slice_static": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "ctsim_backprojector_info": (_i, [_vp, _vp]),
+    "ctsim_backprojector_mirror_members": (_i, [_vp, _vp, _i]),
     "ctsim_procsignal_create": (_i, [_vp, _pp, _cp, _cp, _d, _d, _i, _d, _cp, _cp, _i, _i, _i,
                                      _d, _d]),
     "ctsim_procsignal_destroy": (None, [_vp]),

@@ -238,7 +238,8 @@ class Batch:
         check(lib.ctsim_batch_status(self.ctx.handle, self.handle), "ctsim_batch_status")
 
     def set_shape(self, P: int, lanesY: int, chunkBytes: int, fused: bool, sym: int = 1):
-        """sym: 0 plain body; 1, 2, 4 symmetric body with that many view groups."""
+        """sym: 0 plain body; 1, 2, 4 symmetric body with that many view groups;
+        + 8 the mirror body for the other slices (two-phase mode)."""
         check(lib.ctsim_batch_set_shape(self.ctx.handle, self.handle, P, lanesY, chunkBytes,
                                         int(fused), int(sym)), "ctsim_batch_set_shape")
 
@@ -289,6 +290,14 @@ class Backprojector:
         check(lib.ctsim_backprojector_info(self.handle, _hptr(o)), "ctsim_backprojector_info")
         return o
 
+    def mirror_members(self, angles: np.ndarray) -> int:
+        """Mirror members (4, 2 or 1) of the mirror kernel for these angles."""
+        a = np.ascontiguousarray(angles, np.float64)
+        n = lib.ctsim_backprojector_mirror_members(self.handle, _hptr(a), len(a))
+        if n < 1:
+            check(n, "ctsim_backprojector_mirror_members")
+        return n
+
     def backproject(self, filtered, angles: np.ndarray, image, precision: int = PREC_F64):
         """Add all rows of ``filtered`` (device [nView, nDet]; float64 for F64,
         float32 for MIXED) into ``image`` (device float32 [nx, ny])."""
@@ -330,7 +339,8 @@ class Backprojector:
                   stagingKB: int = 0):
         """Pins the MIXED launch shape; sym = 1 runs the rotationally symmetric
         kernel when the image and the views allow it (P <= 2), staging its
-        windows in stagingKB KB of LDS (0: 62)."""
+        windows in stagingKB KB of LDS (0: 62); 2 its pipelined form; 3 the
+        mirror kernel when the views have mirror partners."""
         s = np.array([lanesY, P, split, chunk, sym, stagingKB], np.int32)
         check(lib.ctsim_backprojector_set_shape(self.handle, _hptr(s)), "ctsim_backprojector_set_shape")
 

@@ -397,6 +397,24 @@ __global__ __launch_bounds__(256) void bp_mixed_kernel(MixedArgs a) {
                                          s_lo);
 }
 
+// The mirror variant: one workgroup per (tile of the mirror domain, view
+// group), dealt to XCDs as above.
+template <int GK, int INTERP, int P, int DEG, int NM>
+__global__ __launch_bounds__(256) void bp_mirror_kernel(MixedArgs a) {
+  const int nTiles = a.tilesX * a.tilesY;
+  int tile, grp;
+  if ((a.split & 7) == 0) {
+    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+    grp = xcd * (a.split >> 3) + local / nTiles;
+    tile = local % nTiles;
+  } else {
+    const int bid = dev::xcd_tile(blockIdx.x, gridDim.x);
+    tile = bid % nTiles;
+    grp = bid / nTiles;
+  }
+  dev::bp_mirror_tile<GK, INTERP, P, DEG, NM>(a, tile, grp);
+}
+
 // The rotationally symmetric variant: one workgroup per (quadrant tile, group
 // of base views), dealt to XCDs as above.
 template <int GK, int INTERP, int P, int DEG, int S, int K>
@@ -520,6 +538,33 @@ void launch_mixed(int gk, int deg, int P, const MixedArgs& a, int grid, size_t l
     case 6: launch_mixed_p<1, INTERP, 6>(P, a, grid, lds, s); break;
     case 8: launch_mixed_p<1, INTERP, 8>(P, a, grid, lds, s); break;
     default: launch_mixed_p<1, INTERP, 0>(P, a, grid, lds, s); break;
+  }
+}
+
+template <int GK, int INTERP, int DEG, int NM>
+void launch_mirror_p(int P, const MixedArgs& a, int grid, size_t lds, hipStream_t s) {
+  switch (P) {
+    case 1: hipLaunchKernelGGL((bp_mirror_kernel<GK, INTERP, 1, DEG, NM>), dim3(grid), dim3(256), lds, s, a); break;
+    case 2: hipLaunchKernelGGL((bp_mirror_kernel<GK, INTERP, 2, DEG, NM>), dim3(grid), dim3(256), lds, s, a); break;
+    default: hipLaunchKernelGGL((bp_mirror_kernel<GK, INTERP, 4, DEG, NM>), dim3(grid), dim3(256), lds, s, a); break;
+  }
+}
+
+template <int GK, int INTERP, int DEG>
+void launch_mirror_nm(int NM, int P, const MixedArgs& a, int grid, size_t lds, hipStream_t s) {
+  if (NM == 4) launch_mirror_p<GK, INTERP, DEG, 4>(P, a, grid, lds, s);
+  else launch_mirror_p<GK, INTERP, DEG, 2>(P, a, grid, lds, s);
+}
+
+template <int INTERP>
+void launch_mirror(int gk, int deg, int NM, int P, const MixedArgs& a, int grid, size_t lds, hipStream_t s) {
+  if (gk == 0) return launch_mirror_nm<0, INTERP, 0>(NM, P, a, grid, lds, s);
+  if (gk == 2) return launch_mirror_nm<2, INTERP, 0>(NM, P, a, grid, lds, s);
+  switch (deg) {
+    case 4: launch_mirror_nm<1, INTERP, 4>(NM, P, a, grid, lds, s); break;
+    case 6: launch_mirror_nm<1, INTERP, 6>(NM, P, a, grid, lds, s); break;
+    case 8: launch_mirror_nm<1, INTERP, 8>(NM, P, a, grid, lds, s); break;
+    default: launch_mirror_nm<1, INTERP, 0>(NM, P, a, grid, lds, s); break;
   }
 }
 
@@ -668,10 +713,24 @@ bool bp_sym_ok(const BpPlan& p, const MixedShape& m, int nView) {
 }
 
 bool bp_sym_ok_for(const BpPlan& p, const MixedShape& m, const SymViews& sv) {
-  if (!p.symImage || sv.S == 0 || m.P > 2 || m.W <= 0) return false;
+  if (m.sym == 3 || !p.symImage || sv.S == 0 || m.P > 2 || m.W <= 0) return false;
   const size_t elem = p.interp == INTERP_CUBIC ? sizeof(float4) : sizeof(float2);
   if (m.sym == 2) return m.W <= dev::kPipeRow;  // the pipelined body stages rows of kPipeRow cells, two per lane
   return static_cast<size_t>(4 * sv.S) * m.W * elem <= sym_lds(m);
+}
+
+int bp_mirror_chunk(const BpPlan& p, const MixedShape& m, int NM, int nView) {
+  const bool cubic = p.interp == INTERP_CUBIC;
+  const int unit = cubic ? 16 : 4, upc = cubic ? 1 : 2;
+  const int Wv = m.W > 0 ? m.W : p.nDet + 4;
+  // one buffer resource spans every coefficient row
+  if (static_cast<double>(nView) * p.nDet * unit * upc >= 2147483648.0) return 0;
+  // a power of two up to 32 (a 64-view block of the kernel then takes an
+  // even number of chunks)
+  int c = 1;
+  while (2 * c <= m.chunk && 2 * c <= 32) c *= 2;
+  while (c > 0 && dev::mirror_units(c, NM, Wv, upc) * unit > dev::kMirBufBytes) c /= 2;
+  return c;
 }
 
 size_t sym_lds(const MixedShape& m) {
@@ -760,6 +819,20 @@ int bp_mixed_tiles(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const v
   const size_t npix = static_cast<size_t>(p.nx) * p.ny;
   const SymViews sv = views ? *views : p.cachedSym;
   const bool sym = m.sym && (views ? bp_sym_ok_for(p, m, sv) : bp_sym_ok(p, m, nView));
+  // the mirror body: the mirror partners of these views (the cached ones
+  // must be of these nView angles)
+  const bool known = views || (static_cast<int>(p.cachedAngles.size()) == nView && p.cachedKind == 1);
+  const int mirNM = m.sym == 3 && known && sv.NM >= 2 && bp_mirror_chunk(p, m, sv.NM, nView) > 0 ? sv.NM : 1;
+  a.domX = p.nx;
+  a.domY = p.ny;
+  if (mirNM > 1) {
+    a.domX = (p.nx + 1) / 2;
+    a.domY = mirNM == 4 ? (p.ny + 1) / 2 : p.ny;
+    a.mapO = sv.mapO;
+    a.tilesX = (a.domX + tx - 1) / tx;
+    a.tilesY = (a.domY + ty - 1) / ty;
+    a.chunk = bp_mirror_chunk(p, m, mirNM, nView);
+  }
   if (sym) {
     // tiles of the quadrant; groups and chunks of base views
     a.half = p.nx / 2;
@@ -772,7 +845,9 @@ int bp_mixed_tiles(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const v
     a.split = (a.Q + a.viewsPerGroup - 1) / a.viewsPerGroup;
     a.capW = m.sym == 2 ? 4 * sv.S * dev::kPipeRow : static_cast<int>(sym_lds(m) / elem);
   }
-  const size_t lds = sym ? static_cast<size_t>(a.capW) * elem : static_cast<size_t>(a.chunk) * a.W * elem;
+  // (the mirror kernel stages into static LDS buffers of its own)
+  const size_t lds = sym ? static_cast<size_t>(a.capW) * elem
+                         : (mirNM > 1 ? 0 : static_cast<size_t>(a.chunk) * a.W * elem);
   if (lds > kMixedLdsMax) return fail(CTSIM_ERR_UNSUPPORTED, "backprojector: detector row too long for LDS");
   if (a.split > 1) {
     if (partial) {
@@ -788,6 +863,9 @@ int bp_mixed_tiles(ctsim_ctx* ctx, const BpPlan& p, const MixedShape& m, const v
     const bool pipe = m.sym == 2;
     if (cubic) launch_sym<INTERP_CUBIC>(pipe, gk, deg, sv.S, sv.K, P, a, grid, lds, s);
     else launch_sym<INTERP_LINEAR>(pipe, gk, deg, sv.S, sv.K, P, a, grid, lds, s);
+  } else if (mirNM > 1) {
+    if (cubic) launch_mirror<INTERP_CUBIC>(gk, deg, mirNM, P, a, grid, lds, s);
+    else launch_mirror<INTERP_LINEAR>(gk, deg, mirNM, P, a, grid, lds, s);
   } else if (cubic) {
     launch_mixed<INTERP_CUBIC>(gk, deg, P, a, grid, lds, s);
   } else {
@@ -811,8 +889,10 @@ int bp_view_consts_mixed(ctsim_ctx* ctx, const BpPlan& p, const double* h_angles
   CTSIM_CHECK(view_consts<ViewConstF32>(
       ctx, p, 1, h_angles, nView,
       [&](ViewConstF32* o) {
-        bp_view_consts_f32(p, h_angles, nView, o);
+        const MirrorViews mv = bp_view_consts_f32(p, h_angles, nView, o);
         p.cachedSym = bp_sym_views(h_angles, nView);
+        p.cachedSym.NM = mv.NM;
+        p.cachedSym.mapO = mv.mapO;
       },
       &vc));
   *out = static_cast<const ViewConstF32*>(vc);
@@ -843,7 +923,8 @@ int bp_autotune(ctsim_ctx* ctx, BpPlan& p, const float* d_filtered, const double
   // candidates: the plain kernel over lanesY x P x split x chunk, then the
   // symmetric kernel (when the views and the image allow it) over
   // lanesY x P x split (of the base views) x base views per chunk
-  for (int sym = 0; sym <= 2 && rc == CTSIM_OK; sym++) {
+  for (int sym = 0; sym <= 3 && rc == CTSIM_OK; sym++) {
+    if (sym == 3 && p.cachedSym.NM < 2) continue;  // no mirror partners
     for (int lanesY : {8, 16, 32}) {
       for (int P : {1, 2, 4}) {
         if (lanesY * P > 2 * p.ny && P > 1) continue;
@@ -852,18 +933,26 @@ int bp_autotune(ctsim_ctx* ctx, BpPlan& p, const float* d_filtered, const double
         const int Wv = W > 0 ? W : p.nDet + 4;
         MixedShape m;
         m.lanesY = lanesY; m.P = P; m.W = W; m.sym = sym;
-        if (sym && !bp_sym_ok(p, m, nView)) continue;  // not even in 62 KB
-        const int views = sym ? nView / p.cachedSym.S : nView;
-        for (int split : {1, 2, 4, 8, 16}) {
-          if (split > 1 && views / split < (sym ? 4 : 8)) continue;
-          for (int chunk : {2, 4, 8, 16, 32}) {
-            if (sym ? chunk > 16 : (chunk < 4 || static_cast<size_t>(chunk) * Wv * elem > kMixedLdsMax))
+        const bool rot = sym == 1 || sym == 2;
+        if (rot && !bp_sym_ok(p, m, nView)) continue;  // not even in 62 KB
+        const int views = rot ? nView / p.cachedSym.S : nView;
+        const int nm = sym == 3 ? p.cachedSym.NM : 1;
+        for (int split : {1, 2, 4, 8, 16, 32}) {
+          if (split > 1 && views / split < (rot ? 4 : 8)) continue;
+          if (split == 32 && sym != 3) continue;  // the mirror kernel's quarter domain has few tiles
+          for (int chunk : {1, 2, 3, 4, 8, 16, 32}) {
+            if (sym == 3) {  // chunks that fit the mirror kernel's staging buffer
+              m.chunk = chunk;
+              if (chunk == 3 || chunk > 16 || bp_mirror_chunk(p, m, nm, nView) != chunk) continue;
+            } else if (chunk == 1 || chunk == 3 ||
+                       (rot ? chunk > 16 : (chunk < 4 || static_cast<size_t>(chunk) * Wv * elem > kMixedLdsMax))) {
               continue;
+            }
            for (int symKB : {16, 24, 32, 48, 62}) {
             if (sym != 1 && symKB != 62) continue;  // only the chunked body stages into symKB
             if (sym == 2 && chunk != 2) continue;   // the pipelined body has no chunk
             m.symKB = symKB;
-            if (sym && !bp_sym_ok(p, m, nView)) continue;
+            if (rot && !bp_sym_ok(p, m, nView)) continue;
             m.split = split; m.chunk = chunk;
             float ms = 0, tot = 0;
             for (int rep = 0; rep < 3 && rc == CTSIM_OK; rep++) {
@@ -963,7 +1052,7 @@ int ctsim_backprojector_autotune(ctsim_ctx* ctx, ctsim_backprojector* bp, const 
   if (shape7) {
     const MixedShape& m = bp->plan.mixed;
     shape7[0] = m.lanesY; shape7[1] = m.P; shape7[2] = m.split; shape7[3] = m.chunk; shape7[4] = m.W;
-    shape7[5] = m.sym; shape7[6] = m.sym ? static_cast<int>(sym_lds(m) / 1024) : 0;
+    shape7[5] = m.sym; shape7[6] = m.sym == 1 || m.sym == 2 ? static_cast<int>(sym_lds(m) / 1024) : 0;
   }
   return CTSIM_OK;
 }
@@ -973,7 +1062,7 @@ int ctsim_backprojector_set_shape(ctsim_backprojector* bp, const int* shape6) {
   const int lanesY = shape6[0], P = shape6[1], split = shape6[2], chunk = shape6[3], sym = shape6[4],
             symKB = shape6[5];
   if (!(lanesY == 8 || lanesY == 16 || lanesY == 32 || lanesY == 64) || !(P == 1 || P == 2 || P == 4) ||
-      split < 1 || split > 64 || chunk < 1 || chunk > kMaxChunk || sym < 0 || sym > 2 || symKB < 0 ||
+      split < 1 || split > 64 || chunk < 1 || chunk > kMaxChunk || sym < 0 || sym > 3 || symKB < 0 ||
       symKB > 62)
     return fail(CTSIM_ERR_INVALID, "ctsim_backprojector_set_shape: shape out of range");
   MixedShape m;
@@ -1034,6 +1123,13 @@ int ctsim_fbp_slice(ctsim_ctx* ctx, ctsim_procsignal* ps, const ctsim_backprojec
   const ViewConstF32* vc = nullptr;
   CTSIM_CHECK(bp_view_consts_mixed(ctx, p, h_angles, nView, &vc));
   return bp_mixed_tiles(ctx, p, p.mixed, ctx->scratch2.ptr, vc, nView, d_image, p.rotScale, true);
+}
+
+int ctsim_backprojector_mirror_members(const ctsim_backprojector* bp, const double* h_angles, int nView) {
+  if (!bp || !h_angles || nView < 1)
+    return fail(CTSIM_ERR_INVALID, "ctsim_backprojector_mirror_members: NULL argument or no views");
+  std::vector<ViewConstF32> vc(nView);
+  return bp_view_consts_f32(bp->plan, h_angles, nView, vc.data()).NM;
 }
 
 int ctsim_backprojector_info(const ctsim_backprojector* bp, double* o) {

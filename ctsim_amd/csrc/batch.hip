@@ -63,6 +63,7 @@ struct SliceDev {
   int gk, interp, lastIncl, deg;
   int sym, symS, symK;  // rotationally symmetric body (tiles of the quadrant)
   int P;                // pixels per lane of this slice's tiles
+  int mirNM;            // plain slices: 2 or 4 runs the mirror body (two-phase mode), 1 the plain one
   MixedArgs bp;  // bp.coef, bp.image, bp.vc set; overwrite = 1, split = 1
 };
 
@@ -114,6 +115,45 @@ __device__ __forceinline__ void bp_item(const MixedArgs& a, int gk, int interp, 
     else if (deg == 6) bp_mixed_tile<1, INTERP_LINEAR, P, 6>(a, tile, 0, w, s_v, s_lo);
     else if (deg == 8) bp_mixed_tile<1, INTERP_LINEAR, P, 8>(a, tile, 0, w, s_v, s_lo);
     else bp_mixed_tile<1, INTERP_LINEAR, P, 0>(a, tile, 0, w, s_v, s_lo);
+  }
+}
+
+// The mirror body of a plain slice whose views have mirror partners.
+template <int P>
+__device__ __forceinline__ void bp_mirror_item(const MixedArgs& a, int gk, int interp, int deg, int NM, int tile) {
+  using dev::bp_mirror_tile;
+  if (interp == INTERP_CUBIC) {
+    if (NM == 4) {
+      if (gk == 0) bp_mirror_tile<0, INTERP_CUBIC, P, 0, 4>(a, tile, 0);
+      else if (gk == 2) bp_mirror_tile<2, INTERP_CUBIC, P, 0, 4>(a, tile, 0);
+      else if (deg == 4) bp_mirror_tile<1, INTERP_CUBIC, P, 4, 4>(a, tile, 0);
+      else if (deg == 6) bp_mirror_tile<1, INTERP_CUBIC, P, 6, 4>(a, tile, 0);
+      else if (deg == 8) bp_mirror_tile<1, INTERP_CUBIC, P, 8, 4>(a, tile, 0);
+      else bp_mirror_tile<1, INTERP_CUBIC, P, 0, 4>(a, tile, 0);
+    } else {
+      if (gk == 0) bp_mirror_tile<0, INTERP_CUBIC, P, 0, 2>(a, tile, 0);
+      else if (gk == 2) bp_mirror_tile<2, INTERP_CUBIC, P, 0, 2>(a, tile, 0);
+      else if (deg == 4) bp_mirror_tile<1, INTERP_CUBIC, P, 4, 2>(a, tile, 0);
+      else if (deg == 6) bp_mirror_tile<1, INTERP_CUBIC, P, 6, 2>(a, tile, 0);
+      else if (deg == 8) bp_mirror_tile<1, INTERP_CUBIC, P, 8, 2>(a, tile, 0);
+      else bp_mirror_tile<1, INTERP_CUBIC, P, 0, 2>(a, tile, 0);
+    }
+  } else {
+    if (NM == 4) {
+      if (gk == 0) bp_mirror_tile<0, INTERP_LINEAR, P, 0, 4>(a, tile, 0);
+      else if (gk == 2) bp_mirror_tile<2, INTERP_LINEAR, P, 0, 4>(a, tile, 0);
+      else if (deg == 4) bp_mirror_tile<1, INTERP_LINEAR, P, 4, 4>(a, tile, 0);
+      else if (deg == 6) bp_mirror_tile<1, INTERP_LINEAR, P, 6, 4>(a, tile, 0);
+      else if (deg == 8) bp_mirror_tile<1, INTERP_LINEAR, P, 8, 4>(a, tile, 0);
+      else bp_mirror_tile<1, INTERP_LINEAR, P, 0, 4>(a, tile, 0);
+    } else {
+      if (gk == 0) bp_mirror_tile<0, INTERP_LINEAR, P, 0, 2>(a, tile, 0);
+      else if (gk == 2) bp_mirror_tile<2, INTERP_LINEAR, P, 0, 2>(a, tile, 0);
+      else if (deg == 4) bp_mirror_tile<1, INTERP_LINEAR, P, 4, 2>(a, tile, 0);
+      else if (deg == 6) bp_mirror_tile<1, INTERP_LINEAR, P, 6, 2>(a, tile, 0);
+      else if (deg == 8) bp_mirror_tile<1, INTERP_LINEAR, P, 8, 2>(a, tile, 0);
+      else bp_mirror_tile<1, INTERP_LINEAR, P, 0, 2>(a, tile, 0);
+    }
   }
 }
 
@@ -179,7 +219,7 @@ __device__ __forceinline__ void filter_item(const SliceDev& s, int pair, Cplx<fl
 // and the backprojection items: the fused launch takes both, the two-phase
 // filter launches only the first (the two-phase backprojection runs in the
 // class kernels below, one body each).
-template <int P, bool SYM, bool DO_F, bool DO_B, int FL = 0>
+template <int P, bool SYM, bool DO_F, bool DO_B, int FL = 0, bool MIR = false>
 __device__ __forceinline__ void fbp_batch_body(const SliceDev* __restrict__ slices,
                                                const Item* __restrict__ items,
                                                const int* __restrict__ queueStart, int* counters,
@@ -262,6 +302,8 @@ __device__ __forceinline__ void fbp_batch_body(const SliceDev* __restrict__ slic
         } else {
           bp_item<P>(s.bp, s.gk, s.interp, s.deg, it.a, smem, s_v, s_lo);
         }
+      } else if constexpr (MIR) {
+        bp_mirror_item<P>(s.bp, s.gk, s.interp, s.deg, s.mirNM, it.a);
       } else {
         bp_item<P>(s.bp, s.gk, s.interp, s.deg, it.a, smem, s_v, s_lo);
       }
@@ -301,6 +343,16 @@ __global__ __launch_bounds__(256) void fbp_batch_bp_kernel(const SliceDev* __res
                                                            const int* __restrict__ queueStart,
                                                            int* counters, int* error) {
   fbp_batch_body<P, false, false, true>(slices, items, queueStart, counters, error);
+}
+
+// The two-phase backprojection launch of the mirror-body items of a batch
+// whose mirror slices span several classes (one launch, as above).
+template <int P>
+__global__ __launch_bounds__(256) void fbp_batch_mirror_kernel(const SliceDev* __restrict__ slices,
+                                                               const Item* __restrict__ items,
+                                                               const int* __restrict__ queueStart,
+                                                               int* counters, int* error) {
+  fbp_batch_body<P, false, false, true, 0, true>(slices, items, queueStart, counters, error);
 }
 
 // The two-phase backprojection launch of the symmetric slices of one class
@@ -354,7 +406,7 @@ const void* sym_class_kernel_g(int gk, int deg, int S, int K) {
 // The two-phase backprojection launch of the plain-body items of one class
 // (geometry, interpolation, fan polynomial degree) at that body's register
 // budget.
-template <int P, int GK, int INTERP, int DEG>
+template <int P, int GK, int INTERP, int DEG, int NM>
 __global__ __launch_bounds__(256) void fbp_batch_plain_class_kernel(const SliceDev* __restrict__ slices,
                                                                     const Item* __restrict__ items,
                                                                     const int* __restrict__ queueStart,
@@ -373,29 +425,42 @@ __global__ __launch_bounds__(256) void fbp_batch_plain_class_kernel(const SliceD
     if (i >= qEnd) break;  // every workgroup ends when its queue is drained
     const Item it = items[i];
     const SliceDev& s = slices[it.slice];
-    dev::bp_mixed_tile<GK, INTERP, P, DEG>(s.bp, it.a, 0, reinterpret_cast<CoefT<INTERP>*>(smem), s_v, s_lo);
+    if constexpr (NM > 1)
+      dev::bp_mirror_tile<GK, INTERP, P, DEG, NM>(s.bp, it.a, 0);
+    else
+      dev::bp_mixed_tile<GK, INTERP, P, DEG>(s.bp, it.a, 0, reinterpret_cast<CoefT<INTERP>*>(smem), s_v, s_lo);
     __syncthreads();  // LDS is reused by the next item
   }
 }
 
-template <int P, int INTERP>
+template <int P, int INTERP, int NM>
 const void* plain_class_kernel_g(int gk, int deg) {
-  if (gk == 0) return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 0, INTERP, 0>);
-  if (gk == 2) return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 2, INTERP, 0>);
+  if (gk == 0) return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 0, INTERP, 0, NM>);
+  if (gk == 2) return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 2, INTERP, 0, NM>);
   switch (deg) {
-    case 4: return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 1, INTERP, 4>);
-    case 6: return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 1, INTERP, 6>);
-    case 8: return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 1, INTERP, 8>);
-    default: return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 1, INTERP, 0>);
+    case 4: return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 1, INTERP, 4, NM>);
+    case 6: return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 1, INTERP, 6, NM>);
+    case 8: return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 1, INTERP, 8, NM>);
+    default: return reinterpret_cast<const void*>(fbp_batch_plain_class_kernel<P, 1, INTERP, 0, NM>);
   }
 }
 
-// The class kernel of plain-body slices (P 1, 2 or 4).
-const void* plain_class_kernel(int P, int gk, int interp, int deg) {
+template <int P, int INTERP>
+const void* plain_class_kernel_nm(int gk, int deg, int NM) {
+  if (NM == 4) return plain_class_kernel_g<P, INTERP, 4>(gk, deg);
+  if (NM == 2) return plain_class_kernel_g<P, INTERP, 2>(gk, deg);
+  return plain_class_kernel_g<P, INTERP, 1>(gk, deg);
+}
+
+// The class kernel of plain-body slices (P 1, 2 or 4; NM 1, or 2 and 4 for
+// the mirror body).
+const void* plain_class_kernel(int P, int gk, int interp, int deg, int NM) {
   const bool cubic = interp == INTERP_CUBIC;
-  if (P == 1) return cubic ? plain_class_kernel_g<1, INTERP_CUBIC>(gk, deg) : plain_class_kernel_g<1, INTERP_LINEAR>(gk, deg);
-  if (P == 2) return cubic ? plain_class_kernel_g<2, INTERP_CUBIC>(gk, deg) : plain_class_kernel_g<2, INTERP_LINEAR>(gk, deg);
-  return cubic ? plain_class_kernel_g<4, INTERP_CUBIC>(gk, deg) : plain_class_kernel_g<4, INTERP_LINEAR>(gk, deg);
+  if (P == 1)
+    return cubic ? plain_class_kernel_nm<1, INTERP_CUBIC>(gk, deg, NM) : plain_class_kernel_nm<1, INTERP_LINEAR>(gk, deg, NM);
+  if (P == 2)
+    return cubic ? plain_class_kernel_nm<2, INTERP_CUBIC>(gk, deg, NM) : plain_class_kernel_nm<2, INTERP_LINEAR>(gk, deg, NM);
+  return cubic ? plain_class_kernel_nm<4, INTERP_CUBIC>(gk, deg, NM) : plain_class_kernel_nm<4, INTERP_LINEAR>(gk, deg, NM);
 }
 
 // The class kernel of symmetric slices (P 1 or 2).
@@ -442,12 +507,15 @@ struct ctsim_batch {
     int N = 0, logN = 0;
     DevBuf mul, tw;  // FFT-convolution multipliers and twiddles (convolution filters)
     DevBuf vc, coef;
+    MirrorViews mir;  // mirror partners of the views (in vc)
     double cost = 0;
   };
   std::vector<Slice> s;
   int P = 2, lanesY = 16, chunkBudget = 32 * 1024;
   bool fused = true;
-  int sym = 1;  // 0: plain body; 1, 2, 4: symmetric body with that many view groups per tile
+  // 0: plain body; 1, 2, 4: symmetric body with that many view groups per
+  // tile; + 8: the mirror body for the other slices (two-phase mode)
+  int sym = 1;
   int ldsFilter = 0;
   std::vector<SliceDev> host;
   DevBuf d_slices, d_itemsF, d_itemsAll, d_qF, d_qAll, d_counters, d_error;
@@ -600,7 +668,7 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
     // of kPipeRow cells) at P <= 2; in two-phase mode the symmetric classes
     // have launches of their own, so with P = 4 their slices take P = 2.
     const SymViews sv = bp_sym_views(sl.angles.data(), p.nView);
-    const bool symViews = b.sym > 0 && p.symImage && sv.S > 0;
+    const bool symViews = (b.sym & 7) > 0 && p.symImage && sv.S > 0;
     MixedShape m;
     m.lanesY = b.lanesY;
     m.split = 1;
@@ -614,18 +682,31 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
     d.P = m.P;
     const int Wv = m.W > 0 ? m.W : p.nDet + 4;
     const size_t elem = p.interp == INTERP_CUBIC ? 16 : 8;
-    // the plain body stages chunks of whole view windows within the budget
+    // the plain body stages chunks of whole view windows within the budget;
+    // the mirror body NM windows per view (when one view's fit)
     if (!d.sym && static_cast<size_t>(Wv) * elem > static_cast<size_t>(b.chunkBudget))
       return fail(CTSIM_ERR_UNSUPPORTED, "batch: a view's detector window exceeds the chunk budget");
-    int chunk = static_cast<int>(b.chunkBudget / (Wv * elem));
+    int chunk = static_cast<int>(b.chunkBudget / (static_cast<size_t>(Wv) * elem));
     chunk = chunk < 1 ? 1 : (chunk > dev::kMaxChunk ? dev::kMaxChunk : chunk);
     m.chunk = chunk;
     if (static_cast<size_t>(chunk) * Wv * elem > 62 * 1024)
       return fail(CTSIM_ERR_UNSUPPORTED, "batch: a slice's detector window does not fit in LDS");
+    // the mirror body (two-phase mode) stages into static buffers of its
+    // own, as many views per chunk as fit them (none: the plain body)
+    d.mirNM = 1;
+    if (!d.sym && (b.sym & 8) && !b.fused && sl.mir.NM > 1) {
+      MixedShape mm = m;
+      mm.chunk = dev::kMaxChunk;
+      const int mc = bp_mirror_chunk(p, mm, sl.mir.NM, p.nView);
+      if (mc > 0) {
+        d.mirNM = sl.mir.NM;
+        m.chunk = mc;
+      }
+    }
     const size_t symLds = static_cast<size_t>(4 * sv.S) * dev::kPipeRow * elem;
     d.symS = sv.S;
     d.symK = sv.K;
-    maxBpLds = std::max(maxBpLds, d.sym ? symLds : static_cast<size_t>(chunk) * Wv * elem);
+    if (d.mirNM == 1) maxBpLds = std::max(maxBpLds, d.sym ? symLds : static_cast<size_t>(chunk) * Wv * elem);
     MixedArgs& a = d.bp;
     a = MixedArgs{};
     a.nx = p.nx; a.ny = p.ny; a.nDet = p.nDet; a.nView = p.nView;
@@ -650,6 +731,15 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
     a.coef = sl.coef.ptr;
     a.image = b.images.empty() ? nullptr : b.images[k];
     a.overwrite = 1;
+    a.domX = p.nx;
+    a.domY = p.ny;
+    if (d.mirNM > 1) {  // tiles of the mirror domain
+      a.domX = (p.nx + 1) / 2;
+      a.domY = d.mirNM == 4 ? (p.ny + 1) / 2 : p.ny;
+      a.mapO = sl.mir.mapO;
+      a.tilesX = (a.domX + tx - 1) / tx;
+      a.tilesY = (a.domY + ty - 1) / ty;
+    }
     if (d.sym) {
       a.half = p.nx / 2;
       a.Q = p.nView / sv.S;
@@ -659,7 +749,7 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
       a.capW = 4 * sv.S * dev::kPipeRow;
       // view groups of at least 8 base views; each group's partial image
       // goes to the batch's partial buffer, added by batch_combine_kernel
-      int G = std::max(1, std::min(b.sym, a.Q / 8));
+      int G = std::max(1, std::min(b.sym & 7, a.Q / 8));
       a.viewsPerGroup = (a.Q + G - 1) / G;
       G = (a.Q + a.viewsPerGroup - 1) / a.viewsPerGroup;
       a.split = G;
@@ -743,18 +833,22 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
   // the two-phase backprojection by class (body, geometry, interpolation,
   // polynomial degree, symmetry, P), each class's items in the per-XCD order
   // above
-  // plain-body slices of several classes share one generic launch (key -1)
+  // plain-body slices of several classes share one generic launch (key -1),
+  // and so do mirror-body slices (key -2)
   auto plainKey = [&](int k) {
     const SliceDev& d = b.host[k];
-    return d.gk + 3 * (d.interp == INTERP_CUBIC ? 1 : 0) + 6 * d.deg + 900 * d.P;
+    return d.gk + 3 * (d.interp == INTERP_CUBIC ? 1 : 0) + 6 * d.deg + 60 * d.mirNM + 900 * d.P;
   };
-  std::vector<int> plainKeys;
-  for (int k = 0; k < S; k++)
-    if (!b.host[k].sym && std::find(plainKeys.begin(), plainKeys.end(), plainKey(k)) == plainKeys.end())
-      plainKeys.push_back(plainKey(k));
-  const bool genericPlain = plainKeys.size() > 1;
+  std::vector<int> plainKeys, mirrorKeys;
+  for (int k = 0; k < S; k++) {
+    if (b.host[k].sym) continue;
+    std::vector<int>& keys = b.host[k].mirNM > 1 ? mirrorKeys : plainKeys;
+    if (std::find(keys.begin(), keys.end(), plainKey(k)) == keys.end()) keys.push_back(plainKey(k));
+  }
+  const bool genericPlain = plainKeys.size() > 1, genericMirror = mirrorKeys.size() > 1;
   auto bclassKey = [&](int k) {
     const SliceDev& d = b.host[k];
+    if (!d.sym && d.mirNM > 1) return genericMirror ? -2 : plainKey(k);
     if (!d.sym) return genericPlain ? -1 : plainKey(k);
     return d.gk + 3 * (d.interp == INTERP_CUBIC ? 1 : 0) + 6 * d.deg + 60 * d.symS + 300 * d.symK +
            900 * d.P + 9000;
@@ -785,14 +879,18 @@ int batch_layout(ctsim_ctx* ctx, ctsim_batch& b) {
       bc.lds = static_cast<size_t>(4 * d.symS) * dev::kPipeRow * elem;
     } else {
       bc.generic = key < 0;
-      bc.kernel = !bc.generic ? plain_class_kernel(d.P, d.gk, d.interp, d.deg)
+      bc.kernel = !bc.generic ? plain_class_kernel(d.P, d.gk, d.interp, d.deg, d.mirNM)
+                  : key == -2 ? (d.P == 4   ? reinterpret_cast<const void*>(fbp_batch_mirror_kernel<4>)
+                                 : d.P == 1 ? reinterpret_cast<const void*>(fbp_batch_mirror_kernel<1>)
+                                            : reinterpret_cast<const void*>(fbp_batch_mirror_kernel<2>))
                   : d.P == 4  ? reinterpret_cast<const void*>(fbp_batch_bp_kernel<4>)
                   : d.P == 1  ? reinterpret_cast<const void*>(fbp_batch_bp_kernel<1>)
                               : reinterpret_cast<const void*>(fbp_batch_bp_kernel<2>);
       for (int k = 0; k < S; k++)  // the largest staging of the launch's slices
         if (bclassKey(k) == key)
-          bc.lds = std::max(bc.lds, static_cast<size_t>(b.host[k].bp.chunk) * b.host[k].bp.W *
-                                        (b.host[k].interp == INTERP_CUBIC ? 16 : 8));
+          if (b.host[k].mirNM == 1)  // (the mirror body's staging is static)
+            bc.lds = std::max(bc.lds, static_cast<size_t>(b.host[k].bp.chunk) * b.host[k].bp.W *
+                                          (b.host[k].interp == INTERP_CUBIC ? 16 : 8));
     }
     b.bclass.push_back(bc);
   }
@@ -978,7 +1076,7 @@ int ctsim_batch_create(ctsim_ctx* ctx, ctsim_batch** out, const ctsim_slice_desc
     maxN = std::max(maxN, sl.N);
     sl.angles.assign(sd.angles, sd.angles + sd.nView);
     std::vector<ViewConstF32> vc(sd.nView);
-    bp_view_consts_f32(sl.bp, sd.angles, sd.nView, vc.data());
+    sl.mir = bp_view_consts_f32(sl.bp, sd.angles, sd.nView, vc.data());
     if ((rc = sl.vc.reserve(vc.size() * sizeof(ViewConstF32))) != CTSIM_OK) break;
     if (hipMemcpy(sl.vc.ptr, vc.data(), vc.size() * sizeof(ViewConstF32), hipMemcpyHostToDevice) != hipSuccess) {
       rc = fail(CTSIM_ERR_HIP, "batch: view constant upload failed");
@@ -1029,7 +1127,8 @@ int ctsim_batch_set_shape(ctsim_ctx* ctx, ctsim_batch* b, int P, int lanesY, int
                           int sym) {
   if (!ctx || !b) return fail(CTSIM_ERR_INVALID, "ctsim_batch_set_shape: NULL argument");
   if (!(P == 1 || P == 2 || P == 4) || !(lanesY == 8 || lanesY == 16 || lanesY == 32) ||
-      chunkBytes < 1024 || chunkBytes > 62 * 1024 || !(sym == 0 || sym == 1 || sym == 2 || sym == 4))
+      chunkBytes < 1024 || chunkBytes > 62 * 1024 ||
+      !((sym & ~8) == 0 || (sym & ~8) == 1 || (sym & ~8) == 2 || (sym & ~8) == 4) || sym < 0 || sym > 12)
     return fail(CTSIM_ERR_INVALID, "ctsim_batch_set_shape: shape out of range");
   CTSIM_CHECK(ctx->activate());
   CTSIM_HIP(hipStreamSynchronize(b->stream));
@@ -1089,13 +1188,14 @@ int ctsim_batch_autotune(ctsim_ctx* ctx, ctsim_batch* b, const float* const* d_s
   int rc = CTSIM_OK;
   // P = 1 only with the symmetric body; P = 4 with it only in two-phase
   // mode (its symmetric slices then take P = 2 in launches of their own)
+  // (the mirror body, sym + 8, only in two-phase mode)
   for (int P : {1, 2, 4})
-    for (int sym : {1, 4, 0})
+    for (int sym : {1, 4, 0, 9, 12, 8})
     for (int lanesY : {8, 16, 32})
       for (int chunk : {16 * 1024, 32 * 1024})
         for (int fused : {1, 0}) {
           if (rc != CTSIM_OK) break;
-          if ((P == 1 && !sym) || (P == 4 && sym && fused)) continue;
+          if ((P == 1 && !(sym & 7)) || (P == 4 && (sym & 7) && fused) || ((sym & 8) && fused)) continue;
           if ((rc = ctsim_batch_set_shape(ctx, b, P, lanesY, chunk, fused, sym)) != CTSIM_OK) {
             if (rc == CTSIM_ERR_UNSUPPORTED) { rc = CTSIM_OK; continue; }
             break;

@@ -2,8 +2,10 @@
 // so every FP64 table comes out bit-identical to the reference's own.
 #include <strings.h>
 
+#include <algorithm>
 #include <cmath>
 #include <utility>
+#include <vector>
 #include <cstring>
 
 #include "bp_plan.h"
@@ -184,6 +186,11 @@ int bp_plan_init(BpPlan& p, int geometry, int nDet, double detInc, int nView,
       sym = fabs(static_cast<double>(p.xs[i]) + p.xs[nx - 1 - i]) <= tol &&
             fabs(static_cast<double>(p.ys[i]) - p.xs[i]) <= tol;
     p.symImage = sym;
+    bool mx = true, my = true;
+    for (int i = 0; mx && i < nx; i++) mx = fabs(static_cast<double>(p.xs[i]) + p.xs[nx - 1 - i]) <= 1e-4 * fabs(p.xInc);
+    for (int i = 0; my && i < ny; i++) my = fabs(static_cast<double>(p.ys[i]) + p.ys[ny - 1 - i]) <= 1e-4 * fabs(p.yInc);
+    p.mirX = mx;
+    p.mirY = my;
   }
   p.mixed = bp_default_shape(p, 256);
   return CTSIM_OK;
@@ -239,8 +246,60 @@ void bp_view_consts_f64(const BpPlan& p, const double* angles, int nView, ViewCo
   }
 }
 
-// Per-view constants of the mixed path: FP64 trig on the host, rounded once.
-void bp_view_consts_f32(const BpPlan& p, const double* angles, int nView, ViewConstF32* out) {
+namespace {
+
+// Partners of every view under one pixel mirror (0: Y, 1: X, 2: O), as
+// view * 2 + reflected; false when some view has none or two views share
+// one (angles equal to 1e-9 rad modulo 2 pi).
+bool mirror_partners(const BpPlan& p, const double* angles, int nView, int type, std::vector<int>& out) {
+  const double twoPi = 2 * kPi, tol = 1e-9;
+  auto wrap = [&](double a) {
+    a = fmod(a, twoPi);
+    return a < 0 ? a + twoPi : a;
+  };
+  std::vector<std::pair<double, int>> sorted(nView);
+  for (int v = 0; v < nView; v++) sorted[v] = {wrap(angles[v]), v};
+  std::sort(sorted.begin(), sorted.end());
+  auto find = [&](double target) -> int {
+    const double t = wrap(target);
+    auto it = std::lower_bound(sorted.begin(), sorted.end(), std::make_pair(t - tol, -1));
+    if (it != sorted.end() && fabs(it->first - t) <= tol) return it->second;
+    if (t < tol && !sorted.empty() && fabs(sorted.back().first - twoPi - t) <= tol) return sorted.back().second;
+    if (t > twoPi - tol && !sorted.empty() && fabs(sorted.front().first + twoPi - t) <= tol)
+      return sorted.front().second;
+    return -1;
+  };
+  const bool par = p.geom == GEOM_PARALLEL;
+  out.assign(nView, -1);
+  std::vector<char> hit(nView, 0);
+  for (int v = 0; v < nView; v++) {
+    const double th = angles[v];
+    // the reflecting target, then (parallel) the one at the same position
+    const double refl = type == 0 ? -th : (type == 1 ? kPi - th : (par ? th : th + kPi));
+    const double same = type == 0 ? kPi - th : (type == 1 ? -th : th + kPi);
+    int w = -1, r = 0;
+    if (!par && type == 2) {
+      w = find(same);
+    } else {
+      w = find(refl);
+      r = 1;
+      if (w < 0 && par) {
+        w = find(same);
+        r = 0;
+      }
+    }
+    if (w < 0 || hit[w]) return false;
+    hit[w] = 1;
+    out[v] = 2 * w + r;
+  }
+  return true;
+}
+
+}  // namespace
+
+// Per-view constants of the mixed path: FP64 trig on the host, rounded once,
+// and the views' mirror partners.
+MirrorViews bp_view_consts_f32(const BpPlan& p, const double* angles, int nView, ViewConstF32* out) {
   for (int v = 0; v < nView; v++) {
     double c = cos(angles[v]), s = sin(angles[v]);
     if (p.geom == GEOM_PARALLEL) {
@@ -250,8 +309,43 @@ void bp_view_consts_f32(const BpPlan& p, const double* angles, int nView, ViewCo
       out[v].ca = static_cast<float>(c);
       out[v].sa = static_cast<float>(s);
     }
-    out[v].pad0 = out[v].pad1 = 0.f;
+    out[v].mir[0] = out[v].mir[1] = out[v].mir[2] = 0;
+    out[v].order = v;
+    out[v].pad[0] = out[v].pad[1] = 0.f;
   }
+  MirrorViews mv;
+  // the mirror body reflects about the centre cell, which needs 2 * centre
+  // to be a whole cell (it is: the centre is (nDet - 1) / 2 rounded down)
+  std::vector<int> y, x, o;
+  const bool hasY = p.mirX && mirror_partners(p, angles, nView, 0, y);
+  const bool hasX = p.mirY && mirror_partners(p, angles, nView, 1, x);
+  const bool hasO = p.mirX && p.mirY && mirror_partners(p, angles, nView, 2, o);
+  if (hasY && hasX && hasO) {
+    mv.NM = 4;
+    for (int v = 0; v < nView; v++) {
+      out[v].mir[0] = y[v];
+      out[v].mir[1] = x[v];
+      out[v].mir[2] = o[v];
+    }
+  } else if (hasY || hasO) {
+    mv.NM = 2;
+    mv.mapO = hasY ? 0 : 1;
+    for (int v = 0; v < nView; v++) out[v].mir[0] = hasY ? y[v] : o[v];
+  }
+  if (mv.NM > 1) {  // orbits contiguous: each view followed by its partners
+    std::vector<char> seen(nView, 0);
+    int s = 0;
+    for (int v = 0; v < nView; v++) {
+      if (seen[v]) continue;
+      int orbit[4] = {v, out[v].mir[0] >> 1, out[v].mir[1] >> 1, out[v].mir[2] >> 1};
+      for (int m = 0; m < mv.NM; m++)
+        if (!seen[orbit[m]]) {
+          seen[orbit[m]] = 1;
+          out[s++].order = orbit[m];
+        }
+    }
+  }
+  return mv;
 }
 
 // Upper bound on the detector positions one tile of tileX x tileY pixels can

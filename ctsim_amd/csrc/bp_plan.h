@@ -24,10 +24,18 @@ struct ViewConstF64 {
   double c;         // Diff: detPosColStart       IntDiff: bit pattern of int64 colStart
 };
 
-// Per-view constants of the mixed path (16 bytes).
+// Per-view constants of the mixed path (32 bytes).  mir: the view's partners
+// under the pixel-grid mirrors of the mirror body (MirrorViews), each
+// partner view * 2 + 1 when the partner sees the mirrored pixel at the
+// reflected detector position, + 0 when at the same position.  order: the
+// mirror body takes its base views in the order vc[0].order, vc[1].order,
+// ..., which lists each view's partners next to it, so that a view group
+// holds whole orbits and reads no other group's rows.
 struct ViewConstF32 {
   float ca, sa;     // parallel: cos/detInc, sin/detInc; fan: cos, sin
-  float pad0, pad1;
+  int mir[3];
+  int order;
+  float pad[2];
 };
 
 // Launch shape of the mixed kernel: a workgroup of 256 lanes covers
@@ -37,7 +45,9 @@ struct ViewConstF32 {
 // sym = 1 selects the rotationally symmetric kernel when the image and the
 // views allow it (P of 1 or 2; views staged up to chunk base views at a time
 // in symKB KB of LDS, 0: 62 KB); sym = 2 its software-pipelined form (one
-// base view at a time, windows of at most 128 cells).
+// base view at a time, windows of at most 128 cells); sym = 3 the mirror
+// body (the plain body's staging, 2 or 4 updates per position) when the
+// views have mirror partners.
 struct MixedShape {
   int lanesY = 16, P = 4, split = 1, chunk = 16, W = 0, sym = 0, symKB = 0;
 };
@@ -45,9 +55,27 @@ struct MixedShape {
 // Rotational symmetry of a view set: views v + s*Q (s < S, Q = nView / S)
 // are view v turned by s*K quarter turns, counter-clockwise (dir = 1) or
 // clockwise (dir = -1); S = 0 when the views have no such symmetry.
+// NM, mapO: the mirror members of the same views (MirrorViews).
 struct SymViews {
   int S = 0, K = 0;
   float dir = 1.f;
+  int NM = 1, mapO = 0;
+};
+
+// Mirror symmetry of a view set.  The mirrors Y (x -> -x), X (y -> -y) and
+// O (both) map the pixel grid onto itself when it is centred.  A pixel's
+// mirror image M p in a partner view sees the detector where p sees it in
+// the view itself, reflected about the centre cell or not:
+//   fan beam:  Y with theta' = -theta, X with pi - theta (reflected),
+//              O with theta + pi (same position);
+//   parallel:  Y with -theta (reflected) or pi - theta (same), X with
+//              pi - theta (reflected) or -theta (same), O with theta itself
+//              (reflected) or theta + pi (same).
+// NM = 4: every view has Y, X and O partners (mir[0..2]); NM = 2: every view
+// has a Y partner (mapO = 0) or else an O partner (mapO = 1), in mir[0];
+// NM = 1: no mirror body.
+struct MirrorViews {
+  int NM = 1, mapO = 0;
 };
 
 struct BpPlan {
@@ -72,6 +100,9 @@ struct BpPlan {
   // Square, even-sized pixel grid centred on the origin: the quarter turn
   // maps pixel centres onto pixel centres.
   bool symImage = false;
+  // xs (ys) symmetric about 0: the Y (X) mirror maps pixel centres onto
+  // pixel centres.
+  bool mirX = false, mirY = false;
   MixedShape mixed;                 // current launch shape (tuned or default)
   bool mixedTuned = false;
   // Per-view constants of the last angle set, kept on the device so repeated
@@ -87,7 +118,9 @@ int bp_plan_init(BpPlan& p, int geometry, int nDet, double detInc, int nView,
                  double viewDiameter, double F, double sdl, const char* backprojName,
                  const char* interpName, int interpFactor, int nx, int ny, const double* roi);
 void bp_view_consts_f64(const BpPlan& p, const double* angles, int nView, ViewConstF64* out);
-void bp_view_consts_f32(const BpPlan& p, const double* angles, int nView, ViewConstF32* out);
+// Also fills the mirror partners (mir) and returns which mirror members the
+// views allow on this plan's grid.
+MirrorViews bp_view_consts_f32(const BpPlan& p, const double* angles, int nView, ViewConstF32* out);
 int bp_window_bound(const BpPlan& p, int tileX, int tileY);
 SymViews bp_sym_views(const double* angles, int nView);
 MixedShape bp_default_shape(const BpPlan& p, int numCUs);
@@ -116,6 +149,11 @@ int bp_view_consts_mixed(ctsim_ctx* ctx, const BpPlan& p, const double* h_angles
 bool bp_mixed_tiled(const BpPlan& p);
 bool bp_sym_ok(const BpPlan& p, const MixedShape& m, int nView);
 bool bp_sym_ok_for(const BpPlan& p, const MixedShape& m, const SymViews& sv);
+// The mirror body runs for shape sym = 3 when the views have mirror partners
+// (NM >= 2): the views per chunk it stages (NM windows each, in one of its
+// two LDS buffers), the largest power of two up to min(m.chunk, 32) that fits; 0
+// when one view does not fit or the coefficient rows span 2 GiB or more.
+int bp_mirror_chunk(const BpPlan& p, const MixedShape& m, int NM, int nView);
 size_t sym_lds(const MixedShape& m);  // LDS bytes the symmetric kernel stages into
 
 }  // namespace ctsim

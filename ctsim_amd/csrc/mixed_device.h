@@ -105,6 +105,9 @@ struct MixedArgs {
   // capW coefficients of LDS; dirf = +1 (counter-clockwise steps) or -1.
   int Q, half, capW;
   float dirf;
+  // Mirror body (bp_mirror_tile): tiles cover columns [0, domX) and rows
+  // [0, domY); with two members, mapO = 1 when the second is O (else Y).
+  int domX, domY, mapO;
 };
 
 // A pointer known to be the same in every lane of the wave, moved to scalar
@@ -168,18 +171,18 @@ __device__ __forceinline__ float atan_f32(float t) {
   return copysignf(r, t);
 }
 
-// Detector offset (before the centre) of the equiangular geometry:
-// atan(t) / detInc with t = tan(gamma).
+// Detector position of the equiangular geometry: atan(t) / detInc + add
+// with t = tan(gamma).
 template <int DEG>
-__device__ __forceinline__ float fan_angle(const float* c, float t) {
+__device__ __forceinline__ float fan_angle(const float* c, float t, float add) {
   if constexpr (DEG > 0) {
     const float z = t * t;
     float p = c[DEG];
 #pragma unroll
     for (int k = DEG - 1; k >= 0; k--) p = fmaf(p, z, c[k]);
-    return t * p;
+    return fmaf(t, p, add);
   } else {
-    return atan_f32(t) * c[0];
+    return fmaf(atan_f32(t), c[0], add);
   }
 }
 
@@ -193,13 +196,14 @@ __device__ __forceinline__ void pos_mixed(const MixedArgs& a, float ca, float sa
     pos = fmaf(y, sa, fmaf(x, ca, cshift));
     wgt = 1.f;
   } else {
+    // the x terms are common to a lane's rows: x ca and F + x sa once per view
     const float rc = fmaf(y, sa, x * ca);             // r cos(beta - phi)
-    const float fl = fmaf(x, sa, fmaf(-y, ca, a.F));  // F + r sin(beta - phi)
+    const float fl = fmaf(-y, ca, fmaf(x, sa, a.F));  // F + r sin(beta - phi)
     if constexpr (GK == 1) {
       // one reciprocal for both 1/fl and 1/L^2, L^2 = fl^2 + rc^2 = 2 F fl + r^2 - F^2
       const float L2 = fmaf(a.twoF, fl, r2mF2);
       const float rr = __builtin_amdgcn_rcpf(fl * L2);
-      pos = fan_angle<DEG>(a.atan, rc * (L2 * rr)) + cshift;
+      pos = fan_angle<DEG>(a.atan, rc * (L2 * rr), cshift);
       wgt = fl * rr;
     } else {
       const float q = __builtin_amdgcn_rcpf(fl);
@@ -367,6 +371,272 @@ __device__ __forceinline__ void bp_mixed_tile(const MixedArgs& a, int tile, int 
   }
 }
 
+// The mirror body's staging buffers: two static LDS arrays of this many
+// bytes (separate objects, so that the compiler sees that reads of one do
+// not wait for the direct-to-LDS loads into the other).
+constexpr int kMirBufBytes = 16 * 1024;
+
+// LDS slots one staged chunk of the mirror body takes: nv * NM windows of W
+// cells in units of the direct-to-LDS load (one 16-byte cubic cell, or half
+// of an 8-byte linear cell), rounded up to whole rounds of 256 units.
+__host__ __device__ constexpr int mirror_units(int nv, int NM, int W, int unitsPerCell) {
+  return ((nv * NM * W * unitsPerCell + 255) / 256) * 256;
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// The staging buffers and view-block records of the mirror body, shared by
+// all of its instantiations in a kernel: per block of up to 64 base views,
+// the view constants (ca, sa, centre - lo) and, per window (view vv, member m) at vv * NM + m, the
+// detector cell of its slot 0 and its row * 2 + reflected (slot j then holds
+// cell first + j, or first - j).  Plain arrays indexed from their own base,
+// so that the compiler can tell every LDS access's object.
+static __shared__ __attribute__((aligned(16))) unsigned char s_mirBuf0[kMirBufBytes];
+static __shared__ __attribute__((aligned(16))) unsigned char s_mirBuf1[kMirBufBytes];
+static __shared__ float4 s_mirView[64];
+static __shared__ int2 s_mirSeg[64 * 4];
+
+// The mirror body: the plain body with NM updates per position.  Member 0 is
+// the lane's pixel p in view v; member m > 0 is the mirror image M_m p in the
+// partner view vc[v].mir[m - 1] (NM = 4: members Y, X, O; NM = 2: Y, or O
+// when mapO).  M_m p sees the partner's detector at p's position or at its
+// reflection 2c - pos about the centre cell c, so each partner window is
+// staged so that its slot j holds the interval that p's position falls in
+// when it falls in slot j of view v's window [lo, lo + W): cell lo + j of
+// the partner row, or, reflected, cell 2c - 1 - lo - j, read at fraction
+// 1 - u.  Every member thus reads the slot of the base pixel.
+// Base views come in orbit order (vc[s].order), so a view group reads only
+// its own views' rows.  Tiles cover columns [0, domX) (and rows [0, domY)
+// with four members); a member whose mirror image is the pixel itself (the
+// middle column or row of an odd size) or another member's is not stored,
+// so every pixel takes every view once.
+// Windows are found for 64 base views at a time (four lanes per view, one
+// tile corner each).  Within a block, staging is software-pipelined: the
+// windows of the next chunk of views (a power of two, so chunks never
+// straddle a block) go straight from global memory to one LDS buffer (one
+// flat copy, 256 units a round; cells outside the detector read as zero by
+// the buffer range check) while the lanes interpolate the current chunk from
+// the other.  The two buffers are separate objects used at fixed call sites
+// (chunks unrolled by two), so the compiler sees that reads of one do not
+// wait for the copy into the other.  The FP32 sums are folded into
+// Kahan-compensated totals every 8 or more views.
+template <int GK, int INTERP, int P, int DEG, int NM>
+__device__ __forceinline__ void bp_mirror_tile(const MixedArgs& a, int tile, int grp) {
+  using CT = CoefT<INTERP>;
+  static_assert(NM == 2 || NM == 4, "two or four members");
+  constexpr int kUnit = INTERP == INTERP_CUBIC ? 16 : 4;  // bytes per direct-to-LDS load
+  constexpr int kUpc = static_cast<int>(sizeof(CT)) / kUnit;  // units per cell
+  CT* const s_buf0 = reinterpret_cast<CT*>(s_mirBuf0);
+  CT* const s_buf1 = reinterpret_cast<CT*>(s_mirBuf1);
+  const int t = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int tx = 256 / a.lanesY, ty = a.lanesY * P;
+  const int ix0 = (tile / a.tilesY) * tx, iy0 = (tile % a.tilesY) * ty;
+  const int ix = ix0 + t / a.lanesY, iyb = iy0 + (t % a.lanesY) * P;
+  const int cx1 = min(ix0 + tx - 1, a.domX - 1), cy1 = min(iy0 + ty - 1, a.domY - 1);
+  const int W = a.W;
+  const float x = a.xs[min(ix, a.domX - 1)];
+  float y[P], r2mF2[P];
+#pragma unroll
+  for (int k = 0; k < P; k++) {
+    y[k] = a.ys[min(iyb + k, a.domY - 1)];
+    r2mF2[k] = fmaf(x, x, fmaf(y[k], y[k], -a.F * a.F));
+  }
+  const int vBegin = grp * a.viewsPerGroup;
+  const int vEnd = min(a.nView, vBegin + a.viewsPerGroup);
+  float sum[NM][P], comp[NM][P], part[NM][P];
+#pragma unroll
+  for (int m = 0; m < NM; m++)
+#pragma unroll
+    for (int k = 0; k < P; k++) sum[m][k] = comp[m][k] = part[m][k] = 0.f;
+  const int nDet = __builtin_amdgcn_readfirstlane(a.nDet);
+  const int twoC = static_cast<int>(2.f * a.center);  // the centre is a whole cell
+  // one buffer resource over every row (the host keeps nView * nDet cells
+  // below 2^31 bytes); offsets at or beyond it read zero
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<CT*>(uniform_ptr(static_cast<const CT*>(a.coef))), 0,
+      __builtin_amdgcn_readfirstlane(a.nView * nDet * static_cast<int>(sizeof(CT))), 0x00020000);
+  // this lane's first cell of the flat copy and its step per round
+  constexpr int kStep = 256 / kUpc;
+  const int cell0 = t / kUpc, part0 = (t % kUpc) * kUnit;
+  const int seg0 = cell0 / W, j0 = cell0 % W;
+  const int qW = kStep / W, rW = kStep % W;
+
+  // windows of base views [vb, vb + nb), nb <= 64: lane (vv, q) = (t / 4,
+  // t % 4) takes tile corner q; the four lanes' minimum gives lo
+  auto meta = [&](int vb, int nb) {
+    const int vv = t >> 2, q = t & 3;
+    if (vv < nb) {
+      const int view = a.vc[vb + vv].order;  // base views in orbit order
+      const ViewConstF32 c = a.vc[view];
+      int lo = -2;
+      if (a.W < a.nDet + 4) {
+        const float cx = (q & 1) ? a.xs[cx1] : a.xs[ix0], cy = (q & 2) ? a.ys[cy1] : a.ys[iy0];
+        float pos, wd;
+        pos_mixed<GK, DEG>(a, c.ca, c.sa, a.center, cx, cy, fmaf(cx, cx, fmaf(cy, cy, -a.F * a.F)), pos, wd);
+        pos = fminf(pos, __shfl_xor(pos, 1, 64));
+        pos = fminf(pos, __shfl_xor(pos, 2, 64));
+        if (!(pos > -1e9f && pos < 1e9f)) pos = 0.f;
+        lo = static_cast<int>(floorf(pos)) - 2;
+      }
+      if (q == 0) s_mirView[vv] = make_float4(c.ca, c.sa, a.center - static_cast<float>(lo), 0.f);
+      if (q < NM) {
+        const int e = q == 0 ? 2 * view : c.mir[q - 1];
+        s_mirSeg[vv * NM + q] = make_int2((e & 1) ? twoC - 1 - lo : lo, e);
+      }
+    }
+  };
+  // the flat copy of windows [s0, s0 + nseg) of block M into buf: unit
+  // f = r * 256 + t of round r lands at byte f * kUnit (a wave's 64 units
+  // are contiguous); every offset is read from the block record before the
+  // first copy is issued
+  constexpr int kRounds = kMirBufBytes / (256 * kUnit);
+  auto issue = [&](int s0, int nseg, CT* buf) {
+    const int total = nseg * W * kUpc;
+    unsigned off[kRounds];
+    int seg = seg0, j = j0;
+#pragma unroll
+    for (int r = 0; r < kRounds; r++) {
+      off[r] = 0xFFFFFFF0u;  // beyond the resource: reads zero
+      if (r * 256 + t < total) {
+        const int2 sg = s_mirSeg[s0 + seg];
+        const int cell = (sg.y & 1) ? sg.x - j : sg.x + j;
+        if (static_cast<unsigned>(cell) < static_cast<unsigned>(nDet))
+          off[r] = static_cast<unsigned>(((sg.y >> 1) * nDet + cell) * static_cast<int>(sizeof(CT)) + part0);
+      }
+      seg += qW;
+      j += rW;
+      if (j >= W) {
+        j -= W;
+        seg++;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);  // every record read before the first copy
+#pragma unroll
+    for (int r = 0; r < kRounds; r++) {
+      if (r * 256 >= total) break;
+      lds_void* dst = (lds_void*)(reinterpret_cast<unsigned char*>(buf) + (r * 256 + wave * 64) * kUnit);
+      if constexpr (INTERP == INTERP_CUBIC)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, dst, 16, static_cast<int>(off[r]), 0, 0, 0);
+      else
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, dst, 4, static_cast<int>(off[r]), 0, 0, 0);
+    }
+  };
+  // views [b0, b0 + nv) of block M from buf (window vv * NM + m of the
+  // chunk at (vv * NM + m) * W)
+  auto compute = [&](int b0, int nv, const CT* buf) {
+    const float wmax = static_cast<float>(W - 1);
+    for (int vv = 0; vv < nv; vv++) {
+      // (a uniform index through readfirstlane and no read-ahead: no pointer
+      // induction variable, whose object the compiler could not tell)
+      const float4 c = s_mirView[__builtin_amdgcn_readfirstlane(b0 + vv)];
+      float u[P], u1[P], wg[P];
+      int j[P];
+#pragma unroll
+      for (int k = 0; k < P; k++) {
+        float pos;
+        pos_mixed<GK, DEG>(a, c.x, c.y, c.z, x, y[k], r2mF2[k], pos, wg[k]);
+        u[k] = __builtin_amdgcn_fractf(pos);
+        u1[k] = 1.f - u[k];
+        j[k] = static_cast<int>(__builtin_amdgcn_fmed3f(pos, 0.f, wmax));
+      }
+#pragma unroll
+      for (int m = 0; m < NM; m++) {
+        const int sgi = __builtin_amdgcn_readfirstlane((b0 + vv) * NM + m);
+        const int e = __builtin_amdgcn_readfirstlane(s_mirSeg[sgi].y);
+        const int wb = __builtin_amdgcn_readfirstlane((vv * NM + m) * W);
+        CT q[P];
+#pragma unroll
+        for (int k = 0; k < P; k++) q[k] = buf[wb + j[k]];
+        if (e & 1) {  // reflected window: the interval's polynomial at 1 - u
+#pragma unroll
+          for (int k = 0; k < P; k++) {
+            float val;
+            if constexpr (INTERP == INTERP_CUBIC)
+              val = fmaf(u1[k], fmaf(u1[k], fmaf(u1[k], q[k].w, q[k].z), q[k].y), q[k].x);
+            else
+              val = fmaf(u1[k], q[k].y, q[k].x);
+            part[m][k] = fmaf(val, wg[k], part[m][k]);
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < P; k++) {
+            float val;
+            if constexpr (INTERP == INTERP_CUBIC)
+              val = fmaf(u[k], fmaf(u[k], fmaf(u[k], q[k].w, q[k].z), q[k].y), q[k].x);
+            else
+              val = fmaf(u[k], q[k].y, q[k].x);
+            part[m][k] = fmaf(val, wg[k], part[m][k]);
+          }
+        }
+      }
+    }
+  };
+  auto kahan = [&]() {
+#pragma unroll
+    for (int m = 0; m < NM; m++)
+#pragma unroll
+      for (int k = 0; k < P; k++) {
+        const float yk = part[m][k] - comp[m][k];
+        const float tk = sum[m][k] + yk;
+        comp[m][k] = (tk - sum[m][k]) - yk;
+        sum[m][k] = tk;
+        part[m][k] = 0.f;
+      }
+  };
+  // one pipeline step: chunk (v0, nv) of the block at vb from cur while the
+  // block's next chunk goes to nxt; false when the block has no next chunk
+  int v0 = vBegin, nv = 0, vb = vBegin, pending = 0;
+  auto step = [&](const CT* cur, CT* nxt) -> bool {
+    const int v1 = v0 + nv, nv1 = v1 - vb < 64 ? min(a.chunk, vEnd - v1) : 0;
+    if (nv1 > 0) issue((v1 - vb) * NM, nv1 * NM, nxt);
+    compute(v0 - vb, nv, cur);
+    pending += nv;
+    if (pending >= 8 || v1 >= vEnd) {
+      kahan();
+      pending = 0;
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
+    __syncthreads();
+    v0 = v1;
+    nv = nv1;
+    return nv1 > 0;
+  };
+  for (vb = vBegin; vb < vEnd; vb += 64) {
+    // the block's windows, then its first chunk (not overlapped)
+    meta(vb, min(64, vEnd - vb));
+    __syncthreads();
+    v0 = vb;
+    nv = min(a.chunk, vEnd - vb);
+    issue(0, nv * NM, s_buf0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    while (step(s_buf0, s_buf1) && step(s_buf1, s_buf0)) {
+    }
+  }
+  if (ix >= a.domX) return;
+  float* out = a.split == 1 ? a.image : a.partial + static_cast<size_t>(grp) * a.nx * a.ny;
+  const int mx = a.nx - 1 - ix;
+#pragma unroll
+  for (int m = 0; m < NM; m++) {
+    const bool flipX = NM == 4 ? (m == 1 || m == 3) : m == 1;
+    const bool flipY = NM == 4 ? m >= 2 : (m == 1 && a.mapO);
+    if (flipX && mx == ix) continue;  // the middle column: member 0 has it
+    const int col = flipX ? mx : ix;
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+      const int iy = iyb + k;
+      if (iy >= a.domY) break;
+      const int row = flipY ? a.ny - 1 - iy : iy;
+      if (NM == 4 && flipY && row == iy) continue;  // the middle row
+      const size_t idx = static_cast<size_t>(col) * a.ny + row;
+      const float v = sum[m][k] - comp[m][k];
+      if (a.split > 1) out[idx] = v;
+      else out[idx] = a.overwrite ? a.endScale * v : fmaf(a.endScale, v, out[idx]);
+    }
+  }
+}
+
 // Position and weight of orbit member m (pixel R^m (x, y), R the quarter
 // turn (x, y) -> (-y, x)) in one view, from A = x ca + y sa and
 // B = x sa - y ca of the base pixel: the member's r cos and F + r sin are
@@ -383,7 +653,7 @@ __device__ __forceinline__ void pos_member(const MixedArgs& a, float A, float B,
     if constexpr (GK == 1) {
       const float L2 = fmaf(a.twoF, fl, r2mF2);
       const float rr = __builtin_amdgcn_rcpf(fl * L2);
-      pos = fan_angle<DEG>(a.atan, rc * (L2 * rr)) + cshift;
+      pos = fan_angle<DEG>(a.atan, rc * (L2 * rr), cshift);
       wgt = fl * rr;
     } else {
       const float q = __builtin_amdgcn_rcpf(fl);

@@ -108,8 +108,10 @@ extern "C" int ctsim_fbp_slice_static(ctsim_ctx* ctx, ctsim_procsignal* ps,
   {
     // the copy from pageable memory has read g_vcHost when it returns
     std::lock_guard<std::mutex> lock(g_mutex);
-    bp_view_consts_f32(p, h_angles, nView, g_vcHost);
+    const MirrorViews mv = bp_view_consts_f32(p, h_angles, nView, g_vcHost);
     sv = bp_sym_views(h_angles, nView);
+    sv.NM = mv.NM;
+    sv.mapO = mv.mapO;
     CTSIM_HIP(hipMemcpyAsync(d_vc, g_vcHost, sizeof(ViewConstF32) * nView, hipMemcpyHostToDevice,
                              ctx->stream));
   }

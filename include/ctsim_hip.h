@@ -147,6 +147,13 @@ int ctsim_fbp_slice_static(ctsim_ctx* ctx, ctsim_procsignal* ps, const ctsim_bac
 /* Geometry the backprojector derived: xMin, xMax, yMin, yMax, xInc, yInc,
  * rotScale, iDetCenter (8 doubles). */
 int ctsim_backprojector_info(const ctsim_backprojector* bp, double* out8);
+/* No counterpart in the reference (introspection of the MIXED mirror kernel).
+ * The mirror members these view angles allow on this backprojector's grid:
+ * 4 (every view has partners under x -> -x, y -> -y and both), 2 (under one
+ * of x -> -x and the point mirror) or 1 (none: the plain kernel runs).  Host
+ * computation only. */
+int ctsim_backprojector_mirror_members(const ctsim_backprojector* bp, const double* h_angles,
+                                       int nView);
 /* No counterpart in the reference (launch tuning of the MIXED kernel).
  * Times candidate shapes (lanes along y, rows per lane, view groups, views
  * per LDS chunk, plain or rotationally symmetric kernel and the latter's LDS
@@ -159,8 +166,10 @@ int ctsim_backprojector_autotune(ctsim_ctx* ctx, ctsim_backprojector* bp, const 
 /* Sets the MIXED launch shape explicitly: lanesY (8, 16, 32, 64), P (1, 2,
  * 4), split (1..64), chunk (1..64), sym (0; 1: the symmetric kernel where the
  * image is square and centred, the views come in 2 or 4 rotated copies, and
- * P <= 2; 2: its software-pipelined form, windows of at most 128 cells),
- * staging KB of the chunked symmetric kernel (0..62, 0: 62). */
+ * P <= 2; 2: its software-pipelined form, windows of at most 128 cells;
+ * 3: the mirror kernel where the grid is centred and every view has partner
+ * views under the grid's mirrors, 2 or 4 updates per position, else the
+ * plain kernel), staging KB of the chunked symmetric kernel (0..62, 0: 62). */
 int ctsim_backprojector_set_shape(ctsim_backprojector* bp, const int* shape6);
 
 /* ------------------------------------------------------------ filtering */
@@ -285,7 +294,8 @@ int ctsim_batch_status(ctsim_ctx* ctx, ctsim_batch* b);
  * rotationally symmetric body, when P <= 2 or in two-phase mode, where they
  * take P = 2 beside a P = 4 launch of the others, with each quadrant tile's
  * views in that many groups whose partial images are added after the
- * backprojection). */
+ * backprojection; + 8: in two-phase mode the other slices whose views have
+ * mirror partners run the mirror body). */
 int ctsim_batch_set_shape(ctsim_ctx* ctx, ctsim_batch* b, int P, int lanesY, int chunkBytes,
                           int fused, int sym);
 /* Times candidate shapes on these inputs and keeps the fastest; shape5 gets

@@ -3,8 +3,8 @@ single-slice paths.  Bars: every slice within d <= 1e-5 of the FP64
 single-slice image (Herman's d; the batch filters convolutions by FP32 FFT
 convolution, measured d ~ 1e-6) and within d <= 1e-4 of the single-slice
 mixed image (which filters convolutions by the BF16x3 Toeplitz product);
-every launch shape, with and without the rotationally symmetric body, gives
-the same images to FP32 summation-order noise;
+every launch shape, with and without the rotationally symmetric body and
+the mirror body, gives the same images to FP32 summation-order noise;
 graph replays are bitwise identical; new image buffers are picked up."""
 import numpy as np
 import pytest
@@ -78,7 +78,12 @@ def test_batch_matches_single_slice(gpu_ctx, batch_case):
                           (1, 16, 8192, True, 1), (2, 16, 32768, True, 0),
                           (1, 8, 16384, False, 1), (2, 8, 16384, True, 4),
                           (1, 16, 16384, False, 2), (2, 8, 32768, False, 4),
-                          (4, 8, 16384, False, 2)])
+                          (4, 8, 16384, False, 2),
+                          # + 8: the other slices (here 181 and 97 views: one
+                          # mirror, the rest four) run the mirror body
+                          (4, 8, 16384, False, 8), (2, 16, 32768, False, 9),
+                          (1, 8, 16384, False, 12), (4, 16, 32768, False, 10),
+                          (2, 32, 8192, False, 8)])
 def test_batch_shapes(gpu_ctx, batch_case, P, lanesY, chunk, fused, sym):
     import torch
     specs, b, sinos, images, ref64, refmx = batch_case
@@ -154,11 +159,12 @@ def test_batch_all_symmetric(gpu_ctx, P, fused, sym):
             assert d <= 1e-5
 
 
-@pytest.mark.parametrize("P", [1, 4])
-def test_batch_one_plain_class(gpu_ctx, P):
+@pytest.mark.parametrize("P,sym", [(1, 1), (4, 1), (4, 8), (2, 9)])
+def test_batch_one_plain_class(gpu_ctx, P, sym):
     """A batch whose slices share one plain-body class (no rotated view copies:
-    odd view counts) runs that class's own kernel in two-phase mode; each
-    slice against its FP64 single-slice image."""
+    odd view counts) runs that class's own kernel in two-phase mode (with
+    sym + 8 the mirror body: the views' only mirror is x -> -x, theta ->
+    -theta); each slice against its FP64 single-slice image."""
     import torch
     import ctsim_amd
     from ctsim_amd import fbp
@@ -167,12 +173,12 @@ def test_batch_one_plain_class(gpu_ctx, P):
                          nDet=300 + 40 * k, nView=181 + 20 * k, rotLen=2 * np.pi, filter="abs_hanning",
                          method="fft", interp="linear", backproj="idiff", nx=128 + 64 * k, ny=128 + 64 * k)
              for k in range(3)]
-    print(f"case one-plain-class batch P={P}", flush=True)
+    print(f"case one-plain-class batch P={P} sym={sym}", flush=True)
     sims = [fbp.simulate(gpu_ctx, s) for s in specs]
     ref64 = [fbp.reconstruct(gpu_ctx, s, sino, ang, info, ctsim_amd.PREC_F64)
              for s, (sino, ang, info) in zip(specs, sims)]
     b = ctsim_amd.Batch(gpu_ctx, [fbp.slice_desc(s, info, ang) for s, (_, ang, info) in zip(specs, sims)])
-    b.set_shape(P, 16, 16384, False, 1)
+    b.set_shape(P, 16, 16384, False, sym)
     images = [torch.full((s.nx, s.ny), 7.0, dtype=torch.float32, device="cuda") for s in specs]
     b.run([x[0] for x in sims], images)
     b.status()
