@@ -721,15 +721,16 @@ bool bp_sym_ok_for(const BpPlan& p, const MixedShape& m, const SymViews& sv) {
 
 int bp_mirror_chunk(const BpPlan& p, const MixedShape& m, int NM, int nView) {
   const bool cubic = p.interp == INTERP_CUBIC;
-  const int unit = cubic ? 16 : 4, upc = cubic ? 1 : 2;
   const int Wv = m.W > 0 ? m.W : p.nDet + 4;
   // one buffer resource spans every coefficient row
-  if (static_cast<double>(nView) * p.nDet * unit * upc >= 2147483648.0) return 0;
+  if (static_cast<double>(nView) * p.nDet * (cubic ? 16 : 8) >= 2147483648.0) return 0;
+  const int upw = dev::mirror_window_units(Wv, cubic);
   // a power of two up to 32 (a 64-view block of the kernel then takes an
-  // even number of chunks)
+  // even number of chunks) whose windows, copied in whole rounds of 256
+  // units, fit one buffer
   int c = 1;
   while (2 * c <= m.chunk && 2 * c <= 32) c *= 2;
-  while (c > 0 && dev::mirror_units(c, NM, Wv, upc) * unit > dev::kMirBufBytes) c /= 2;
+  while (c > 0 && (c * NM * upw + 255) / 256 * 256 * 16 > dev::kMirBufBytes) c /= 2;
   return c;
 }
 

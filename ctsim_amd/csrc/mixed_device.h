@@ -376,25 +376,25 @@ __device__ __forceinline__ void bp_mixed_tile(const MixedArgs& a, int tile, int 
 // not wait for the direct-to-LDS loads into the other).
 constexpr int kMirBufBytes = 16 * 1024;
 
-// LDS slots one staged chunk of the mirror body takes: nv * NM windows of W
-// cells in units of the direct-to-LDS load (one 16-byte cubic cell, or half
-// of an 8-byte linear cell), rounded up to whole rounds of 256 units.
-__host__ __device__ constexpr int mirror_units(int nv, int NM, int W, int unitsPerCell) {
-  return ((nv * NM * W * unitsPerCell + 255) / 256) * 256;
-}
+// Staging of one window of W cells in the mirror body: the direct-to-LDS
+// load unit (16 bytes) is one cubic cell or two linear cells (a pair
+// starting at an even cell).  Units per window:
+__host__ __device__ constexpr int mirror_window_units(int W, bool cubic) { return cubic ? W : (W + 1) / 2; }
 
 typedef __attribute__((address_space(3))) void lds_void;
 
 // The staging buffers and view-block records of the mirror body, shared by
 // all of its instantiations in a kernel: per block of up to 64 base views,
 // the view constants (ca, sa, centre - lo) and, per window (view vv, member m) at vv * NM + m, the
-// detector cell of its slot 0 and its row * 2 + reflected (slot j then holds
-// cell first + j, or first - j).  Plain arrays indexed from their own base,
+// detector cell of its slot 0 (low word) and its row * 2 + reflected (high
+// word; slot j then holds cell first + j, or first - j).  Plain arrays indexed from their own base,
 // so that the compiler can tell every LDS access's object.
 static __shared__ __attribute__((aligned(16))) unsigned char s_mirBuf0[kMirBufBytes];
 static __shared__ __attribute__((aligned(16))) unsigned char s_mirBuf1[kMirBufBytes];
 static __shared__ float4 s_mirView[64];
-static __shared__ int2 s_mirSeg[64 * 4];
+static __shared__ long long s_mirSeg[64 * 4];
+// reflected flags of the block's windows: bit 4 (vv % 16) + m of word vv / 16
+static __shared__ unsigned long long s_mirRefl[4];
 
 // The mirror body: the plain body with NM updates per position.  Member 0 is
 // the lane's pixel p in view v; member m > 0 is the mirror image M_m p in the
@@ -424,8 +424,6 @@ template <int GK, int INTERP, int P, int DEG, int NM>
 __device__ __forceinline__ void bp_mirror_tile(const MixedArgs& a, int tile, int grp) {
   using CT = CoefT<INTERP>;
   static_assert(NM == 2 || NM == 4, "two or four members");
-  constexpr int kUnit = INTERP == INTERP_CUBIC ? 16 : 4;  // bytes per direct-to-LDS load
-  constexpr int kUpc = static_cast<int>(sizeof(CT)) / kUnit;  // units per cell
   CT* const s_buf0 = reinterpret_cast<CT*>(s_mirBuf0);
   CT* const s_buf1 = reinterpret_cast<CT*>(s_mirBuf1);
   const int t = threadIdx.x;
@@ -456,16 +454,21 @@ __device__ __forceinline__ void bp_mirror_tile(const MixedArgs& a, int tile, int
   const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<CT*>(uniform_ptr(static_cast<const CT*>(a.coef))), 0,
       __builtin_amdgcn_readfirstlane(a.nView * nDet * static_cast<int>(sizeof(CT))), 0x00020000);
-  // this lane's first cell of the flat copy and its step per round
-  constexpr int kStep = 256 / kUpc;
-  const int cell0 = t / kUpc, part0 = (t % kUpc) * kUnit;
-  const int seg0 = cell0 / W, j0 = cell0 % W;
-  const int qW = kStep / W, rW = kStep % W;
+  // staging: windows of upw 16-byte units packed one after another (slot
+  // stride Wp cells; linear cells in pairs from an even cell); lane t's
+  // first unit is unit k0 of window seg0 and each round steps 256 units
+  constexpr bool kCubic = INTERP == INTERP_CUBIC;
+  const int upw = mirror_window_units(W, kCubic);
+  const int Wp = kCubic ? upw : 2 * upw;
+  const int seg0 = t / upw, k0 = t % upw, qU = 256 / upw, rU = 256 % upw;
+  constexpr int kPairFlip = kCubic ? 0 : 1;  // a reflected pair holds its two slots swapped
+  const int lane = t & 63;
 
   // windows of base views [vb, vb + nb), nb <= 64: lane (vv, q) = (t / 4,
   // t % 4) takes tile corner q; the four lanes' minimum gives lo
   auto meta = [&](int vb, int nb) {
     const int vv = t >> 2, q = t & 3;
+    bool refl = false;
     if (vv < nb) {
       const int view = a.vc[vb + vv].order;  // base views in orbit order
       const ViewConstF32 c = a.vc[view];
@@ -478,55 +481,95 @@ __device__ __forceinline__ void bp_mirror_tile(const MixedArgs& a, int tile, int
         pos = fminf(pos, __shfl_xor(pos, 2, 64));
         if (!(pos > -1e9f && pos < 1e9f)) pos = 0.f;
         lo = static_cast<int>(floorf(pos)) - 2;
+        if (!kCubic) lo &= ~1;  // pairs start at even cells (rounds down)
       }
       if (q == 0) s_mirView[vv] = make_float4(c.ca, c.sa, a.center - static_cast<float>(lo), 0.f);
       if (q < NM) {
         const int e = q == 0 ? 2 * view : c.mir[q - 1];
-        s_mirSeg[vv * NM + q] = make_int2((e & 1) ? twoC - 1 - lo : lo, e);
+        const unsigned first = static_cast<unsigned>((e & 1) ? twoC - 1 - lo : lo);
+        s_mirSeg[vv * NM + q] = static_cast<long long>((static_cast<unsigned long long>(e) << 32) | first);
+        refl = (e & 1) != 0;
       }
     }
+    // wave w holds views 16 w ... 16 w + 15 (lane 4 (vv % 16) + q)
+    const unsigned long long bits = __ballot(refl);
+    if (lane == 0) s_mirRefl[wave] = bits;
   };
-  // the flat copy of windows [s0, s0 + nseg) of block M into buf: unit
-  // f = r * 256 + t of round r lands at byte f * kUnit (a wave's 64 units
-  // are contiguous); every offset is read from the block record before the
-  // first copy is issued
-  constexpr int kRounds = kMirBufBytes / (256 * kUnit);
+  // the flat copy of windows [s0, s0 + nseg) of the block into buf: unit
+  // f = r * 256 + t of round r is unit f % upw of window f / upw and lands at
+  // byte 16 f (a wave's 64 units are contiguous).  Slot j of a direct window
+  // holds cell first + j, of a reflected one cell first - j (a reflected
+  // pair lands with its two slots swapped, read at j ^ 1).  With an odd nDet
+  // the pair (nDet - 1, nDet) straddles the row end: its lane copies the
+  // cell's two words and two zero words alone, one 4-byte copy each.
+  constexpr int kRounds = kMirBufBytes / (256 * 16);
   auto issue = [&](int s0, int nseg, CT* buf) {
-    const int total = nseg * W * kUpc;
-    unsigned off[kRounds];
-    int seg = seg0, j = j0;
-#pragma unroll
-    for (int r = 0; r < kRounds; r++) {
-      off[r] = 0xFFFFFFF0u;  // beyond the resource: reads zero
-      if (r * 256 + t < total) {
-        const int2 sg = s_mirSeg[s0 + seg];
-        const int cell = (sg.y & 1) ? sg.x - j : sg.x + j;
-        if (static_cast<unsigned>(cell) < static_cast<unsigned>(nDet))
-          off[r] = static_cast<unsigned>(((sg.y >> 1) * nDet + cell) * static_cast<int>(sizeof(CT)) + part0);
-      }
-      seg += qW;
-      j += rW;
-      if (j >= W) {
-        j -= W;
-        seg++;
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);  // every record read before the first copy
+    const int total = nseg * upw;
+    int seg = seg0, k = k0;
 #pragma unroll
     for (int r = 0; r < kRounds; r++) {
       if (r * 256 >= total) break;
-      lds_void* dst = (lds_void*)(reinterpret_cast<unsigned char*>(buf) + (r * 256 + wave * 64) * kUnit);
-      if constexpr (INTERP == INTERP_CUBIC)
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, dst, 16, static_cast<int>(off[r]), 0, 0, 0);
-      else
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, dst, 4, static_cast<int>(off[r]), 0, 0, 0);
+      unsigned off = 0xFFFFFFF0u;  // beyond the resource: reads zero
+      int edge = -1;               // byte offset of cell nDet - 1 of a straddling pair
+      if (r * 256 + t < total) {
+        const long long rec = s_mirSeg[s0 + seg];
+        const int first = static_cast<int>(rec), e = static_cast<int>(rec >> 32);
+        const bool refl = (e & 1) != 0;
+        const int rowBase = (e >> 1) * nDet;
+        if (kCubic) {
+          const int cm = refl ? first - k : first + k;
+          if (static_cast<unsigned>(cm) < static_cast<unsigned>(nDet))
+            off = static_cast<unsigned>((rowBase + cm) * 16);
+        } else {
+          const int cm = refl ? first - 2 * k - 1 : first + 2 * k;  // the pair's lower cell
+          if (cm >= 0 && cm + 1 < nDet) off = static_cast<unsigned>((rowBase + cm) * 8);
+          else if (cm + 1 == nDet) edge = (rowBase + cm) * 8;
+        }
+      }
+      unsigned char* base = reinterpret_cast<unsigned char*>(buf) + (r * 256 + wave * 64) * 16;
+      if (kCubic || edge < 0)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_void*)base, 16, static_cast<int>(off), 0, 0, 0);
+      if constexpr (!kCubic) {
+        // straddling pairs, one lane at a time: lane l writes at M0 + 4 l
+        unsigned long long todo = __ballot(edge >= 0);
+        while (todo) {
+          const int l = static_cast<int>(__builtin_ctzll(todo));
+          todo &= todo - 1;
+          const int eo = __builtin_amdgcn_readlane(edge, l);
+          if (lane == l) {
+            unsigned char* d = base + 16 * l - 4 * l;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_void*)d, 4, eo, 0, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_void*)(d + 4), 4, eo + 4, 0, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_void*)(d + 8), 4, static_cast<int>(0xFFFFFFF0u), 0, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_void*)(d + 12), 4, static_cast<int>(0xFFFFFFF0u), 0, 0, 0);
+          }
+        }
+      }
+      seg += qU;
+      k += rU;
+      if (k >= upw) {
+        k -= upw;
+        seg++;
+      }
     }
   };
   // views [b0, b0 + nv) of block M from buf (window vv * NM + m of the
   // chunk at (vv * NM + m) * W)
   auto compute = [&](int b0, int nv, const CT* buf) {
     const float wmax = static_cast<float>(W - 1);
+    // the chunk's reflected flags (a chunk of at most 32 views starts at a
+    // multiple of its size, so two words hold it)
+    const int w0 = b0 >> 4;
+    // (readfirstlane returns a signed int: widen through unsigned)
+    auto uniform64 = [](unsigned long long v) {
+      const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<unsigned>(v >> 32)));
+      const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<unsigned>(v)));
+      return (static_cast<unsigned long long>(hi) << 32) | lo;
+    };
+    const unsigned long long fl0 = uniform64(s_mirRefl[w0]), fl1 = uniform64(s_mirRefl[min(w0 + 1, 3)]);
     for (int vv = 0; vv < nv; vv++) {
+      const int bv = b0 + vv;
+      const unsigned vflags = static_cast<unsigned>(((bv >> 4) == w0 ? fl0 : fl1) >> (4 * (bv & 15)));
       // (a uniform index through readfirstlane and no read-ahead: no pointer
       // induction variable, whose object the compiler could not tell)
       const float4 c = s_mirView[__builtin_amdgcn_readfirstlane(b0 + vv)];
@@ -542,13 +585,11 @@ __device__ __forceinline__ void bp_mirror_tile(const MixedArgs& a, int tile, int
       }
 #pragma unroll
       for (int m = 0; m < NM; m++) {
-        const int sgi = __builtin_amdgcn_readfirstlane((b0 + vv) * NM + m);
-        const int e = __builtin_amdgcn_readfirstlane(s_mirSeg[sgi].y);
-        const int wb = __builtin_amdgcn_readfirstlane((vv * NM + m) * W);
+        const int wb = __builtin_amdgcn_readfirstlane((vv * NM + m) * Wp);
         CT q[P];
+        if ((vflags >> m) & 1) {  // reflected window: the interval's polynomial at 1 - u
 #pragma unroll
-        for (int k = 0; k < P; k++) q[k] = buf[wb + j[k]];
-        if (e & 1) {  // reflected window: the interval's polynomial at 1 - u
+          for (int k = 0; k < P; k++) q[k] = buf[wb + (j[k] ^ kPairFlip)];
 #pragma unroll
           for (int k = 0; k < P; k++) {
             float val;
@@ -559,6 +600,8 @@ __device__ __forceinline__ void bp_mirror_tile(const MixedArgs& a, int tile, int
             part[m][k] = fmaf(val, wg[k], part[m][k]);
           }
         } else {
+#pragma unroll
+          for (int k = 0; k < P; k++) q[k] = buf[wb + j[k]];
 #pragma unroll
           for (int k = 0; k < P; k++) {
             float val;
