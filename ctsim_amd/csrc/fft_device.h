@@ -401,10 +401,36 @@ __device__ __forceinline__ void fft_filter_pair(const float* __restrict__ in, co
 // of 256 from 512 points up).  Callers read the result through fft_slot.
 __host__ __device__ constexpr int fft_slot(int i) { return i ^ ((i >> 4) & 15); }
 
-// Lengths 2^6 ... 2^12 take these transforms; their LDS holds the N points and
-// the N-entry per-pass twiddle table (64 KB at 4096 points).
+// Lengths 2^6 ... 2^12 take these transforms; their LDS holds the N points
+// and, at 4096 points, the N-entry per-pass twiddle table (64 KB); up to 2048
+// points the twiddles are held in registers (RegTwiddles).
 __host__ __device__ constexpr bool fft_c_length(int logN) { return logN >= 6 && logN <= 12; }
-__host__ __device__ constexpr size_t fft_c_lds(int N) { return 2 * static_cast<size_t>(N) * sizeof(Cplx<float>); }
+__host__ __device__ constexpr size_t fft_c_lds(int N) {
+  return (N <= 2048 ? 1 : 2) * static_cast<size_t>(N) * sizeof(Cplx<float>);
+}
+
+// Twiddles of the compile-time transforms up to 2048 points in registers:
+// each radix-8 pass with span Ls > 1 has at most one butterfly per lane
+// (N / 8 <= 256), which needs w^(r k N / (8 Ls)), r = 1..7, k = lane mod Ls;
+// the forward transform uses them and the inverse their conjugates.  Loaded
+// once from the global half table, they keep the twiddle table out of LDS
+// and its reads out of the passes.
+template <int LOGN>
+struct RegTwiddles {
+  static constexpr int kL0 = fft8_first_radix(LOGN) == 2 ? 1 : (fft8_first_radix(LOGN) == 4 ? 2 : 3);
+  static constexpr int kPasses = LOGN > kL0 ? (LOGN - kL0 + 2) / 3 : 0;
+  Cplx<float> w[kPasses > 0 ? kPasses : 1][7];
+  __device__ __forceinline__ void load(const Cplx<float>* __restrict__ twg) {
+    constexpr int N = 1 << LOGN;
+#pragma unroll
+    for (int p = 0; p < kPasses; p++) {
+      const int lg = kL0 + 3 * p;
+      const int k = threadIdx.x & ((1 << lg) - 1);
+#pragma unroll
+      for (int r = 1; r < 8; r++) w[p][r - 1] = twiddle(twg, (r * k) << (LOGN - 3 - lg), N / 2, -1);
+    }
+  }
+};
 
 template <int DIR, class T>
 __device__ __forceinline__ void dft4c(Cplx<T>& c0, Cplx<T>& c1, Cplx<T>& c2, Cplx<T>& c3) {
@@ -448,7 +474,7 @@ __device__ __forceinline__ void dft_smallc(Cplx<T>* a) {
 }
 
 template <int DIR, int R, int LOGN, int LG, int OFF, class T>
-__device__ __forceinline__ void radix_passc(Cplx<T>* z, const Cplx<T>* tw) {
+__device__ __forceinline__ void radix_passc(Cplx<T>* z, const Cplx<T>* tw, const RegTwiddles<LOGN>* rt) {
   constexpr int N = 1 << LOGN, Ls = 1 << LG, nb = N / R;
   constexpr int NB = nb >= 256 ? nb / 256 : 1;
   Cplx<T> y[NB][R];
@@ -464,7 +490,9 @@ __device__ __forceinline__ void radix_passc(Cplx<T>* z, const Cplx<T>* tw) {
       if constexpr (Ls > 1) {
 #pragma unroll
         for (int r = 1; r < R; r++) {
-          Cplx<T> w = tw[OFF + (r - 1) * Ls + k];
+          Cplx<T> w;
+          if constexpr (LOGN <= 11) w = rt->w[(LG - RegTwiddles<LOGN>::kL0) / 3][r - 1];
+          else w = tw[OFF + (r - 1) * Ls + k];
           if constexpr (DIR > 0) w.y = -w.y;
           y[b][r] = cmul(w, y[b][r]);
         }
@@ -486,29 +514,31 @@ __device__ __forceinline__ void radix_passc(Cplx<T>* z, const Cplx<T>* tw) {
 
 // radix-8 passes with span 2^LG, 2^(LG+3), ... < N; table offset OFF
 template <int DIR, int LOGN, int LG, int OFF, class T>
-__device__ __forceinline__ void radix8_passesc(Cplx<T>* z, const Cplx<T>* tw) {
+__device__ __forceinline__ void radix8_passesc(Cplx<T>* z, const Cplx<T>* tw, const RegTwiddles<LOGN>* rt) {
   if constexpr (LG < LOGN) {
-    radix_passc<DIR, 8, LOGN, LG, OFF>(z, tw);
-    radix8_passesc<DIR, LOGN, LG + 3, (LG > 0 ? OFF + 7 * (1 << LG) : OFF)>(z, tw);
+    radix_passc<DIR, 8, LOGN, LG, OFF>(z, tw, rt);
+    radix8_passesc<DIR, LOGN, LG + 3, (LG > 0 ? OFF + 7 * (1 << LG) : OFF)>(z, tw, rt);
   }
 }
 
+// tw8: the LDS table (4096 points) or, up to 2048 points, unused (rt)
 template <int DIR, int LOGN, class T>
-__device__ __forceinline__ void stockham8c(Cplx<T>* z, const Cplx<T>* tw8) {
+__device__ __forceinline__ void stockham8c(Cplx<T>* z, const Cplx<T>* tw8, const RegTwiddles<LOGN>* rt) {
   constexpr int first = fft8_first_radix(LOGN);
   if constexpr (first == 2) {
-    radix_passc<DIR, 2, LOGN, 0, 0>(z, tw8);
-    radix8_passesc<DIR, LOGN, 1, 0>(z, tw8);
+    radix_passc<DIR, 2, LOGN, 0, 0>(z, tw8, rt);
+    radix8_passesc<DIR, LOGN, 1, 0>(z, tw8, rt);
   } else if constexpr (first == 4) {
-    radix_passc<DIR, 4, LOGN, 0, 0>(z, tw8);
-    radix8_passesc<DIR, LOGN, 2, 0>(z, tw8);
+    radix_passc<DIR, 4, LOGN, 0, 0>(z, tw8, rt);
+    radix8_passesc<DIR, LOGN, 2, 0>(z, tw8, rt);
   } else {
-    radix8_passesc<DIR, LOGN, 0, 0>(z, tw8);
+    radix8_passesc<DIR, LOGN, 0, 0>(z, tw8, rt);
   }
 }
 
-// fft_filter_pair for FP32 with N = 2^LOGN <= 4096 at compile time (tw holds
-// N entries); filtered cell j of the rows is z[fft_slot(j)].x (va) and .y (vb).
+// fft_filter_pair for FP32 with N = 2^LOGN <= 4096 at compile time (at 4096
+// points tw holds the N-entry table; up to 2048 tw is unused); filtered cell
+// j of the rows is z[fft_slot(j)].x (va) and .y (vb).
 template <int LOGN>
 __device__ __forceinline__ void fft_filter_pair_c(const float* __restrict__ in, const double* __restrict__ w,
                                                   const float* __restrict__ hre, const float* __restrict__ him,
@@ -540,7 +570,9 @@ __device__ __forceinline__ void fft_filter_pair_c(const float* __restrict__ in, 
       mi[i] = him[k];
     }
   }
-  pass_twiddles8<float, (N > 2048 ? 16 : 8)>(twg, tw, N, LOGN);
+  RegTwiddles<LOGN> rt;
+  if constexpr (LOGN <= 11) rt.load(twg);
+  else pass_twiddles8<float, 16>(twg, tw, N, LOGN);
 #pragma unroll
   for (int i = 0; i < IT; i++) {
     const int j = threadIdx.x + i * 256;
@@ -554,7 +586,7 @@ __device__ __forceinline__ void fft_filter_pair_c(const float* __restrict__ in, 
     }
   }
   __syncthreads();
-  stockham8c<-1, LOGN>(z, tw);
+  stockham8c<-1, LOGN>(z, tw, &rt);
 #pragma unroll
   for (int i = 0; i < HT; i++) {
     const int k = threadIdx.x + i * 256;
@@ -571,7 +603,7 @@ __device__ __forceinline__ void fft_filter_pair_c(const float* __restrict__ in, 
     if (kn != k) z[fft_slot(kn)] = wn;
   }
   __syncthreads();
-  stockham8c<+1, LOGN>(z, tw);
+  stockham8c<+1, LOGN>(z, tw, &rt);
 }
 
 }  // namespace dev
