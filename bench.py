@@ -218,12 +218,15 @@ def main():
                            spec.focalRatio, spec.centerRatio)
     info = sc.info
     sino = torch.empty((spec.nView, spec.nDet), dtype=torch.float32, device=dev)
+    sino_m = torch.empty_like(sino)
+    # a first call of each precision uploads the elements; the second is timed
+    ang = sc.collect(ctx, sino, ctsim_amd.PREC_F64)
+    sc.collect(ctx, sino_m, ctsim_amd.PREC_MIXED)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ang = sc.collect(ctx, sino, ctsim_amd.PREC_F64)
     torch.cuda.synchronize()
     t_raysum64 = time.perf_counter() - t0
-    sino_m = torch.empty_like(sino)
     t0 = time.perf_counter()
     sc.collect(ctx, sino_m, ctsim_amd.PREC_MIXED)
     torch.cuda.synchronize()

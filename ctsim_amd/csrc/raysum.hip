@@ -20,6 +20,8 @@
 //               reference's clip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "crmath.h"
 #include "scan_plan.h"
 
@@ -54,6 +56,7 @@ __device__ __forceinline__ ViewSetup view_setup(const ViewEnds& e, int nDet, uns
 struct WalkArgs {
   const ViewEnds* ends;
   double4* walk;  // [nView][nDet]: xs_maj, ys_maj, xd_maj (or angle), yd_maj
+  int* counter;   // the ray-sum kernel's work counter, zeroed here
   int nView, nDet, geom;
   unsigned nSample;
   double angDetInc, angDetLen;
@@ -61,6 +64,7 @@ struct WalkArgs {
 
 // The detector walk of projectSingleView (scanner.cpp:449-531).
 __global__ void raysum_walk_kernel(WalkArgs a) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a.counter = 0;
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= a.nView) return;
   const ViewEnds e = a.ends[v];
@@ -215,50 +219,94 @@ __device__ bool clip_triangle(double& x1, double& y1, double& x2, double& y2, do
   return true;
 }
 
-// clip_segment, libctsupport/clip.cpp:44-77
-__device__ bool clip_segment(double& x1, double& y1, double& x2, double& y2, double u, double v) {
-  double xc1 = x1 * u, yc1 = y1 * v, xc2 = x2 * u, yc2 = y2 * v;
-  if (yc1 > 0. && yc2 > 0.) return false;
-  const double radius = sqrt(u * u + v * v);
-  if (!clip_circle(xc1, yc1, xc2, yc2, 0.0, v, radius, 0.0, 0.0)) return false;
-  if (yc1 > 0. && yc2 > 0.) return false;
-  if (yc1 > 0.) {
-    xc1 = xc1 + (xc2 - xc1) * (0.0 - yc1) / (yc2 - yc1);
-    yc1 = 0.0;
-  } else if (yc2 > 0.) {
-    xc2 = xc1 + (xc2 - xc1) * (0.0 - yc1) / (yc2 - yc1);
-    yc2 = 0.0;
-  }
-  x1 = xc1 / u; y1 = yc1 / v; x2 = xc2 / u; y2 = yc2 / v;
-  return true;
+// clip_segment (libctsupport/clip.cpp:44-77) and clip_sector (clip.cpp:
+// 90-111) share the ellipse's clip_circle call below, so the kernel holds
+// one copy of the circle clip (and its atan2, sin and cos): each case does
+// the same operations in the same order as the reference.
+
+// False only when the segment certainly misses the box: separating axes
+// (the box's two axes and the segment's normal) on the box widened by a
+// margin far above the rounding of the reference's Cohen-Sutherland clip,
+// so every segment that clip keeps passes.  No division.
+__device__ __forceinline__ bool seg_may_hit_box(double x1, double y1, double x2, double y2, const double* r) {
+  const double scale = fmax(fmax(fabs(r[0]), fabs(r[2])), fmax(fmax(fabs(r[1]), fabs(r[3])), 1.0));
+  const double m = 1e-9 * scale;
+  const double bx0 = r[0] - m, by0 = r[1] - m, bx1 = r[2] + m, by1 = r[3] + m;
+  if (fmax(x1, x2) < bx0 || fmin(x1, x2) > bx1 || fmax(y1, y2) < by0 || fmin(y1, y2) > by1) return false;
+  const double nx = y1 - y2, ny = x2 - x1;  // the segment's normal
+  const double s0 = nx * (bx0 - x1) + ny * (by0 - y1), s1 = nx * (bx1 - x1) + ny * (by0 - y1);
+  const double s2 = nx * (bx0 - x1) + ny * (by1 - y1), s3 = nx * (bx1 - x1) + ny * (by1 - y1);
+  const double tol = m * (fabs(nx) + fabs(ny));
+  const double lo = fmin(fmin(s0, s1), fmin(s2, s3)), hi = fmax(fmax(s0, s1), fmax(s2, s3));
+  return lo <= tol && hi >= -tol;
 }
 
-// clip_sector, libctsupport/clip.cpp:90-111
-__device__ bool clip_sector(double& x1, double& y1, double& x2, double& y2, double u, double v) {
-  double xc1 = x1 * u, yc1 = y1 * v, xc2 = x2 * u, yc2 = y2 * v;
-  const double radius = sqrt(u * u + v * v);
-  if (!clip_circle(xc1, yc1, xc2, yc2, 0.0, v, radius, 0.0, 0.0)) return false;
-  if (!clip_triangle(xc1, yc1, xc2, yc2, u, v, false)) return false;
-  x1 = xc1 / u; y1 = yc1 / v; x2 = xc2 / u; y2 = yc2 / v;
-  return true;
+// True only when the segment certainly crosses the box: the same test on the
+// box narrowed by that margin, with strict inequalities.  Such a segment is
+// kept by the reference's Cohen-Sutherland clip, whose clipped points the
+// chord discards, so the clip can be skipped.
+__device__ __forceinline__ bool seg_surely_hits_box(double x1, double y1, double x2, double y2, const double* r) {
+  const double scale = fmax(fmax(fabs(r[0]), fabs(r[2])), fmax(fmax(fabs(r[1]), fabs(r[3])), 1.0));
+  const double m = 1e-9 * scale;
+  const double bx0 = r[0] + m, by0 = r[1] + m, bx1 = r[2] - m, by1 = r[3] - m;
+  if (!(bx0 < bx1 && by0 < by1)) return false;
+  if (!(fmax(x1, x2) > bx0 && fmin(x1, x2) < bx1 && fmax(y1, y2) > by0 && fmin(y1, y2) < by1)) return false;
+  const double nx = y1 - y2, ny = x2 - x1;
+  const double s0 = nx * (bx0 - x1) + ny * (by0 - y1), s1 = nx * (bx1 - x1) + ny * (by0 - y1);
+  const double s2 = nx * (bx0 - x1) + ny * (by1 - y1), s3 = nx * (bx1 - x1) + ny * (by1 - y1);
+  const double tol = m * (fabs(nx) + fabs(ny));
+  const double lo = fmin(fmin(s0, s1), fmin(s2, s3)), hi = fmax(fmax(s0, s1), fmax(s2, s3));
+  return lo < -tol && hi > tol;
 }
 
 // PhantomElement::clipLineWorldCoords / clipLineNormalizedCoords and
 // Scanner::projectLineAgainstPElem, phantom.cpp:830-898, scanner.cpp:633-656
-__device__ double chord_ref(const ElemGPU& e, double x1, double y1, double x2, double y2) {
-  double cx1 = x1, cy1 = y1, cx2 = x2, cy2 = y2;
-  if (!clip_rect(cx1, cy1, cx2, cy2, e.lim)) return 0.0;
+__device__ __forceinline__ double chord_ref(const ElemGPU& e, double x1, double y1, double x2, double y2) {
+  if (!seg_surely_hits_box(x1, y1, x2, y2, e.lim)) {  // else the box clip keeps the segment
+    double cx1 = x1, cy1 = y1, cx2 = x2, cy2 = y2;
+    if (!clip_rect(cx1, cy1, cx2, cy2, e.lim)) return 0.0;
+  }
   xf(e.p2o, x1, y1);
   xf(e.p2o, x2, y2);
   bool ok = false;
-  const double unit[4] = {-1.0, -1.0, 1.0, 1.0};
-  switch (e.type) {
-    case PE_RECT: ok = clip_rect(x1, y1, x2, y2, unit); break;
-    case PE_ELLIPSE: ok = clip_circle(x1, y1, x2, y2, 0.0, 0.0, 1.0, 0.0, 0.0); break;
-    case PE_TRI: ok = clip_triangle(x1, y1, x2, y2, 1.0, 1.0, true); break;
-    case PE_SEGMENT: ok = clip_segment(x1, y1, x2, y2, e.u, e.v); break;
-    case PE_SECTOR: ok = clip_sector(x1, y1, x2, y2, e.u, e.v); break;
-    default: break;
+  const int type = e.type;
+  if (type == PE_RECT) {
+    const double unit[4] = {-1.0, -1.0, 1.0, 1.0};
+    ok = clip_rect(x1, y1, x2, y2, unit);
+  } else if (type == PE_TRI) {
+    ok = clip_triangle(x1, y1, x2, y2, 1.0, 1.0, true);
+  } else if (type == PE_ELLIPSE || type == PE_SEGMENT || type == PE_SECTOR) {
+    const bool scaled = type != PE_ELLIPSE;  // segment, sector: x * u, y * v
+    const double u = e.u, v = e.v;
+    double xc1 = x1, yc1 = y1, xc2 = x2, yc2 = y2;
+    double ccy = 0.0, radius = 1.0;
+    ok = true;
+    if (scaled) {
+      xc1 = x1 * u; yc1 = y1 * v; xc2 = x2 * u; yc2 = y2 * v;
+      if (type == PE_SEGMENT && yc1 > 0. && yc2 > 0.) ok = false;
+      radius = sqrt(u * u + v * v);
+      ccy = v;
+    }
+    if (ok) ok = clip_circle(xc1, yc1, xc2, yc2, 0.0, ccy, radius, 0.0, 0.0);
+    if (ok && type == PE_SEGMENT) {
+      if (yc1 > 0. && yc2 > 0.) {
+        ok = false;
+      } else if (yc1 > 0.) {
+        xc1 = xc1 + (xc2 - xc1) * (0.0 - yc1) / (yc2 - yc1);
+        yc1 = 0.0;
+      } else if (yc2 > 0.) {
+        xc2 = xc1 + (xc2 - xc1) * (0.0 - yc1) / (yc2 - yc1);
+        yc2 = 0.0;
+      }
+    }
+    if (ok && type == PE_SECTOR) ok = clip_triangle(xc1, yc1, xc2, yc2, u, v, false);
+    if (ok) {
+      if (scaled) {
+        x1 = xc1 / u; y1 = yc1 / v; x2 = xc2 / u; y2 = yc2 / v;
+      } else {
+        x1 = xc1; y1 = yc1; x2 = xc2; y2 = yc2;
+      }
+    }
   }
   if (!ok) return 0.0;
   xf(e.o2p, x1, y1);
@@ -349,24 +397,92 @@ struct RaysumArgs {
   const double4* walk;
   const ElemGPU* __restrict__ elems;
   float* sino;
+  int* counter;  // items taken so far (zeroed by the walk kernel)
   int nView, nDet, nElem, geom, storageOffset;
   unsigned nSample;
   double angDetInc, centerDetLength;
   int unitPulse;
 };
 
+// One lane's ray sum: detector d of view v.
+// Detector end (a) and source end (b) of sample i of detector d's ray, as
+// raysum_ray walks them (to rounding: the hull test below has a margin).
+__device__ __forceinline__ void ray_ends(const RaysumArgs& a, const ViewSetup& s, int v, int d, unsigned i,
+                                         double& ax, double& ay, double& bx, double& by) {
+  const double4 w = a.walk[static_cast<size_t>(v) * a.nDet + d];
+  bx = w.x;
+  by = w.y;
+  if (a.geom == GEOM_EQUIANGULAR) {
+    const double ang = w.z + i * s.aSampInc;
+    ax = a.centerDetLength * cos(ang);
+    ay = a.centerDetLength * sin(ang);
+  } else {
+    ax = w.z + i * s.ddx2;
+    ay = w.w + i * s.ddy2;
+  }
+}
+
+// Candidate elements of one wave's rays (detectors d0..d1 of view v, every
+// sample): lane k tests element 64 c + k's box against the hull of the
+// rays, the quadrilateral of the first and the last ray (widened by the
+// sagitta of the detector arc in the equiangular geometry and a margin), by
+// separating axes; bit k of mask c is set unless the box is certainly
+// outside.  Every ray of the wave lies in the hull, so a clear bit marks an
+// element whose chords are all exactly zero.
+constexpr int kCandChunks = 16;  // up to 1024 elements; more take no culling
+__device__ __forceinline__ void wave_candidates(const RaysumArgs& a, const ViewSetup& s, int v, int d0, int d1,
+                                                unsigned long long* masks, int nChunks) {
+  double px[4], py[4];
+  ray_ends(a, s, v, d0, 0, px[0], py[0], px[1], py[1]);
+  ray_ends(a, s, v, d1, a.nSample - 1, px[3], py[3], px[2], py[2]);
+  double mg = 1e-9 * fmax(fmax(fmax(fabs(px[0]), fabs(px[1])), fmax(fabs(px[2]), fabs(px[3]))),
+                          fmax(fmax(fmax(fabs(py[0]), fabs(py[1])), fmax(fabs(py[2]), fabs(py[3]))), 1.0));
+  if (a.geom == GEOM_EQUIANGULAR) {  // the detector ends lie on an arc bulging past the chord
+    const double cx = px[3] - px[0], cy = py[3] - py[0];
+    mg += (cx * cx + cy * cy) / (2.0 * a.centerDetLength) + 1e-12;
+  }
+  const double qx0 = fmin(fmin(px[0], px[1]), fmin(px[2], px[3])), qx1 = fmax(fmax(px[0], px[1]), fmax(px[2], px[3]));
+  const double qy0 = fmin(fmin(py[0], py[1]), fmin(py[2], py[3])), qy1 = fmax(fmax(py[0], py[1]), fmax(py[2], py[3]));
+  const int lane = threadIdx.x;
+  for (int c = 0; c < nChunks; c++) {
+    const int k = 64 * c + lane;
+    bool cand = false;
+    if (k < a.nElem) {
+      const double* r = a.elems[k].lim;
+      cand = !(qx1 < r[0] - mg || qx0 > r[2] + mg || qy1 < r[1] - mg || qy0 > r[3] + mg);
+#pragma unroll
+      for (int ed = 0; ed < 4 && cand; ed++) {
+        const int e2 = (ed + 1) & 3;
+        const double nx = py[ed] - py[e2], ny = px[e2] - px[ed];
+        double h0 = 1e300, h1 = -1e300;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const double p = nx * px[j] + ny * py[j];
+          h0 = fmin(h0, p);
+          h1 = fmax(h1, p);
+        }
+        const double c00 = nx * r[0] + ny * r[1], c10 = nx * r[2] + ny * r[1];
+        const double c01 = nx * r[0] + ny * r[3], c11 = nx * r[2] + ny * r[3];
+        const double b0 = fmin(fmin(c00, c10), fmin(c01, c11)), b1 = fmax(fmax(c00, c10), fmax(c01, c11));
+        const double tol = mg * (fabs(nx) + fabs(ny)) + 1e-300;
+        if (b1 < h0 - tol || b0 > h1 + tol) cand = false;
+      }
+    }
+    const unsigned long long bits = __ballot(cand);
+    if (lane == 0) masks[c] = bits;
+  }
+}
+
+// masks: the nChunks candidate masks (wave_candidates), or nullptr for
+// every element
 template <bool EXACT>
-__global__ __launch_bounds__(256) void raysum_kernel(RaysumArgs a) {
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
-  const int v = blockIdx.y;
-  if (d >= a.nDet) return;
+__device__ __forceinline__ void raysum_ray(const RaysumArgs& a, const ViewSetup& s, int v, int d,
+                                           const unsigned long long* masks, int nChunks) {
   float* out = a.sino + static_cast<size_t>(v + a.storageOffset) * a.nDet;
   if (a.unitPulse) {  // scanner.cpp:475-478
     out[d] = d == a.nDet / 2 ? 1.f : 0.f;
     return;
   }
-  const ViewEnds e = a.ends[v];
-  const ViewSetup s = view_setup(e, a.nDet, a.nSample, a.geom, a.angDetInc);
   const double4 w = a.walk[static_cast<size_t>(v) * a.nDet + d];
   const double xs = w.x, ys = w.y;
   double xd = w.z, yd = w.w, ang = w.z;
@@ -378,19 +494,62 @@ __global__ __launch_bounds__(256) void raysum_kernel(RaysumArgs a) {
       xd = a.centerDetLength * ca;
       yd = a.centerDetLength * sa;
     }
-    // projectSingleLine: the element list is newest-first (phantom.cpp:247)
+    // projectSingleLine: the element list is newest-first (phantom.cpp:247).
+    // A segment that misses an element's box adds the reference's exact
+    // zero, which changes no sum: such pairs are skipped (elements outside
+    // the wave's hull by the masks, then lane by lane), in the same order.
     double rsum = 0.0;
-    if (EXACT) {
-      for (int k = a.nElem - 1; k >= 0; k--) rsum += chord_ref(a.elems[k], xd, yd, xs, ys);
-    } else {
-      const double segLen = sqrt((xs - xd) * (xs - xd) + (ys - yd) * (ys - yd));
-      for (int k = a.nElem - 1; k >= 0; k--) rsum += chord_closed(a.elems[k], xd, yd, xs, ys, segLen);
+    const double segLen = EXACT ? 0.0 : sqrt((xs - xd) * (xs - xd) + (ys - yd) * (ys - yd));
+    for (int c = nChunks - 1; c >= 0; c--) {
+      unsigned long long m;
+      if (masks) {
+        const unsigned long long mc = masks[c];
+        m = (static_cast<unsigned long long>(static_cast<unsigned>(
+                 __builtin_amdgcn_readfirstlane(static_cast<unsigned>(mc >> 32)))) << 32) |
+            static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<unsigned>(mc)));
+      } else {  // every element of the chunk
+        const int rem = a.nElem - 64 * c;
+        m = rem >= 64 ? ~0ull : (1ull << rem) - 1;
+      }
+      while (m) {
+        const int b = 63 - __builtin_clzll(m);
+        m &= ~(1ull << b);
+        const int k = 64 * c + b;
+        const bool may = seg_may_hit_box(xd, yd, xs, ys, a.elems[k].lim);
+        if (!__any(may)) continue;
+        if (may) rsum += EXACT ? chord_ref(a.elems[k], xd, yd, xs, ys)
+                               : chord_closed(a.elems[k], xd, yd, xs, ys, segLen);
+      }
     }
     sum += rsum;
     if (a.geom == GEOM_EQUIANGULAR) ang += s.aSampInc;
     else { xd += s.ddx2; yd += s.ddy2; }
   }
   out[d] = static_cast<float>(sum / a.nSample);
+}
+
+// A persistent wave per block takes items (view, 64 detectors) from a
+// counter until none is left: the rays' work differs widely (a ray through
+// the phantom's middle meets many elements), and taking items as waves free
+// up keeps every CU busy to the end.
+template <bool EXACT>
+__global__ __launch_bounds__(64) void raysum_kernel(RaysumArgs a) {
+  const int blocks = (a.nDet + 63) / 64, nItems = blocks * a.nView;
+  const int nChunks = (a.nElem + 63) / 64;
+  const bool cull = nChunks <= kCandChunks && !a.unitPulse;
+  __shared__ unsigned long long masks[kCandChunks];  // the wave's candidate masks
+  while (true) {
+    int item = 0;
+    if (threadIdx.x == 0) item = atomicAdd(a.counter, 1);
+    item = __builtin_amdgcn_readfirstlane(item);
+    if (item >= nItems) break;
+    const int v = item / blocks, d0 = (item - v * blocks) * 64;
+    const int d = d0 + static_cast<int>(threadIdx.x), d1 = min(d0 + 63, a.nDet - 1);
+    const ViewEnds e = a.ends[v];
+    const ViewSetup s = view_setup(e, a.nDet, a.nSample, a.geom, a.angDetInc);
+    if (cull) wave_candidates(a, s, v, d0, d1, masks, nChunks);
+    if (d < a.nDet) raysum_ray<EXACT>(a, s, v, d, cull ? masks : nullptr, nChunks);
+  }
 }
 
 }  // namespace
@@ -413,19 +572,29 @@ int raysum_launch(ctsim_ctx* ctx, const ScannerPlan& s, PhantomPlan& p, int prec
   if (h_angles)
     for (int v = 0; v < iNumViews; v++) h_angles[v + iStorageOffset] = ends[v].viewAngle;
   CTSIM_CHECK(ctx->stage.upload(ctx->viewConsts, eb, st));
-  CTSIM_CHECK(ctx->scratch.reserve(static_cast<size_t>(iNumViews) * s.nDet * sizeof(double4)));
-  WalkArgs wa{ctx->viewConsts.as<ViewEnds>(), ctx->scratch.as<double4>(), iNumViews, s.nDet,
+  // the walk, then the ray-sum kernel's work counter
+  const size_t walkBytes = static_cast<size_t>(iNumViews) * s.nDet * sizeof(double4);
+  CTSIM_CHECK(ctx->scratch.reserve(walkBytes + 256));
+  int* counter = reinterpret_cast<int*>(static_cast<unsigned char*>(ctx->scratch.ptr) + walkBytes);
+  WalkArgs wa{ctx->viewConsts.as<ViewEnds>(), ctx->scratch.as<double4>(), counter, iNumViews, s.nDet,
               s.geom, s.nSample, s.angDetInc, s.angDetLen};
   hipLaunchKernelGGL(raysum_walk_kernel, dim3((iNumViews + 63) / 64), dim3(64), 0, st, wa);
   CTSIM_HIP(hipGetLastError());
   RaysumArgs ra{ctx->viewConsts.as<ViewEnds>(), ctx->scratch.as<double4>(), p.d_elems.as<ElemGPU>(),
-                d_sino, iNumViews, s.nDet, static_cast<int>(p.elems.size()), s.geom, iStorageOffset,
+                d_sino, counter, iNumViews, s.nDet, static_cast<int>(p.elems.size()), s.geom, iStorageOffset,
                 s.nSample, s.angDetInc, s.centerDetLength, p.unitPulse ? 1 : 0};
-  const dim3 grid((s.nDet + 255) / 256, iNumViews);
+  // persistent single-wave blocks, as many as are resident at once, no
+  // more than the items
+  const int items = ((s.nDet + 63) / 64) * iNumViews;
+  const void* kern = precision == CTSIM_PREC_F64 ? reinterpret_cast<const void*>(raysum_kernel<true>)
+                                                 : reinterpret_cast<const void*>(raysum_kernel<false>);
+  int perCU = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, 64, 0) != hipSuccess || perCU < 1) perCU = 8;
+  const dim3 grid(std::min(items, ctx->numCUs * perCU));
   if (precision == CTSIM_PREC_F64)
-    hipLaunchKernelGGL((raysum_kernel<true>), grid, dim3(256), 0, st, ra);
+    hipLaunchKernelGGL((raysum_kernel<true>), grid, dim3(64), 0, st, ra);
   else
-    hipLaunchKernelGGL((raysum_kernel<false>), grid, dim3(256), 0, st, ra);
+    hipLaunchKernelGGL((raysum_kernel<false>), grid, dim3(64), 0, st, ra);
   CTSIM_HIP(hipGetLastError());
   return CTSIM_OK;
 }
