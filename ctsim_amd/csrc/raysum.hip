@@ -55,36 +55,54 @@ __device__ __forceinline__ ViewSetup view_setup(const ViewEnds& e, int nDet, uns
 
 struct WalkArgs {
   const ViewEnds* ends;
-  double4* walk;  // [nView][nDet]: xs_maj, ys_maj, xd_maj (or angle), yd_maj
+  double4* walk;  // [nView][nDet]: xs, ys, xd (or the detector angle), yd
   int* counter;   // the ray-sum kernel's work counter, zeroed here
   int nView, nDet, geom;
   unsigned nSample;
   double angDetInc, angDetLen;
 };
 
-// The detector walk of projectSingleView (scanner.cpp:449-531).
-__global__ void raysum_walk_kernel(WalkArgs a) {
+// The detector walk of projectSingleView (scanner.cpp:449-531): per view
+// four chains of rounded sums (xs, ys, and xd, yd or the detector angle),
+// one lane per chain, 16 views per block.  Each run of 64 detectors is
+// gathered in LDS and stored as whole rows of the walk.
+constexpr int kWalkViews = 16;
+constexpr int kWalkRow = 64 * 4 + 4;  // doubles per view in LDS (padded against bank conflicts)
+__global__ __launch_bounds__(64) void raysum_walk_kernel(WalkArgs a) {
+  __shared__ double tile[kWalkViews * kWalkRow];
   if (blockIdx.x == 0 && threadIdx.x == 0) *a.counter = 0;
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= a.nView) return;
-  const ViewEnds e = a.ends[v];
-  const ViewSetup s = view_setup(e, a.nDet, a.nSample, a.geom, a.angDetInc);
-  double xs = e.xs1 + (s.sdx / 2), ys = e.ys1 + (s.sdy / 2);
-  double xd = 0, yd = 0, am = 0;
-  if (a.geom == GEOM_EQUIANGULAR) {
-    const double aOff = s.aSampInc / 2;
-    am = e.detAngle - (a.angDetLen / 2) + aOff;
-  } else {
-    xd = e.xd1 + s.ddx2 / 2;
-    yd = e.yd1 + s.ddy2 / 2;
+  const int lane = threadIdx.x, vl = lane >> 2, comp = lane & 3, v0 = blockIdx.x * kWalkViews;
+  const int v = v0 + vl;
+  double x = 0.0, inc = 0.0;
+  if (v < a.nView) {
+    const ViewEnds e = a.ends[v];
+    const ViewSetup s = view_setup(e, a.nDet, a.nSample, a.geom, a.angDetInc);
+    const bool equi = a.geom == GEOM_EQUIANGULAR;
+    if (comp == 0) { x = e.xs1 + (s.sdx / 2); inc = s.sdx; }
+    else if (comp == 1) { x = e.ys1 + (s.sdy / 2); inc = s.sdy; }
+    else if (comp == 2) {
+      if (equi) { x = e.detAngle - (a.angDetLen / 2) + s.aSampInc / 2; inc = s.aInc; }
+      else { x = e.xd1 + s.ddx2 / 2; inc = s.ddx; }
+    } else if (!equi) {  // yd (the equiangular walk keeps 0 here)
+      x = e.yd1 + s.ddy2 / 2;
+      inc = s.ddy;
+    }
   }
-  double4* w = a.walk + static_cast<size_t>(v) * a.nDet;
-  for (int d = 0; d < a.nDet; d++) {
-    w[d] = a.geom == GEOM_EQUIANGULAR ? make_double4(xs, ys, am, 0.0) : make_double4(xs, ys, xd, yd);
-    xs += s.sdx;
-    ys += s.sdy;
-    if (a.geom == GEOM_EQUIANGULAR) am += s.aInc;
-    else { xd += s.ddx; yd += s.ddy; }
+  for (int d0 = 0; d0 < a.nDet; d0 += 64) {
+    const int n = min(64, a.nDet - d0);
+    for (int j = 0; j < n; j++) {
+      tile[vl * kWalkRow + 4 * j + comp] = x;
+      x += inc;
+    }
+    __syncthreads();
+    for (int t = lane; t < kWalkViews * n; t += 64) {
+      const int tv = t / n, tj = t - tv * n;
+      if (v0 + tv < a.nView) {
+        const double* c = tile + tv * kWalkRow + 4 * tj;
+        a.walk[static_cast<size_t>(v0 + tv) * a.nDet + d0 + tj] = make_double4(c[0], c[1], c[2], c[3]);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -257,6 +275,21 @@ __device__ __forceinline__ bool seg_surely_hits_box(double x1, double y1, double
   const double tol = m * (fabs(nx) + fabs(ny));
   const double lo = fmin(fmin(s0, s1), fmin(s2, s3)), hi = fmax(fmax(s0, s1), fmax(s2, s3));
   return lo < -tol && hi > tol;
+}
+
+// False only when the line through the segment certainly misses the unit
+// circle of the element frame m (an ellipse): its distance from the centre,
+// |p1 x p2| / |p2 - p1| in that frame, exceeds 1 by a margin far above the
+// rounding of the reference's circle clip, which then rejects the segment
+// (clip.cpp:152, |ccy| > radius).
+__device__ __forceinline__ bool line_may_hit_unit_circle(const double* m, double x1, double y1, double x2,
+                                                         double y2) {
+  const double X1 = x1 * m[0] + y1 * m[2] + m[4], Y1 = x1 * m[1] + y1 * m[3] + m[5];
+  const double X2 = x2 * m[0] + y2 * m[2] + m[4], Y2 = x2 * m[1] + y2 * m[3] + m[5];
+  const double dx = X2 - X1, dy = Y2 - Y1, cr = X1 * Y2 - X2 * Y1;
+  const double sc = fmax(fmax(fabs(X1), fabs(Y1)), fmax(fmax(fabs(X2), fabs(Y2)), 1.0));
+  const double r = 1.0 + 1e-9 * sc;
+  return cr * cr <= r * r * (dx * dx + dy * dy);
 }
 
 // PhantomElement::clipLineWorldCoords / clipLineNormalizedCoords and
@@ -473,16 +506,168 @@ __device__ __forceinline__ void wave_candidates(const RaysumArgs& a, const ViewS
   }
 }
 
-// masks: the nChunks candidate masks (wave_candidates), or nullptr for
-// every element
-template <bool EXACT>
-__device__ __forceinline__ void raysum_ray(const RaysumArgs& a, const ViewSetup& s, int v, int d,
-                                           const unsigned long long* masks, int nChunks) {
+// A wave's deferred chords.  The (lane, element) pairs whose chord is
+// needed are queued in the reference's element order and computed 64 at a
+// time, one pair per lane, so lanes whose rays miss an element do not idle
+// through its clip; each lane then adds its own chords in queue order, the
+// order of the element list.  Rings of kQueue entries: at most 63 pairs
+// wait before an element's (up to 64) are added, and a flush of 64 follows.
+constexpr int kQueue = 128;
+struct WaveQueue {
+  double4 ray[64];                  // each lane's ray: detector end, source end
+  double val[kQueue];               // computed chords, by pair slot
+  unsigned long long vmask[kQueue];  // per queued element: the lanes that need it
+  int vbase[kQueue];                // ... and the slot of its first pair
+  int pair[kQueue];                 // lane | element << 6, by pair slot
+};
+
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(m), 0));
+}
+
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long x) {
+  return (static_cast<unsigned long long>(static_cast<unsigned>(
+              __builtin_amdgcn_readfirstlane(static_cast<unsigned>(x >> 32)))) << 32) |
+         static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<unsigned>(x)));
+}
+
+// One lane's ray sum: detector d of view v (a lane past the last detector,
+// active false, only helps its wave's chords).  masks: the nChunks
+// candidate masks (wave_candidates), or nullptr for every element.
+__device__ __forceinline__ void raysum_ray(const RaysumArgs& a, const ViewSetup& s, int v, int d, bool active,
+                                           const unsigned long long* masks, int nChunks, WaveQueue* qp) {
+  const int lane = threadIdx.x;
+  const double4 w = a.walk[static_cast<size_t>(v) * a.nDet + (active ? d : a.nDet - 1)];
+  const double xs = w.x, ys = w.y;
+  double xd = w.z, yd = w.w, ang = w.z;
+  double sum = 0.0;
+  for (unsigned i = 0; i < a.nSample; i++) {
+    if (a.geom == GEOM_EQUIANGULAR) {
+      double sa, ca;
+      crm_sincos(ang, &sa, &ca);
+      xd = a.centerDetLength * ca;
+      yd = a.centerDetLength * sa;
+    }
+    qp->ray[lane] = make_double4(xd, yd, xs, ys);
+    // projectSingleLine: the element list is newest-first (phantom.cpp:247).
+    // A segment that misses an element's box, or an ellipse's line that
+    // misses it, adds the reference's exact zero, which changes no sum: such
+    // pairs are never queued (elements outside the wave's hull by the masks,
+    // then lane by lane).
+    double rsum = 0.0;
+    int enq = 0, done = 0, nv = 0, vc = 0;
+    auto flush = [&](int n) {
+      __syncthreads();  // rays, pairs and visits written above are visible
+      if (lane < n) {
+        const int slot = (done + lane) & (kQueue - 1);
+        const int pr = qp->pair[slot];
+        const double4 r = qp->ray[pr & 63];
+        qp->val[slot] = chord_ref(a.elems[pr >> 6], r.x, r.y, r.z, r.w);
+      }
+      __syncthreads();
+      const int prev = done;
+      done += n;
+      // each lane adds its chords among slots [prev, done), element by element
+      while (vc < nv) {
+        const int vs = vc & (kQueue - 1);
+        const unsigned long long vm = uniform_u64(qp->vmask[vs]);
+        const int vb = __builtin_amdgcn_readfirstlane(qp->vbase[vs]);
+        if ((vm >> lane) & 1) {
+          const int slot = vb + lanes_below(vm);
+          if (slot >= prev && slot < done) rsum += qp->val[slot & (kQueue - 1)];
+        }
+        if (vb + __popcll(vm) > done) break;  // the rest of this element comes later
+        vc++;
+      }
+    };
+    for (int c = nChunks - 1; c >= 0; c--) {
+      unsigned long long m;
+      if (masks) {
+        m = uniform_u64(masks[c]);
+      } else {  // every element of the chunk
+        const int rem = a.nElem - 64 * c;
+        m = rem >= 64 ? ~0ull : (1ull << rem) - 1;
+      }
+      while (m) {
+        const int b = 63 - __builtin_clzll(m);
+        m &= ~(1ull << b);
+        const int k = 64 * c + b;
+        // an ellipse by its line alone: every ray spans the phantom, so its
+        // segment meets the ellipse wherever its line does
+        const bool may = active && (a.elems[k].type == PE_ELLIPSE
+                                        ? line_may_hit_unit_circle(a.elems[k].p2o, xd, yd, xs, ys)
+                                        : seg_may_hit_box(xd, yd, xs, ys, a.elems[k].lim));
+        const unsigned long long hit = __ballot(may);
+        if (!hit) continue;
+        if (may) qp->pair[(enq + lanes_below(hit)) & (kQueue - 1)] = lane | (k << 6);
+        if (lane == 0) {
+          qp->vmask[nv & (kQueue - 1)] = hit;
+          qp->vbase[nv & (kQueue - 1)] = enq;
+        }
+        nv++;
+        enq += __popcll(hit);
+        if (enq - done >= 64) flush(64);
+      }
+    }
+    if (enq > done) flush(enq - done);
+    sum += rsum;
+    if (a.geom == GEOM_EQUIANGULAR) ang += s.aSampInc;
+    else { xd += s.ddx2; yd += s.ddy2; }
+  }
+  if (active) {
+    float* out = a.sino + static_cast<size_t>(v + a.storageOffset) * a.nDet;
+    out[d] = static_cast<float>(sum / a.nSample);
+  }
+}
+
+// A persistent wave per block takes items (view, 64 detectors) from a
+// counter until none is left: the rays' work differs widely (a ray through
+// the phantom's middle meets many elements), and taking items as waves free
+// up keeps every CU busy to the end.
+__device__ __forceinline__ void raysum_items(const RaysumArgs& a, WaveQueue* q) {
+  const int blocks = (a.nDet + 63) / 64, nItems = blocks * a.nView;
+  const int nChunks = (a.nElem + 63) / 64;
+  const bool cull = nChunks <= kCandChunks;
+  __shared__ unsigned long long masks[kCandChunks];  // the wave's candidate masks
+  while (true) {
+    int item = 0;
+    if (threadIdx.x == 0) item = atomicAdd(a.counter, 1);
+    item = __builtin_amdgcn_readfirstlane(item);
+    if (item >= nItems) break;
+    const int v = item / blocks, d0 = (item - v * blocks) * 64;
+    const int d = d0 + static_cast<int>(threadIdx.x), d1 = min(d0 + 63, a.nDet - 1);
+    if (a.unitPulse) {  // scanner.cpp:475-478
+      if (d < a.nDet) a.sino[static_cast<size_t>(v + a.storageOffset) * a.nDet + d] = d == a.nDet / 2 ? 1.f : 0.f;
+      continue;
+    }
+    const ViewEnds e = a.ends[v];
+    const ViewSetup s = view_setup(e, a.nDet, a.nSample, a.geom, a.angDetInc);
+    if (cull) wave_candidates(a, s, v, d0, d1, masks, nChunks);
+    raysum_ray(a, s, v, d, d < a.nDet, cull ? masks : nullptr, nChunks, q);
+  }
+}
+
+// The FP64 path at 4 waves per SIMD: the clip chain's registers are capped
+// at 128, the few it then keeps in scratch cost less than the latency a
+// fourth wave hides.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void raysum_f64_kernel(RaysumArgs a) {
+  __shared__ WaveQueue q;
+  raysum_items(a, &q);
+}
+
+// The closed-form chords, a few dozen operations each, one lane per ray over
+// every element: culling costs them more than it saves.
+__global__ __launch_bounds__(256) void raysum_closed_kernel(RaysumArgs a) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y;
+  if (d >= a.nDet) return;
   float* out = a.sino + static_cast<size_t>(v + a.storageOffset) * a.nDet;
   if (a.unitPulse) {  // scanner.cpp:475-478
     out[d] = d == a.nDet / 2 ? 1.f : 0.f;
     return;
   }
+  const ViewSetup s = view_setup(a.ends[v], a.nDet, a.nSample, a.geom, a.angDetInc);
   const double4 w = a.walk[static_cast<size_t>(v) * a.nDet + d];
   const double xs = w.x, ys = w.y;
   double xd = w.z, yd = w.w, ang = w.z;
@@ -494,62 +679,14 @@ __device__ __forceinline__ void raysum_ray(const RaysumArgs& a, const ViewSetup&
       xd = a.centerDetLength * ca;
       yd = a.centerDetLength * sa;
     }
-    // projectSingleLine: the element list is newest-first (phantom.cpp:247).
-    // A segment that misses an element's box adds the reference's exact
-    // zero, which changes no sum: such pairs are skipped (elements outside
-    // the wave's hull by the masks, then lane by lane), in the same order.
+    const double segLen = sqrt((xs - xd) * (xs - xd) + (ys - yd) * (ys - yd));
     double rsum = 0.0;
-    const double segLen = EXACT ? 0.0 : sqrt((xs - xd) * (xs - xd) + (ys - yd) * (ys - yd));
-    for (int c = nChunks - 1; c >= 0; c--) {
-      unsigned long long m;
-      if (masks) {
-        const unsigned long long mc = masks[c];
-        m = (static_cast<unsigned long long>(static_cast<unsigned>(
-                 __builtin_amdgcn_readfirstlane(static_cast<unsigned>(mc >> 32)))) << 32) |
-            static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<unsigned>(mc)));
-      } else {  // every element of the chunk
-        const int rem = a.nElem - 64 * c;
-        m = rem >= 64 ? ~0ull : (1ull << rem) - 1;
-      }
-      while (m) {
-        const int b = 63 - __builtin_clzll(m);
-        m &= ~(1ull << b);
-        const int k = 64 * c + b;
-        const bool may = seg_may_hit_box(xd, yd, xs, ys, a.elems[k].lim);
-        if (!__any(may)) continue;
-        if (may) rsum += EXACT ? chord_ref(a.elems[k], xd, yd, xs, ys)
-                               : chord_closed(a.elems[k], xd, yd, xs, ys, segLen);
-      }
-    }
+    for (int k = a.nElem - 1; k >= 0; k--) rsum += chord_closed(a.elems[k], xd, yd, xs, ys, segLen);
     sum += rsum;
     if (a.geom == GEOM_EQUIANGULAR) ang += s.aSampInc;
     else { xd += s.ddx2; yd += s.ddy2; }
   }
   out[d] = static_cast<float>(sum / a.nSample);
-}
-
-// A persistent wave per block takes items (view, 64 detectors) from a
-// counter until none is left: the rays' work differs widely (a ray through
-// the phantom's middle meets many elements), and taking items as waves free
-// up keeps every CU busy to the end.
-template <bool EXACT>
-__global__ __launch_bounds__(64) void raysum_kernel(RaysumArgs a) {
-  const int blocks = (a.nDet + 63) / 64, nItems = blocks * a.nView;
-  const int nChunks = (a.nElem + 63) / 64;
-  const bool cull = nChunks <= kCandChunks && !a.unitPulse;
-  __shared__ unsigned long long masks[kCandChunks];  // the wave's candidate masks
-  while (true) {
-    int item = 0;
-    if (threadIdx.x == 0) item = atomicAdd(a.counter, 1);
-    item = __builtin_amdgcn_readfirstlane(item);
-    if (item >= nItems) break;
-    const int v = item / blocks, d0 = (item - v * blocks) * 64;
-    const int d = d0 + static_cast<int>(threadIdx.x), d1 = min(d0 + 63, a.nDet - 1);
-    const ViewEnds e = a.ends[v];
-    const ViewSetup s = view_setup(e, a.nDet, a.nSample, a.geom, a.angDetInc);
-    if (cull) wave_candidates(a, s, v, d0, d1, masks, nChunks);
-    if (d < a.nDet) raysum_ray<EXACT>(a, s, v, d, cull ? masks : nullptr, nChunks);
-  }
 }
 
 }  // namespace
@@ -578,23 +715,24 @@ int raysum_launch(ctsim_ctx* ctx, const ScannerPlan& s, PhantomPlan& p, int prec
   int* counter = reinterpret_cast<int*>(static_cast<unsigned char*>(ctx->scratch.ptr) + walkBytes);
   WalkArgs wa{ctx->viewConsts.as<ViewEnds>(), ctx->scratch.as<double4>(), counter, iNumViews, s.nDet,
               s.geom, s.nSample, s.angDetInc, s.angDetLen};
-  hipLaunchKernelGGL(raysum_walk_kernel, dim3((iNumViews + 63) / 64), dim3(64), 0, st, wa);
+  hipLaunchKernelGGL(raysum_walk_kernel, dim3((iNumViews + kWalkViews - 1) / kWalkViews), dim3(64), 0, st, wa);
   CTSIM_HIP(hipGetLastError());
   RaysumArgs ra{ctx->viewConsts.as<ViewEnds>(), ctx->scratch.as<double4>(), p.d_elems.as<ElemGPU>(),
                 d_sino, counter, iNumViews, s.nDet, static_cast<int>(p.elems.size()), s.geom, iStorageOffset,
                 s.nSample, s.angDetInc, s.centerDetLength, p.unitPulse ? 1 : 0};
-  // persistent single-wave blocks, as many as are resident at once, no
-  // more than the items
-  const int items = ((s.nDet + 63) / 64) * iNumViews;
-  const void* kern = precision == CTSIM_PREC_F64 ? reinterpret_cast<const void*>(raysum_kernel<true>)
-                                                 : reinterpret_cast<const void*>(raysum_kernel<false>);
-  int perCU = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, 64, 0) != hipSuccess || perCU < 1) perCU = 8;
-  const dim3 grid(std::min(items, ctx->numCUs * perCU));
-  if (precision == CTSIM_PREC_F64)
-    hipLaunchKernelGGL((raysum_kernel<true>), grid, dim3(64), 0, st, ra);
-  else
-    hipLaunchKernelGGL((raysum_kernel<false>), grid, dim3(64), 0, st, ra);
+  if (precision == CTSIM_PREC_F64) {
+    // persistent single-wave blocks, as many as are resident at once, no
+    // more than the items
+    const int items = ((s.nDet + 63) / 64) * iNumViews;
+    int perCU = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(raysum_f64_kernel), 64,
+                                                     0) != hipSuccess ||
+        perCU < 1)
+      perCU = 8;
+    hipLaunchKernelGGL(raysum_f64_kernel, dim3(std::min(items, ctx->numCUs * perCU)), dim3(64), 0, st, ra);
+  } else {
+    hipLaunchKernelGGL(raysum_closed_kernel, dim3((s.nDet + 255) / 256, iNumViews), dim3(256), 0, st, ra);
+  }
   CTSIM_HIP(hipGetLastError());
   return CTSIM_OK;
 }

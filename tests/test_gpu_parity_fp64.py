@@ -52,6 +52,29 @@ def test_raysums_match_oracle(gpu_ctx, scans, geom, nDet):
     assert record_parity(f"oracle raysum {geom} {nDet}x360", sino.cpu().numpy(), ref["sino"]) == 0
 
 
+@pytest.mark.parametrize("nElem,geom", [(200, "equiangular"), (1100, "equilinear")])
+def test_raysums_many_elements(gpu_ctx, oracle_lib, nElem, geom):
+    """Phantoms with more elements than a wave's lanes: 200 (the per-wave
+    candidate culling, with more queued elements per ray than the chord
+    queue's ring holds) and 1100 (beyond the culling's 1024, every element
+    visited), on a ragged detector count with 2 samples per detector."""
+    import torch
+    import ctsim_amd
+    print(f"case parity raysum {nElem} elements {geom}", flush=True)
+    O = oracle_lib
+    elems = W.mixed_elements(41) + W.seeded_elements(nElem - 19, 42)
+    phm_o = O.Phantom(None, elems)
+    scan = O.ScanSpec(geom, 130, 24, 0, 2, 2 * np.pi)
+    ref = O.pipeline("oracle", phm_o, scan)
+    phm = ctsim_amd.Phantom(None, elems)
+    sc = ctsim_amd.Scanner(phm, geom, scan.nDet, scan.nView, 0, 2, scan.rotLen)
+    sino = torch.zeros((scan.nView, scan.nDet), dtype=torch.float32, device="cuda")
+    sc.collect(gpu_ctx, sino, ctsim_amd.PREC_F64)
+    torch.cuda.synchronize()
+    assert record_parity(f"oracle raysum {nElem} elements {geom}", sino.cpu().numpy(),
+                         ref["sino"]) == 0
+
+
 def _families(geom):
     # fan geometries select their own backprojector whatever is asked
     return ["trig", "table", "diff", "idiff"] if geom == "parallel" else ["idiff"]
