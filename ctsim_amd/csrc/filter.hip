@@ -274,54 +274,37 @@ __global__ __launch_bounds__(256) void fft_coef_kernel_c(
 // 2^-24 (a two-part split leaves 2^-17 per product, which the ramp filter's
 // cancellation amplifies past 1e-4 in the image).
 //
-// A workgroup (4 waves) computes a 64-view x 128-output tile; wave (wm, wn)
-// owns rows wm*32.. and two 32-column MFMA tiles at wn*64.  The Toeplitz
-// operand is never materialised: lane l of a B fragment needs the 8
+// A first kernel splits X once (after the preweight) into three BF16
+// planes in fragment order, rows padded with zeros to a multiple of 128
+// inputs.  In the product a workgroup (4 waves) computes a 128-view x
+// 64-output tile; wave w owns rows w*32.. and two 32-column MFMA tiles.  The
+// Toeplitz operand is never materialised: lane l of a B fragment needs the 8
 // consecutive values g[q..q+8) with q = k0 - col + n - 1, so the plan keeps a
 // record table rec[part][q + kRecOff] = g_part[q..q+8) (16 bytes, zero outside
-// g) and every fragment is one 16-byte load that the L1/L2 serve.  Tile 1
-// (32 columns further) of k-chunk j0 needs exactly tile 0's records of chunk
-// j0 - 32, so each chunk loads records for tile 0 only and tile 1 reuses the
-// previous chunk's.  Projection values and records of the next chunk are
-// loaded while the current chunk's MFMAs run; there is no LDS and no barrier.
-// Workgroups are numbered so that each XCD runs a contiguous range of row
-// blocks: the projection rows are read through one XCD's L2.
+// g) and every fragment is one 16-byte load that the L1/L2 serve.  Tile j
+// (32 j columns further) at k-step s needs exactly tile 0's records of step
+// s - 2 j, so each step loads records for tile 0 only, into a ring: six
+// 16-byte loads per lane feed 12 MFMAs, at 3 waves per SIMD (four tiles per
+// wave halve the loads per MFMA but leave 2 waves per SIMD, and run slower).
+// Projection fragments are loaded three steps ahead (their first use comes
+// from HBM), records one step ahead (they stay in L2); there is no LDS and
+// no barrier.  The four waves of a workgroup read the same records (through
+// L1) and distinct rows.  Workgroups are numbered so that each XCD runs a
+// contiguous range of row blocks: the projection rows are read through one
+// XCD's L2.
 #pragma clang fp contract(fast)
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx16_t __attribute__((ext_vector_type(16)));
-typedef float floatx4_t __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
 
-constexpr int kTM = 64, kTN = 128, kTK = 32;
+constexpr int kTM = 128;     // rows (views) per workgroup: 32 per wave
+constexpr int kToeTJ = 2;    // 32-column MFMA tiles per wave (4: fewer loads, but 2 waves per SIMD)
 constexpr int kRecOff = 192;  // record of g index q sits at q + kRecOff
 
 int toeplitz_records(int n) { return 2 * n + 256; }
-
-struct ToeChunk {
-  floatx4_t x[2][2];   // [k-step][half]: X[row][j0 + 16 s + 8 kg + 4 h + 0..3]
-  floatx4_t w[2][2];   // the same positions of the preweight (PW only)
-  u16x8_t b[2][3];     // [k-step][part]: tile-0 records
-};
-
-template <bool PW>
-__device__ __forceinline__ void toe_load(ToeChunk& c, __amdgpu_buffer_rsrc_t rsA,
-                                         __amdgpu_buffer_rsrc_t rsW, __amdgpu_buffer_rsrc_t rsB,
-                                         int aOff, int wOff, int bIdx, int j0, int nrec) {
-#pragma unroll
-  for (int st = 0; st < 2; st++) {
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      c.x[st][h] = __builtin_bit_cast(
-          floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(rsA, (aOff + j0 + 16 * st + 4 * h) * 4, 0, 0));
-      if constexpr (PW)
-        c.w[st][h] = __builtin_bit_cast(
-            floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(rsW, (wOff + j0 + 16 * st + 4 * h) * 4, 0, 0));
-    }
-#pragma unroll
-    for (int part = 0; part < 3; part++)
-      c.b[st][part] = __builtin_bit_cast(
-          u16x8_t, __builtin_amdgcn_raw_buffer_load_b128(rsB, (part * nrec + bIdx + j0 + 16 * st) * 16, 0, 0));
-  }
+int toeplitz_pad(int n) { return (n + 127) / 128 * 128; }
+size_t toeplitz_split_bytes(int n, int nView) {
+  return 3 * static_cast<size_t>((nView + 31) / 32) * 32 * toeplitz_pad(n) * sizeof(unsigned short);
 }
 
 __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
@@ -331,97 +314,153 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
   l = static_cast<__bf16>(r1 - static_cast<float>(m));
 }
 
-__device__ __forceinline__ void toe_products(floatx16_t& acc, const bf16x8_t& ah, const bf16x8_t& am,
-                                             const bf16x8_t& al, const u16x8_t* b) {
-  const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, b[0]);
-  const bf16x8_t bm = __builtin_bit_cast(bf16x8_t, b[1]);
-  const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, b[2]);
-  // smallest terms first
-  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bm, acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bh, acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bm, acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+// The planes in MFMA fragment order: per part, per block of 32 rows, per
+// k-step of 16 inputs, 1 KB holding lane l's 8 values (row l % 32, inputs
+// 8 (l / 32) ..) at l * 16 bytes, so a fragment is one coalesced 1 KB load.
+// Rows past the last view (up to the block's end) and inputs past n are
+// zero.  A workgroup takes 32 rows x 64 inputs: coalesced row reads (times
+// the preweight, PW), the split into LDS, then 4 KB of contiguous fragments
+// per part.
+template <bool PW>
+__global__ __launch_bounds__(256) void toeplitz_split_kernel(const float* __restrict__ in,
+                                                             const float* __restrict__ w32,
+                                                             u16x8_t* __restrict__ planes, int nView, int n,
+                                                             int nPad) {
+  __shared__ __align__(16) unsigned short sp[3][32][64 + 8];  // [part][row][input] (padded rows)
+  const int nKC = nPad / 64, nRB = (nView + 31) / 32;
+  const int rb = blockIdx.x / nKC, kc = blockIdx.x - rb * nKC, t = threadIdx.x;
+  const bool vec = (n & 3) == 0;
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const int idx = it * 256 + t, r = idx >> 4, c = (idx & 15) * 4;
+    const int v = rb * 32 + r, k = kc * 64 + c;
+    float x[4] = {0.f, 0.f, 0.f, 0.f};
+    if (v < nView) {
+      const float* row = in + static_cast<size_t>(v) * n;
+      if (vec && k + 4 <= n) {
+        const float4 q = *reinterpret_cast<const float4*>(row + k);
+        x[0] = q.x; x[1] = q.y; x[2] = q.z; x[3] = q.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; e++) x[e] = k + e < n ? row[k + e] : 0.f;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      if constexpr (PW)
+        if (k + e < n) x[e] *= w32[k + e];
+      __bf16 h, m, l;
+      split3(x[e], h, m, l);
+      sp[0][r][c + e] = __builtin_bit_cast(unsigned short, h);
+      sp[1][r][c + e] = __builtin_bit_cast(unsigned short, m);
+      sp[2][r][c + e] = __builtin_bit_cast(unsigned short, l);
+    }
+  }
+  __syncthreads();
+  // unit t: k-step kc * 4 + t / 64, lane t % 64 (row (t % 64) % 32, inputs 8 ((t % 64) / 32) ..)
+  const int j = t >> 6, l = t & 63;
+  const size_t plane = static_cast<size_t>(nRB) * (nPad / 16) * 64;  // 16-byte units
+  const size_t at = (static_cast<size_t>(rb) * (nPad / 16) + kc * 4) * 64 + t;
+#pragma unroll
+  for (int part = 0; part < 3; part++)
+    planes[part * plane + at] = *reinterpret_cast<const u16x8_t*>(&sp[part][l & 31][j * 16 + (l >> 5) * 8]);
+}
+
+// The six products of the four tiles (tile j takes records b[j]), term by
+// term, smallest terms first: consecutive MFMAs accumulate into different
+// tiles, so none waits for the one before it.
+template <int TJ>
+__device__ __forceinline__ void toe_products(floatx16_t* acc, const u16x8_t* a, const u16x8_t* const* b) {
+  const bf16x8_t ah = __builtin_bit_cast(bf16x8_t, a[0]);
+  const bf16x8_t am = __builtin_bit_cast(bf16x8_t, a[1]);
+  const bf16x8_t al = __builtin_bit_cast(bf16x8_t, a[2]);
+#define CTSIM_TOE_TERM(A, PART)                                                                     \
+  _Pragma("unroll") for (int j = 0; j < TJ; j++) acc[j] =                                           \
+      __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, __builtin_bit_cast(bf16x8_t, b[j][PART]), acc[j], 0, 0, 0);
+  CTSIM_TOE_TERM(am, 1)
+  CTSIM_TOE_TERM(al, 0)
+  CTSIM_TOE_TERM(ah, 2)
+  CTSIM_TOE_TERM(am, 0)
+  CTSIM_TOE_TERM(ah, 1)
+  CTSIM_TOE_TERM(ah, 0)
+#undef CTSIM_TOE_TERM
 }
 
 // grid: 8 * ceil(tiles / 8) workgroups of 256 lanes, tiles = ceil(n / 128) *
-// ceil(nView / 64); rec: 3 * nrec records (hi, mid, lo), nrec = toeplitz_records(n);
-// PW: the inputs are multiplied by the preweight w32 first.
-template <bool PW>
-__global__ __launch_bounds__(256) void toeplitz_mfma_kernel(
-    const float* __restrict__ in, const float* __restrict__ w32, const u16x8_t* __restrict__ rec,
-    int nrec, float* __restrict__ out, int nView, int n, int outStride) {
-  const int ncb = (n + kTN - 1) / kTN;
+// ceil(nView / 128); planes from toeplitz_split_kernel (nPad = toeplitz_pad(n));
+// rec: 3 * nrec records (hi, mid, lo), nrec = toeplitz_records(n).
+template <int TJ>
+__global__ __launch_bounds__(256) void toeplitz_mfma_kernel(const unsigned short* __restrict__ planes,
+                                                            const u16x8_t* __restrict__ rec, int nrec,
+                                                            float* __restrict__ out, int nView, int n, int nPad,
+                                                            int outStride) {
+  constexpr int TN = 32 * TJ, RS = 2 * TJ, UN = RS > 4 ? RS : 4;  // columns, record ring, unroll
+  const int ncb = (n + TN - 1) / TN;
   const int tiles = ncb * ((nView + kTM - 1) / kTM);
   const int perXcd = (tiles + 7) / 8;
   const int t = (blockIdx.x & 7) * perXcd + (blockIdx.x >> 3);
   if (t >= tiles) return;
-  const int m0 = (t / ncb) * kTM, n0 = (t % ncb) * kTN;
+  const int m0 = (t / ncb) * kTM, n0 = (t % ncb) * TN;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
   const int kg = lane >> 5;
-  const int rows = min(kTM, nView - m0);
-  // rows past the last view read as zero (buffer range); values past the
-  // row end (k >= n) are cleared in the last chunk
-  const __amdgpu_buffer_rsrc_t rsA = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<float*>(in + static_cast<size_t>(m0) * n), 0, rows * n * 4, 0x00020000);
+  // one resource per plane: row blocks past the last read as zero
+  const int nRB = (nView + 31) / 32, nSteps = nPad / 16;  // nSteps: a multiple of 8
+  const size_t planeBytes = static_cast<size_t>(nRB) * nSteps * 1024;
+  __amdgpu_buffer_rsrc_t rsA[3];
+#pragma unroll
+  for (int part = 0; part < 3; part++)
+    rsA[part] = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<unsigned char*>(reinterpret_cast<const unsigned char*>(planes) + part * planeBytes), 0,
+        static_cast<int>(planeBytes), 0x00020000);
   const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<u16x8_t*>(rec), 0, 3 * nrec * 16, 0x00020000);
-  const __amdgpu_buffer_rsrc_t rsW = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<float*>(w32), 0, PW ? n * 4 : 0, 0x00020000);
-  const int wOff = 8 * kg;
-  const int aOff = (wm * 32 + (lane & 31)) * n + 8 * kg;
-  const int bIdx = 8 * kg - (n0 + wn * 64 + (lane & 31)) + n - 1 + kRecOff;
-  floatx16_t acc0 = {}, acc1 = {};
-  ToeChunk cur, nxt;
-  u16x8_t prev[2][3];  // tile-0 records of the previous chunk = tile 1's of this one
-  toe_load<PW>(nxt, rsA, rsW, rsB, aOff, wOff, bIdx, -kTK, nrec);  // only its records are used
+  const int aOff = (m0 / 32 + wave) * nSteps * 1024 + lane * 16;  // bytes
+  const int bIdx = 8 * kg - (n0 + (lane & 31)) + n - 1 + kRecOff;
+  auto loadA = [&](u16x8_t* a, int st) {
 #pragma unroll
-  for (int st = 0; st < 2; st++)
+    for (int part = 0; part < 3; part++)
+      a[part] = __builtin_bit_cast(u16x8_t, __builtin_amdgcn_raw_buffer_load_b128(rsA[part], aOff + st * 1024, 0, 0));
+  };
+  auto loadB = [&](u16x8_t* b, int st) {  // past the table's ends: zeros
 #pragma unroll
-    for (int part = 0; part < 3; part++) prev[st][part] = nxt.b[st][part];
-  toe_load<PW>(cur, rsA, rsW, rsB, aOff, wOff, bIdx, 0, nrec);
-  const int nc = (n + kTK - 1) / kTK;
-  for (int c = 0; c < nc; c++) {
-    const int j0 = c * kTK;
-    toe_load<PW>(nxt, rsA, rsW, rsB, aOff, wOff, bIdx, j0 + kTK, nrec);  // past the end: zeros, unused
-    __builtin_amdgcn_sched_barrier(0);  // keep the next chunk's loads ahead of this chunk's MFMAs
+    for (int part = 0; part < 3; part++)
+      b[part] = __builtin_bit_cast(
+          u16x8_t, __builtin_amdgcn_raw_buffer_load_b128(rsB, (part * nrec + bIdx + 16 * st) * 16, 0, 0));
+  };
+  floatx16_t acc[TJ] = {};
+  // A: steps s .. s+3 at [s & 3] (the projection rows come from HBM on
+  // first use); R: tile-0 records of steps s-2(TJ-1) .. s+1 at [s % RS]
+  // (tile j at step s is tile 0 at step s - 2j; the records stay in L2)
+  u16x8_t A[4][3], R[RS][3];
 #pragma unroll
-    for (int st = 0; st < 2; st++) {
-      const int k0 = j0 + 16 * st + 8 * kg;
-      bf16x8_t ah, am, al;
+  for (int st = 2 - RS; st <= 0; st++) loadB(R[st & (RS - 1)], st);
 #pragma unroll
-      for (int e = 0; e < 8; e++) {
-        float x = cur.x[st][e >> 2][e & 3];
-        if constexpr (PW) x *= cur.w[st][e >> 2][e & 3];
-        if (j0 + kTK > n && k0 + e >= n) x = 0.f;
-        __bf16 h, m, l;
-        split3(x, h, m, l);
-        ah[e] = h;
-        am[e] = m;
-        al[e] = l;
-      }
-      toe_products(acc0, ah, am, al, cur.b[st]);
-      toe_products(acc1, ah, am, al, prev[st]);
+  for (int st = 0; st < 3; st++) loadA(A[st], st);
+  for (int s0 = 0; s0 < nSteps; s0 += UN) {
+#pragma unroll
+    for (int u = 0; u < UN; u++) {
+      const int st = s0 + u;
+      // records first: the wait for them (loads complete in issue order)
+      // then leaves the projection fragments in flight
+      loadB(R[(u + 1) & (RS - 1)], st + 1);
+      loadA(A[(u + 3) & 3], st + 3);  // past the last step: zeros or unused
+      __builtin_amdgcn_sched_barrier(0);  // keep the next step's loads ahead of this step's MFMAs
+      const u16x8_t* bj[TJ];
+#pragma unroll
+      for (int j = 0; j < TJ; j++) bj[j] = R[(u - 2 * j) & (RS - 1)];
+      toe_products<TJ>(acc, A[u & 3], bj);
+      __builtin_amdgcn_sched_barrier(0);
     }
-    __builtin_amdgcn_sched_barrier(0);  // rotate only after every MFMA is issued
-#pragma unroll
-    for (int st = 0; st < 2; st++)
-#pragma unroll
-      for (int part = 0; part < 3; part++) prev[st][part] = cur.b[st][part];
-    cur = nxt;
   }
   // D layout of the 32x32 tile: register r of lane l is row 8(r/4) + 4(l/32) + r%4,
   // column l%32.
 #pragma unroll
-  for (int tile = 0; tile < 2; tile++) {
-    const floatx16_t& acc = tile == 0 ? acc0 : acc1;
-    const int col = n0 + wn * 64 + tile * 32 + (lane & 31);
+  for (int j = 0; j < TJ; j++) {
+    const int col = n0 + j * 32 + (lane & 31);
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-      const int rr = m0 + wm * 32 + 8 * (r / 4) + 4 * kg + (r % 4);
-      if (rr < nView && col < n) out[static_cast<size_t>(rr) * outStride + col] = acc[r];
+      const int rr = m0 + wave * 32 + 8 * (r / 4) + 4 * kg + (r % 4);
+      if (rr < nView && col < n) out[static_cast<size_t>(rr) * outStride + col] = acc[j][r];
     }
   }
 }
@@ -645,21 +684,36 @@ int filter_launch_coef(ctsim_ctx* ctx, FilterPlan& p, const float* d_in, int nVi
   return CTSIM_OK;
 }
 
+size_t filter_work_bytes(const FilterPlan& p, int precision, int nView) {
+  if (p.method == METHOD_CONVOLUTION && precision != CTSIM_PREC_F64) return toeplitz_split_bytes(p.n, nView);
+  return 0;
+}
+
 int filter_launch(ctsim_ctx* ctx, FilterPlan& p, int precision, const float* d_in, int nView,
-                  void* d_out, int outStride) {
+                  void* d_out, int outStride, void* d_work) {
   if (nView <= 0) return CTSIM_OK;
   const hipStream_t s = ctx->stream;
   const double* w = p.preweight.empty() ? nullptr : p.d_weight.as<double>();
   const bool f64 = precision == CTSIM_PREC_F64;
   if (p.method == METHOD_CONVOLUTION && !f64) {
-    const int tiles = ((p.n + kTN - 1) / kTN) * ((nView + kTM - 1) / kTM);
-    if (static_cast<long long>(kTM) * p.n * 4 > INT32_MAX)
+    const int tiles = ((p.n + 32 * kToeTJ - 1) / (32 * kToeTJ)) * ((nView + kTM - 1) / kTM);
+    const int nPad = toeplitz_pad(p.n);
+    if (toeplitz_split_bytes(p.n, nView) / 3 > INT32_MAX - 4096 || 2LL * p.n + 512 > INT32_MAX / 48)
       return fail(CTSIM_ERR_UNSUPPORTED, "mixed convolution: detector count too large");
+    if (!d_work) {
+      CTSIM_CHECK(ctx->scratch.reserve(toeplitz_split_bytes(p.n, nView)));
+      d_work = ctx->scratch.ptr;
+    }
+    u16x8_t* planes = static_cast<u16x8_t*>(d_work);
     const float* w32 = p.preweight.empty() ? nullptr : p.d_weight32.as<float>();
-    auto kern = w32 ? toeplitz_mfma_kernel<true> : toeplitz_mfma_kernel<false>;
-    hipLaunchKernelGGL(kern, dim3(8 * ((tiles + 7) / 8)), dim3(256), 0, s, d_in, w32,
-                       p.d_toeplitz.as<u16x8_t>(), toeplitz_records(p.n),
-                       static_cast<float*>(d_out), nView, p.n, outStride);
+    const dim3 gs(static_cast<unsigned>((nView + 31) / 32 * (nPad / 64)));
+    if (w32)
+      hipLaunchKernelGGL((toeplitz_split_kernel<true>), gs, dim3(256), 0, s, d_in, w32, planes, nView, p.n, nPad);
+    else
+      hipLaunchKernelGGL((toeplitz_split_kernel<false>), gs, dim3(256), 0, s, d_in, w32, planes, nView, p.n, nPad);
+    hipLaunchKernelGGL(toeplitz_mfma_kernel<kToeTJ>, dim3(8 * ((tiles + 7) / 8)), dim3(256), 0, s,
+                       reinterpret_cast<const unsigned short*>(planes), p.d_toeplitz.as<u16x8_t>(),
+                       toeplitz_records(p.n), static_cast<float*>(d_out), nView, p.n, nPad, outStride);
     CTSIM_HIP(hipGetLastError());
     return CTSIM_OK;
   }

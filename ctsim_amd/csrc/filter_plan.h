@@ -45,8 +45,11 @@ bool filter_is_fourier(const FilterPlan& p);
 
 // Device side (filter.hip).
 int filter_upload(FilterPlan& p);
+// d_work: device scratch of filter_work_bytes() for the mixed convolution
+// (its split projection values), or nullptr to take the context's scratch.
 int filter_launch(ctsim_ctx* ctx, FilterPlan& p, int precision, const float* d_in, int nView,
-                  void* d_out, int outStride);
+                  void* d_out, int outStride, void* d_work = nullptr);
+size_t filter_work_bytes(const FilterPlan& p, int precision, int nView);
 // Mixed FFT filtering straight into backprojection interval coefficients
 // (float4 cubic / float2 linear, [nView][n]); CTSIM_ERR_UNSUPPORTED for
 // convolution filters or non-power-of-two transforms.

@@ -80,8 +80,9 @@ extern "C" int ctsim_fbp_slice_static(ctsim_ctx* ctx, ctsim_procsignal* ps,
     if (ctx->device >= 0 && ctx->device < kMaxDevices) base = g_scratchPtr[ctx->device];
   }
   if (!base) return fail(CTSIM_ERR_INVALID, "ctsim_fbp_slice_static: static scratch not set up");
-  // Layout: view constants | interval coefficients | filtered rows
-  // (convolution filters) | view-group partial images.
+  // Layout: view constants | interval coefficients | filtered rows and
+  // split projection values (convolution filters) | view-group partial
+  // images.
   const bool cubic = p.interp == INTERP_CUBIC;
   const size_t elem = cubic ? sizeof(float4) : sizeof(float2);
   const bool coefFilter = filter_can_coef(f);
@@ -91,8 +92,9 @@ extern "C" int ctsim_fbp_slice_static(ctsim_ctx* ctx, ctsim_procsignal* ps,
   const size_t vcBytes = align256(sizeof(ViewConstF32) * nView);
   const size_t coefBytes = align256(static_cast<size_t>(nView) * p.nDet * elem);
   const size_t rowBytes = coefFilter ? 0 : align256(static_cast<size_t>(nView) * f.n * sizeof(float));
+  const size_t workBytes = coefFilter ? 0 : align256(filter_work_bytes(f, CTSIM_PREC_MIXED, nView));
   const size_t partBytes = split > 1 ? align256(static_cast<size_t>(split) * npix * sizeof(float)) : 0;
-  const size_t need = vcBytes + coefBytes + rowBytes + partBytes;
+  const size_t need = vcBytes + coefBytes + rowBytes + workBytes + partBytes;
   if (need > kStaticScratchBytes)
     return fail(CTSIM_ERR_INVALID,
                 "ctsim_fbp_slice_static: this slice needs " + std::to_string(need >> 20) +
@@ -102,7 +104,8 @@ extern "C" int ctsim_fbp_slice_static(ctsim_ctx* ctx, ctsim_procsignal* ps,
   ViewConstF32* d_vc = reinterpret_cast<ViewConstF32*>(b);
   void* d_coef = b + vcBytes;
   float* d_rows = reinterpret_cast<float*>(b + vcBytes + coefBytes);
-  float* d_part = partBytes ? reinterpret_cast<float*>(b + vcBytes + coefBytes + rowBytes) : nullptr;
+  void* d_work = workBytes ? b + vcBytes + coefBytes + rowBytes : nullptr;
+  float* d_part = partBytes ? reinterpret_cast<float*>(b + vcBytes + coefBytes + rowBytes + workBytes) : nullptr;
   CTSIM_CHECK(ctx->activate());
   SymViews sv;
   {
@@ -118,7 +121,7 @@ extern "C" int ctsim_fbp_slice_static(ctsim_ctx* ctx, ctsim_procsignal* ps,
   if (coefFilter) {
     CTSIM_CHECK(filter_launch_coef(ctx, f, d_sino, nView, d_coef, p.interp, p.kind == KIND_IDIFF));
   } else {
-    CTSIM_CHECK(filter_launch(ctx, f, CTSIM_PREC_MIXED, d_sino, nView, d_rows, f.n));
+    CTSIM_CHECK(filter_launch(ctx, f, CTSIM_PREC_MIXED, d_sino, nView, d_rows, f.n, d_work));
     CTSIM_CHECK(bp_mixed_coef_into(ctx, p, d_rows, nView, d_coef));
   }
   return bp_mixed_tiles(ctx, p, m, d_coef, d_vc, nView, d_image, p.rotScale, true, &sv, d_part);

@@ -128,8 +128,9 @@ int ctsim_fbp_slice(ctsim_ctx* ctx, ctsim_procsignal* ps, const ctsim_backprojec
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): ctsim_fbp_slice at CTSIM_PREC_MIXED with every scratch
- * buffer (per-view constants, interval coefficients, filtered rows of
- * convolution filters, view-group partial images) in one static device array
+ * buffer (per-view constants, interval coefficients, filtered rows and
+ * BF16-split projection values of convolution filters, view-group partial
+ * images) in one static device array
  * of CTSIM_STATIC_SCRATCH_BYTES per device, whose address is taken when a
  * context is created on the device.  Calls no hipMalloc/hipFree or other
  * allocating routine; the caller passes the inputs and the output.  Same
