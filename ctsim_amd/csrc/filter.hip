@@ -386,9 +386,10 @@ __device__ __forceinline__ void toe_products(floatx16_t* acc, const u16x8_t* a, 
 #undef CTSIM_TOE_TERM
 }
 
-// grid: 8 * ceil(tiles / 8) workgroups of 256 lanes, tiles = ceil(n / 128) *
-// ceil(nView / 128); planes from toeplitz_split_kernel (nPad = toeplitz_pad(n));
-// rec: 3 * nrec records (hi, mid, lo), nrec = toeplitz_records(n).
+// TJ 32-column tiles per wave; grid: 8 * ceil(tiles / 8) workgroups of 256
+// lanes, tiles = ceil(n / (32 TJ)) * ceil(nView / 128); planes from
+// toeplitz_split_kernel (nPad = toeplitz_pad(n)); rec: 3 * nrec records (hi,
+// mid, lo), nrec = toeplitz_records(n).
 template <int TJ>
 __global__ __launch_bounds__(256) void toeplitz_mfma_kernel(const unsigned short* __restrict__ planes,
                                                             const u16x8_t* __restrict__ rec, int nrec,

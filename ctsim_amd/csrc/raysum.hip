@@ -437,7 +437,6 @@ struct RaysumArgs {
   int unitPulse;
 };
 
-// One lane's ray sum: detector d of view v.
 // Detector end (a) and source end (b) of sample i of detector d's ray, as
 // raysum_ray walks them (to rounding: the hull test below has a margin).
 __device__ __forceinline__ void ray_ends(const RaysumArgs& a, const ViewSetup& s, int v, int d, unsigned i,
