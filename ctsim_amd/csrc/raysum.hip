@@ -592,8 +592,9 @@ __device__ __forceinline__ void raysum_ray(const RaysumArgs& a, const ViewSetup&
         const int b = 63 - __builtin_clzll(m);
         m &= ~(1ull << b);
         const int k = 64 * c + b;
-        // an ellipse by its line alone: every ray spans the phantom, so its
-        // segment meets the ellipse wherever its line does
+        // an ellipse by its line alone (a line that misses it clips to zero;
+        // the rays span the phantom, so the line test queues next to no pair
+        // whose segment the clip then rejects)
         const bool may = active && (a.elems[k].type == PE_ELLIPSE
                                         ? line_may_hit_unit_circle(a.elems[k].p2o, xd, yd, xs, ys)
                                         : seg_may_hit_box(xd, yd, xs, ys, a.elems[k].lim));
