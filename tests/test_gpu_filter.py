@@ -149,7 +149,7 @@ def test_fourier_hanning_cubic_end_to_end(gpu_ctx, geom, nDet):
     assert record_parity(f"golden fourier hanning cubic {geom} {nDet}", img, ref) == 0
 
 
-@pytest.mark.parametrize("n", [33, 130, 367, 1024, 2048])
+@pytest.mark.parametrize("n", [33, 130, 367, 1024, 2048, 2500])
 def test_mixed_convolution_toeplitz_mfma(gpu_ctx, oracle_lib, n):
     """Mixed convolution runs as a three-way split-BF16 Toeplitz product on
     the MFMA units (terms down to 2^-16 relative kept): its error against the
@@ -159,7 +159,7 @@ def test_mixed_convolution_toeplitz_mfma(gpu_ctx, oracle_lib, n):
     O = oracle_lib
     rng = np.random.default_rng(n)
     for geom, f in itertools.product((0, 1, 2), ("abs_bandlimit", "shepp")):
-        sig = rng.standard_normal((70 if n <= 1024 else 20, n)).astype(np.float32)
+        sig = rng.standard_normal((70 if n <= 1024 else (20 if n <= 2048 else 9), n)).astype(np.float32)
         inc = 2.0 / n
         ref, _, _ = O.filter_views("oracle", sig, f, "convolution", bw=1 / inc, inc=inc,
                                    geometry=geom, focal=2.0, sdl=4.0)
