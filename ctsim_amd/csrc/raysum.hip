@@ -88,15 +88,25 @@ __global__ __launch_bounds__(64) void raysum_walk_kernel(WalkArgs a) {
       inc = s.ddy;
     }
   }
+  double* mine = tile + vl * kWalkRow + comp;
   for (int d0 = 0; d0 < a.nDet; d0 += 64) {
     const int n = min(64, a.nDet - d0);
-    for (int j = 0; j < n; j++) {
-      tile[vl * kWalkRow + 4 * j + comp] = x;
-      x += inc;
+    if (n == 64) {  // a whole run: unrolled, the stores off the chain of adds
+#pragma unroll
+      for (int j = 0; j < 64; j++) {
+        mine[4 * j] = x;
+        x += inc;
+      }
+    } else {
+      for (int j = 0; j < n; j++) {
+        mine[4 * j] = x;
+        x += inc;
+      }
     }
     __syncthreads();
+    // row tv's cells tj, 64 lanes a round (n == 64: one view row per round)
     for (int t = lane; t < kWalkViews * n; t += 64) {
-      const int tv = t / n, tj = t - tv * n;
+      const int tv = n == 64 ? t >> 6 : t / n, tj = n == 64 ? t & 63 : t - tv * n;
       if (v0 + tv < a.nView) {
         const double* c = tile + tv * kWalkRow + 4 * tj;
         a.walk[static_cast<size_t>(v0 + tv) * a.nDet + d0 + tj] = make_double4(c[0], c[1], c[2], c[3]);
