@@ -387,6 +387,14 @@ __device__ __forceinline__ bool seg_hits_box(double x1, double y1, double dx, do
   return t0 <= t1;
 }
 
+// The reference's clip chain as a call: the closed form takes it only for
+// triangles, segments and sectors, whose registers then stay out of the
+// ellipse and rectangle path.
+__device__ __attribute__((noinline)) double chord_ref_call(const ElemGPU& e, double x1, double y1, double x2,
+                                                           double y2) {
+  return chord_ref(e, x1, y1, x2, y2);
+}
+
 // Parameter interval of the segment inside the element, in the element frame.
 __device__ double chord_closed(const ElemGPU& e, double x1, double y1, double x2, double y2,
                                double segLen) {
@@ -430,7 +438,7 @@ __device__ double chord_closed(const ElemGPU& e, double x1, double y1, double x2
     t1 = fmin(t1, 1.0);
     return t1 > t0 ? (t1 - t0) * segLen * e.atten : 0.0;
   }
-  return chord_ref(e, x1, y1, x2, y2);
+  return chord_ref_call(e, x1, y1, x2, y2);
 }
 
 #pragma clang fp contract(off)
