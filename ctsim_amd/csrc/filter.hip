@@ -26,6 +26,7 @@ constexpr int kLanes = 64;        // views per wave
 constexpr int kWaves = 4;         // waves per block
 constexpr int kOutPerWave = 8;    // outputs per wave (independent sums)
 constexpr int kJChunk = 64;       // inputs staged per step
+static_assert(kOutPerWave == 8, "conv_seq_kernel's tap window holds 15 = 2 * 8 - 1 taps");
 constexpr int kKChunk = 32;       // spectrum entries staged per step (inverse DFT)
 
 // ---------------------------------------------------- exact convolution
@@ -57,7 +58,24 @@ __global__ __launch_bounds__(kLanes * kWaves) void conv_seq_kernel(
     }
     __syncthreads();
     const int jn = min(kJChunk, n - j0);
-    for (int jj = 0; jj < jn; jj++) {
+    int jj = 0;
+    // 8 steps at a time: output o at step j + u takes tap i0 + o - j - u +
+    // n - 1, so one window of 15 taps serves the block (T[7 + o - u]); the
+    // taps of outputs past n are clamped reads, never used
+    for (; jj + 8 <= jn; jj += 8) {
+      const int base = i0 - (j0 + jj) + n - 1 - 7;
+      double T[15];
+#pragma unroll
+      for (int m = 0; m < 15; m++) T[m] = s_h[min(max(base + m, 0), 2 * n - 2)];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const double xv = s_x[jj + u][lane];
+#pragma unroll
+        for (int o = 0; o < kOutPerWave; o++)
+          if (i0 + o < n) acc[o] += xv * T[7 + o - u];
+      }
+    }
+    for (; jj < jn; jj++) {
       const double xv = s_x[jj][lane];
       const int j = j0 + jj;
 #pragma unroll
