@@ -26,7 +26,7 @@ constexpr int kLanes = 64;        // views per wave
 constexpr int kWaves = 4;         // waves per block
 constexpr int kOutPerWave = 8;    // outputs per wave (independent sums)
 constexpr int kJChunk = 64;       // inputs staged per step
-static_assert(kOutPerWave == 8, "conv_seq_kernel's tap window holds 15 = 2 * 8 - 1 taps");
+constexpr int kConvOut = 16;      // exact convolution: outputs per lane (independent sums)
 constexpr int kKChunk = 32;       // spectrum entries staged per step (inverse DFT)
 
 // ---------------------------------------------------- exact convolution
@@ -36,14 +36,16 @@ __global__ __launch_bounds__(kLanes * kWaves) void conv_seq_kernel(
     const float* __restrict__ in, const double* __restrict__ w, const double* __restrict__ h,
     TO* __restrict__ out, int nView, int n, double dx, int outStride) {
   __shared__ double s_x[kJChunk][kLanes];
-  extern __shared__ double s_h[];  // 2n-1 taps
+  extern __shared__ double s_h[];  // the block's taps: h[ib + k], k < n + kWaves * kConvOut - 1
   const int lane = threadIdx.x & (kLanes - 1), wave = threadIdx.x / kLanes;
   const int v = blockIdx.y * kLanes + lane;
-  const int i0 = (blockIdx.x * kWaves + wave) * kOutPerWave;
-  for (int k = threadIdx.x; k < 2 * n - 1; k += blockDim.x) s_h[k] = h[k];
-  double acc[kOutPerWave];
+  const int ib = blockIdx.x * kWaves * kConvOut, i0 = ib + wave * kConvOut;
+  // output i of the block at input j takes tap i - j + n - 1, in [ib, ib + n - 1 + kWaves kConvOut)
+  const int nTap = n + kWaves * kConvOut - 1;
+  for (int k = threadIdx.x; k < nTap; k += blockDim.x) s_h[k] = h[min(ib + k, 2 * n - 2)];
+  double acc[kConvOut];
 #pragma unroll
-  for (int o = 0; o < kOutPerWave; o++) acc[o] = 0.0;
+  for (int o = 0; o < kConvOut; o++) acc[o] = 0.0;
   for (int j0 = 0; j0 < n; j0 += kJChunk) {
     __syncthreads();
     for (int k = threadIdx.x; k < kJChunk * kLanes; k += blockDim.x) {
@@ -60,18 +62,18 @@ __global__ __launch_bounds__(kLanes * kWaves) void conv_seq_kernel(
     const int jn = min(kJChunk, n - j0);
     int jj = 0;
     // 8 steps at a time: output o at step j + u takes tap i0 + o - j - u +
-    // n - 1, so one window of 15 taps serves the block (T[7 + o - u]); the
-    // taps of outputs past n are clamped reads, never used
+    // n - 1, so one window of kConvOut + 7 taps serves the block (T[7 + o -
+    // u]); every sum still adds its products in input order
     for (; jj + 8 <= jn; jj += 8) {
-      const int base = i0 - (j0 + jj) + n - 1 - 7;
-      double T[15];
+      const int base = i0 - (j0 + jj) + n - 1 - 7 - ib;  // local tap index of T[0]
+      double T[kConvOut + 7];
 #pragma unroll
-      for (int m = 0; m < 15; m++) T[m] = s_h[min(max(base + m, 0), 2 * n - 2)];
+      for (int m = 0; m < kConvOut + 7; m++) T[m] = s_h[max(base + m, 0)];
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const double xv = s_x[jj + u][lane];
 #pragma unroll
-        for (int o = 0; o < kOutPerWave; o++)
+        for (int o = 0; o < kConvOut; o++)
           if (i0 + o < n) acc[o] += xv * T[7 + o - u];
       }
     }
@@ -79,16 +81,15 @@ __global__ __launch_bounds__(kLanes * kWaves) void conv_seq_kernel(
       const double xv = s_x[jj][lane];
       const int j = j0 + jj;
 #pragma unroll
-      for (int o = 0; o < kOutPerWave; o++) {
+      for (int o = 0; o < kConvOut; o++) {
         const int i = i0 + o;
-        const int hk = i - j + n - 1;
-        if (i < n) acc[o] += xv * s_h[hk];
+        if (i < n) acc[o] += xv * s_h[i - j + n - 1 - ib];
       }
     }
   }
   if (v < nView) {
 #pragma unroll
-    for (int o = 0; o < kOutPerWave; o++) {
+    for (int o = 0; o < kConvOut; o++) {
       const int i = i0 + o;
       if (i < n) out[static_cast<size_t>(v) * outStride + i] = static_cast<TO>(acc[o] * dx);
     }
@@ -737,8 +738,8 @@ int filter_launch(ctsim_ctx* ctx, FilterPlan& p, int precision, const float* d_i
     return CTSIM_OK;
   }
   if (p.method == METHOD_CONVOLUTION) {
-    const dim3 grid((p.n + kWaves * kOutPerWave - 1) / (kWaves * kOutPerWave), (nView + kLanes - 1) / kLanes);
-    const size_t lds = static_cast<size_t>(2 * p.n - 1) * sizeof(double);
+    const dim3 grid((p.n + kWaves * kConvOut - 1) / (kWaves * kConvOut), (nView + kLanes - 1) / kLanes);
+    const size_t lds = static_cast<size_t>(p.n + kWaves * kConvOut - 1) * sizeof(double);
     if (lds + sizeof(double) * kJChunk * kLanes > 64 * 1024)
       return fail(CTSIM_ERR_UNSUPPORTED, "convolution filter: kernel too long for LDS");
     if (f64)
