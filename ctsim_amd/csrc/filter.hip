@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void fft_filter_kernel(
     const float* __restrict__ in, const double* __restrict__ w, const T* __restrict__ hre,
     const T* __restrict__ him, const Cplx<T>* __restrict__ twg, TO* __restrict__ out, int nView,
     int n, int N, int logN, int outStride) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw);
   Cplx<T>* tw = z + N;
   const int va = 2 * blockIdx.x, vb = va + 1 < nView ? va + 1 : -1;
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void fft_coef_kernel(
     const float* __restrict__ in, const double* __restrict__ w, const float* __restrict__ hre,
     const float* __restrict__ him, const Cplx<float>* __restrict__ twg, void* __restrict__ coef,
     int nView, int n, int N, int logN) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   Cplx<float>* z = reinterpret_cast<Cplx<float>*>(smem_raw);
   Cplx<float>* tw = z + N;
   const int va = 2 * blockIdx.x, vb = va + 1 < nView ? va + 1 : -1;
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(256) void fft_filter_kernel_c(
     const float* __restrict__ in, const double* __restrict__ w, const float* __restrict__ hre,
     const float* __restrict__ him, const Cplx<float>* __restrict__ twg, float* __restrict__ out,
     int nView, int n, int outStride) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   Cplx<float>* z = reinterpret_cast<Cplx<float>*>(smem_raw);
   Cplx<float>* tw = z + (1 << LOGN);
   const int va = 2 * blockIdx.x, vb = va + 1 < nView ? va + 1 : -1;
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) void fft_coef_kernel_c(
     const float* __restrict__ in, const double* __restrict__ w, const float* __restrict__ hre,
     const float* __restrict__ him, const Cplx<float>* __restrict__ twg, void* __restrict__ coef,
     int nView, int n) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   Cplx<float>* z = reinterpret_cast<Cplx<float>*>(smem_raw);
   Cplx<float>* tw = z + (1 << LOGN);
   const int va = 2 * blockIdx.x, vb = va + 1 < nView ? va + 1 : -1;
