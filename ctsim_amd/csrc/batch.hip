@@ -1057,6 +1057,7 @@ int ctsim_batch_create(ctsim_ctx* ctx, ctsim_batch** out, const ctsim_slice_desc
         tw[2 * kk] = static_cast<float>(cos(a));
         tw[2 * kk + 1] = static_cast<float>(sin(a));
       }
+      dev::fft_append_pass_twiddles(tw, N);
       if ((rc = sl.mul.reserve(mul.size() * sizeof(float))) != CTSIM_OK) break;
       if ((rc = sl.tw.reserve(tw.size() * sizeof(float))) != CTSIM_OK) break;
       if (hipMemcpy(sl.mul.ptr, mul.data(), mul.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||

@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace ctsim {
 namespace dev {
 
@@ -20,6 +22,12 @@ struct alignas(2 * sizeof(T)) Cplx {
 template <class T>
 __device__ __forceinline__ Cplx<T> cmul(Cplx<T> a, Cplx<T> b) {
   return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// The same product with one rounding less per part (the compile-time FP32
+// transforms): two multiplies and two fused multiply-adds.
+__device__ __forceinline__ Cplx<float> cmul_fma(Cplx<float> a, Cplx<float> b) {
+  return {__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.x, b.y, a.y * b.x)};
 }
 
 // In-place radix-2 Stockham pass over z[0..N) held in LDS.  tw[k] =
@@ -413,25 +421,52 @@ __host__ __device__ constexpr size_t fft_c_lds(int N) {
   return (N <= 2048 ? 1 : 2) * static_cast<size_t>(N) * sizeof(Cplx<float>);
 }
 
+// FP32 twiddle tables for the transforms: the half table exp(-2 pi i m / N),
+// m < N / 2 (N floats), followed, for 64 <= N <= 2048, by the radix-8
+// passes' twiddles in pass order: w^(r k N / (8 Ls)) at off + (r - 1) Ls + k
+// for each pass with span Ls > 1 (r = 1..7, k < Ls), copied from the half
+// table with the sign of w^(m + N/2) = -w^m.  A lane's seven twiddles of a
+// pass are then consecutive table runs across the wave (the half table's
+// index r k strides up to 7 entries per lane).
+inline void fft_append_pass_twiddles(std::vector<float>& t32, int N) {
+  int logN = 0;
+  while ((1 << logN) < N) logN++;
+  if (logN < 6 || logN > 11) return;
+  const int half = N / 2;
+  const int l0 = fft8_first_radix(logN) == 2 ? 1 : (fft8_first_radix(logN) == 4 ? 2 : 3);
+  for (int lg = l0; lg < logN; lg += 3)
+    for (int r = 1; r < 8; r++)
+      for (int k = 0; k < (1 << lg); k++) {
+        const int m = (r * k) << (logN - 3 - lg);
+        const int e = m < half ? m : m - half;
+        const float sg = m < half ? 1.f : -1.f;
+        t32.push_back(sg * t32[2 * e]);
+        t32.push_back(sg * t32[2 * e + 1]);
+      }
+}
+
 // Twiddles of the compile-time transforms up to 2048 points in registers:
 // each radix-8 pass with span Ls > 1 has at most one butterfly per lane
 // (N / 8 <= 256), which needs w^(r k N / (8 Ls)), r = 1..7, k = lane mod Ls;
 // the forward transform uses them and the inverse their conjugates.  Loaded
-// once from the global half table, they keep the twiddle table out of LDS
-// and its reads out of the passes.
+// once from the pass-order table (fft_append_pass_twiddles), they keep the
+// twiddle table out of LDS and its reads out of the passes.
 template <int LOGN>
 struct RegTwiddles {
   static constexpr int kL0 = fft8_first_radix(LOGN) == 2 ? 1 : (fft8_first_radix(LOGN) == 4 ? 2 : 3);
   static constexpr int kPasses = LOGN > kL0 ? (LOGN - kL0 + 2) / 3 : 0;
   Cplx<float> w[kPasses > 0 ? kPasses : 1][7];
+  // twg: the half table with the pass-order table after it
   __device__ __forceinline__ void load(const Cplx<float>* __restrict__ twg) {
-    constexpr int N = 1 << LOGN;
+    const Cplx<float>* pt = twg + (1 << LOGN) / 2;
+    int off = 0;
 #pragma unroll
     for (int p = 0; p < kPasses; p++) {
       const int lg = kL0 + 3 * p;
       const int k = threadIdx.x & ((1 << lg) - 1);
 #pragma unroll
-      for (int r = 1; r < 8; r++) w[p][r - 1] = twiddle(twg, (r * k) << (LOGN - 3 - lg), N / 2, -1);
+      for (int r = 1; r < 8; r++) w[p][r - 1] = pt[off + ((r - 1) << lg) + k];
+      off += 7 << lg;
     }
   }
 };
@@ -498,7 +533,7 @@ __device__ __forceinline__ void radix_passc(Cplx<T>* z, const Cplx<T>* tw, const
           if constexpr (LOGN <= 11) w = rt->w[(LG - RegTwiddles<LOGN>::kL0) / 3][r - 1];
           else w = tw[OFF + (r - 1) * Ls + k];
           if constexpr (DIR > 0) w.y = -w.y;
-          y[b][r] = cmul(w, y[b][r]);
+          y[b][r] = cmul_fma(w, y[b][r]);
         }
       }
       dft_smallc<DIR, R>(y[b]);

@@ -611,6 +611,7 @@ int filter_upload(FilterPlan& p) {
       t32[2 * k] = static_cast<float>(t64[2 * k]);
       t32[2 * k + 1] = static_cast<float>(t64[2 * k + 1]);
     }
+    dev::fft_append_pass_twiddles(t32, N);
     CTSIM_CHECK(p.d_tw64.reserve(t64.size() * sizeof(double)));
     CTSIM_CHECK(p.d_tw32.reserve(t32.size() * sizeof(float)));
     CTSIM_HIP(hipMemcpy(p.d_tw64.ptr, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));

@@ -59,31 +59,32 @@ __device__ __forceinline__ void store_stream(T* p, const T& v) {
 // through row(j) (any index the rules below use is in [0, n)).
 template <int INTERP, bool LAST_INCL, class Row>
 __device__ __forceinline__ CoefT<INTERP> interval_coef(const Row& row, int i, int n) {
+  // FP32 with fused multiply-adds: the coefficients carry a few units in the
+  // last place of the row values, the order of the rows' own FP32 rounding
   if constexpr (INTERP == INTERP_CUBIC) {
-    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
     if (i == 0) {
-      if (n >= 2) { a0 = row(0); a1 = static_cast<double>(row(1)) - row(0); }
+      if (n >= 2) { a0 = row(0); a1 = row(1) - row(0); }
     } else if (i + 2 > n) {
       // beyond the last full interval: zero
     } else if (i + 2 == n) {
       a0 = row(n - 2);
-      a1 = static_cast<double>(row(n - 1)) - row(n - 2);
+      a1 = row(n - 1) - row(n - 2);
     } else {
-      const double y0 = row(i - 1), y1 = row(i), y2 = row(i + 1), y3 = row(i + 2);
-      const double third = 1.0 / 3, sixth = 1.0 / 6;  // FP64, rounded once to FP32 below
+      const float y0 = row(i - 1), y1 = row(i), y2 = row(i + 1), y3 = row(i + 2);
+      const float third = 1.f / 3, sixth = 1.f / 6;
       a0 = y1;
-      a1 = -y0 * third - y1 * 0.5 + y2 - y3 * sixth;
-      a2 = (y0 + y2) * 0.5 - y1;
-      a3 = (y3 - y0) * sixth + (y1 - y2) * 0.5;
+      a1 = __builtin_fmaf(-third, y0, __builtin_fmaf(-0.5f, y1, __builtin_fmaf(-sixth, y3, y2)));
+      a2 = __builtin_fmaf(0.5f, y0 + y2, -y1);
+      a3 = __builtin_fmaf(sixth, y3 - y0, 0.5f * (y1 - y2));
     }
-    return make_float4(static_cast<float>(a0), static_cast<float>(a1), static_cast<float>(a2),
-                       static_cast<float>(a3));
+    return make_float4(a0, a1, a2, a3);
   } else {
     const int hiOk = LAST_INCL ? n - 1 : n - 2;
     float a0 = 0.f, a1 = 0.f;
     if (i <= hiOk) {
       a0 = row(i);
-      a1 = i < n - 1 ? static_cast<float>(static_cast<double>(row(i + 1)) - row(i)) : 0.f;
+      a1 = i < n - 1 ? row(i + 1) - row(i) : 0.f;
     }
     return make_float2(a0, a1);
   }
