@@ -35,8 +35,7 @@ __device__ __forceinline__ int xcd_tile(int bid, int n) {
 // with its end rules folded in (CubicPolyInterpolator::interpolate,
 // interpolator.cpp:36-75; linear with the IntDiff last-cell rule,
 // backprojectors.cpp:635-638).  coef_kernel writes them once per view, in
-// FP64 rounded to FP32, so the backprojection inner loop is a window lookup
-// and three FMAs.
+// FP32, so the backprojection inner loop is a window lookup and three FMAs.
 template <int INTERP>
 using CoefT = typename std::conditional<INTERP == INTERP_CUBIC, float4, float2>::type;
 
