@@ -512,8 +512,11 @@ __device__ __forceinline__ void dft_smallc(Cplx<T>* a) {
   }
 }
 
-template <int DIR, int R, int LOGN, int LG, int OFF, class T>
+// HALF (the first pass only, Ls = 1): inputs N/2 .. N - 1 are zeros and are
+// not read (a butterfly's inputs r >= R/2 lie there)
+template <int DIR, int R, int LOGN, int LG, int OFF, class T, bool HALF = false>
 __device__ __forceinline__ void radix_passc(Cplx<T>* z, const Cplx<T>* tw, const RegTwiddles<LOGN>* rt) {
+  static_assert(!HALF || LG == 0, "zero upper half: first pass only");
   constexpr int N = 1 << LOGN, Ls = 1 << LG, nb = N / R;
   constexpr int NB = nb >= 256 ? nb / 256 : 1;
   Cplx<T> y[NB][R];
@@ -525,7 +528,7 @@ __device__ __forceinline__ void radix_passc(Cplx<T>* z, const Cplx<T>* tw, const
     if (nb >= 256 || t < nb) {
       const int k = t & (Ls - 1);
 #pragma unroll
-      for (int r = 0; r < R; r++) y[b][r] = z[fft_slot(t + r * nb)];
+      for (int r = 0; r < R; r++) y[b][r] = (HALF && r >= R / 2) ? Cplx<T>{T(0), T(0)} : z[fft_slot(t + r * nb)];
       if constexpr (Ls > 1) {
 #pragma unroll
         for (int r = 1; r < R; r++) {
@@ -560,18 +563,20 @@ __device__ __forceinline__ void radix8_passesc(Cplx<T>* z, const Cplx<T>* tw, co
   }
 }
 
-// tw8: the LDS table (4096 points) or, up to 2048 points, unused (rt)
-template <int DIR, int LOGN, class T>
+// tw8: the LDS table (4096 points) or, up to 2048 points, unused (rt);
+// HALF: the upper half of the input is zero and was not stored
+template <int DIR, int LOGN, class T, bool HALF = false>
 __device__ __forceinline__ void stockham8c(Cplx<T>* z, const Cplx<T>* tw8, const RegTwiddles<LOGN>* rt) {
   constexpr int first = fft8_first_radix(LOGN);
   if constexpr (first == 2) {
-    radix_passc<DIR, 2, LOGN, 0, 0>(z, tw8, rt);
+    radix_passc<DIR, 2, LOGN, 0, 0, T, HALF>(z, tw8, rt);
     radix8_passesc<DIR, LOGN, 1, 0>(z, tw8, rt);
   } else if constexpr (first == 4) {
-    radix_passc<DIR, 4, LOGN, 0, 0>(z, tw8, rt);
+    radix_passc<DIR, 4, LOGN, 0, 0, T, HALF>(z, tw8, rt);
     radix8_passesc<DIR, LOGN, 2, 0>(z, tw8, rt);
   } else {
-    radix8_passesc<DIR, LOGN, 0, 0>(z, tw8, rt);
+    radix_passc<DIR, 8, LOGN, 0, 0, T, HALF>(z, tw8, rt);
+    radix8_passesc<DIR, LOGN, 3, 0>(z, tw8, rt);
   }
 }
 
@@ -612,10 +617,13 @@ __device__ __forceinline__ void fft_filter_pair_c(const float* __restrict__ in, 
   RegTwiddles<LOGN> rt;
   if constexpr (LOGN <= 11) rt.load(twg);
   else pass_twiddles8<float, 16>(twg, tw, N, LOGN);
+  // rows of at most N/2 cells (the usual zero padding to twice the row):
+  // the zero upper half is neither stored nor read by the first pass
+  const bool half = n <= N / 2;
 #pragma unroll
   for (int i = 0; i < IT; i++) {
     const int j = threadIdx.x + i * 256;
-    if (N >= 256 || j < N) {
+    if ((N >= 256 || j < N) && (j < N / 2 || !half)) {
       double da = static_cast<double>(xa[i]), db = static_cast<double>(xb[i]);
       if (w) {
         da *= wj[i];
@@ -625,7 +633,8 @@ __device__ __forceinline__ void fft_filter_pair_c(const float* __restrict__ in, 
     }
   }
   __syncthreads();
-  stockham8c<-1, LOGN>(z, tw, &rt);
+  if (half) stockham8c<-1, LOGN, float, true>(z, tw, &rt);
+  else stockham8c<-1, LOGN>(z, tw, &rt);
 #pragma unroll
   for (int i = 0; i < HT; i++) {
     const int k = threadIdx.x + i * 256;

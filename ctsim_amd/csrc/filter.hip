@@ -273,12 +273,12 @@ __global__ __launch_bounds__(256) void fft_coef_kernel_c(
   dev::fft_filter_pair_c<LOGN>(in, w, hre, him, twg, va, vb, n, z, tw);
   using CT = dev::CoefT<INTERP>;
   CT* c = static_cast<CT*>(coef);
-  auto ra = [z](int j) { return z[dev::fft_slot(j)].x; };
-  auto rb = [z](int j) { return z[dev::fft_slot(j)].y; };
+  auto rz = [z](int j) { return z[dev::fft_slot(j)]; };
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    dev::store_stream(&c[static_cast<size_t>(va) * n + j], dev::interval_coef<INTERP, LAST_INCL>(ra, j, n));
-    if (vb >= 0)
-      dev::store_stream(&c[static_cast<size_t>(vb) * n + j], dev::interval_coef<INTERP, LAST_INCL>(rb, j, n));
+    CT ca, cb;
+    dev::interval_coef2<INTERP, LAST_INCL>(rz, j, n, ca, cb);
+    dev::store_stream(&c[static_cast<size_t>(va) * n + j], ca);
+    if (vb >= 0) dev::store_stream(&c[static_cast<size_t>(vb) * n + j], cb);
   }
 }
 

@@ -89,6 +89,36 @@ __device__ __forceinline__ CoefT<INTERP> interval_coef(const Row& row, int i, in
   }
 }
 
+// Both rows of a view pair at once (row(j).x and row(j).y), each neighbour
+// read once: the same values as interval_coef on each row, with the cases
+// selected instead of branched on.
+template <int INTERP, bool LAST_INCL, class Row2>
+__device__ __forceinline__ void interval_coef2(const Row2& row, int i, int n, CoefT<INTERP>& ca,
+                                               CoefT<INTERP>& cb) {
+  const auto v1 = row(i), v2 = row(i + 1 < n ? i + 1 : n - 1);
+  if constexpr (INTERP == INTERP_CUBIC) {
+    const auto v0 = row(i > 0 ? i - 1 : 0), v3 = row(i + 2 < n ? i + 2 : n - 1);
+    const bool interior = i > 0 && i + 2 < n;
+    const bool edge = (i == 0 && n >= 2) || i + 2 == n;  // a0 = y1, a1 = y2 - y1
+    auto one = [interior, edge](float y0, float y1, float y2, float y3) {
+      const float third = 1.f / 3, sixth = 1.f / 6;
+      const float b1 = __builtin_fmaf(-third, y0, __builtin_fmaf(-0.5f, y1, __builtin_fmaf(-sixth, y3, y2)));
+      const float b2 = __builtin_fmaf(0.5f, y0 + y2, -y1);
+      const float b3 = __builtin_fmaf(sixth, y3 - y0, 0.5f * (y1 - y2));
+      const float d = y2 - y1;
+      return make_float4(interior || edge ? y1 : 0.f, interior ? b1 : (edge ? d : 0.f), interior ? b2 : 0.f,
+                         interior ? b3 : 0.f);
+    };
+    ca = one(v0.x, v1.x, v2.x, v3.x);
+    cb = one(v0.y, v1.y, v2.y, v3.y);
+  } else {
+    const int hiOk = LAST_INCL ? n - 1 : n - 2;
+    const bool ok = i <= hiOk, diff = ok && i < n - 1;
+    ca = make_float2(ok ? v1.x : 0.f, diff ? v2.x - v1.x : 0.f);
+    cb = make_float2(ok ? v1.y : 0.f, diff ? v2.y - v1.y : 0.f);
+  }
+}
+
 struct MixedArgs {
   int nx, ny, nDet, nView, lanesY, tilesX, tilesY, W, chunk, split, viewsPerGroup;
   float center, F, twoF, posScale, endScale;
