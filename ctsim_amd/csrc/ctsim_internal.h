@@ -87,6 +87,12 @@ struct ctsim_ctx {
   ctsim::DevBuf scratch;      // kernel scratch (partials, counters)
   ctsim::DevBuf scratch2;     // interval coefficients of the mixed path
   ctsim::DevBuf scratch3;     // filtered views of ctsim_fbp_slice
+  // helical interpolation and half-scan feathering: the detector angles of
+  // the last scan (rebuilt when geometry, nDet, detInc or SDL change) and the
+  // feather's error word
+  ctsim::DevBuf helixGam, helixErr;
+  int helixGeom = -1, helixNDet = 0;
+  double helixDetInc = 0, helixSdl = 0;
   int numCUs = 256;
   int activate() const;       // hipSetDevice(device)
   // Kernel timing: while enabled, the main backprojection kernel of each

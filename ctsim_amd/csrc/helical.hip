@@ -160,15 +160,29 @@ __global__ __launch_bounds__(256) void feather_kernel(FeatherArgs a) {
   a.out[k] = v;
 }
 
-int gamma_table(ctsim_ctx* ctx, int geom, int nDet, double detInc, double sdl, DevBuf& d) {
-  std::vector<double> g(nDet);
-  for (int i = 0; i < nDet; i++)
-    g[i] = geom == GEOM_EQUILINEAR ? atan((i - (nDet - 1) / 2) * detInc / sdl)
-                                   : (i - (nDet - 1) / 2) * detInc;
-  CTSIM_CHECK(d.reserve(nDet * sizeof(double)));
-  CTSIM_HIP(hipMemcpyAsync(d.ptr, g.data(), nDet * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-  // the host vector dies at return: wait for the copy
-  CTSIM_HIP(hipStreamSynchronize(ctx->stream));
+// The detector angles of a scan, kept in the context: consecutive calls on
+// one scan (a slice position after another) upload them once.
+int gamma_table(ctsim_ctx* ctx, int geom, int nDet, double detInc, double sdl, const double** out) {
+  if (!(ctx->helixGam.ptr && ctx->helixGeom == geom && ctx->helixNDet == nDet && ctx->helixDetInc == detInc &&
+        ctx->helixSdl == sdl)) {
+    ctx->helixGeom = -1;  // invalid until the upload below has finished
+    std::vector<double> g(nDet);
+    for (int i = 0; i < nDet; i++)
+      g[i] = geom == GEOM_EQUILINEAR ? atan((i - (nDet - 1) / 2) * detInc / sdl)
+                                     : (i - (nDet - 1) / 2) * detInc;
+    // earlier launches on the stream may still read the old table
+    CTSIM_HIP(hipStreamSynchronize(ctx->stream));
+    CTSIM_CHECK(ctx->helixGam.reserve(nDet * sizeof(double)));
+    CTSIM_HIP(hipMemcpyAsync(ctx->helixGam.ptr, g.data(), nDet * sizeof(double), hipMemcpyHostToDevice,
+                             ctx->stream));
+    // the host vector dies at return: wait for the copy
+    CTSIM_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->helixGeom = geom;
+    ctx->helixNDet = nDet;
+    ctx->helixDetInc = detInc;
+    ctx->helixSdl = sdl;
+  }
+  *out = ctx->helixGam.as<double>();
   return CTSIM_OK;
 }
 
@@ -205,14 +219,13 @@ int ctsim_helical180li(ctsim_ctx* ctx, int geometry, int nDet, int nView, double
   const int L = static_cast<int>((PI + fan) / dbeta);
   if (L + 1 > outCapacity) return fail(CTSIM_ERR_INVALID, "ctsim_helical180li: output too small");
   CTSIM_CHECK(ctx->activate());
-  DevBuf gam;
-  CTSIM_CHECK(gamma_table(ctx, geometry, nDet, detInc, sourceDetectorLength, gam));
-  HelixArgs a{nDet, nView, L, offsetView, dbeta, fan, gam.as<double>(), d_in, d_out};
+  const double* gam = nullptr;
+  CTSIM_CHECK(gamma_table(ctx, geometry, nDet, detInc, sourceDetectorLength, &gam));
+  HelixArgs a{nDet, nView, L, offsetView, dbeta, fan, gam, d_in, d_out};
   const size_t n = static_cast<size_t>(L + 1) * nDet;
   hipLaunchKernelGGL(helical_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
                      ctx->stream, a);
   CTSIM_HIP(hipGetLastError());
-  CTSIM_HIP(hipStreamSynchronize(ctx->stream));  // gam is released at return
   *nViewOut = L + 1;
   if (h_anglesOut)
     for (int i = 0; i <= L; i++) h_anglesOut[i] = (i + offsetView) * dbeta;
@@ -230,19 +243,19 @@ int ctsim_halfscan_feather(ctsim_ctx* ctx, int geometry, int nDet, int nView, do
   if (nView != static_cast<int>((PI + fanBeamAngle) / rotInc) + 1)
     return fail(CTSIM_ERR_INVALID, "Data set does seem have enough data to be a halfscan data set");
   CTSIM_CHECK(ctx->activate());
-  DevBuf gam, err;
-  CTSIM_CHECK(gamma_table(ctx, geometry, nDet, detInc, sourceDetectorLength, gam));
-  CTSIM_CHECK(err.reserve(sizeof(int)));
+  const double* gam = nullptr;
+  CTSIM_CHECK(gamma_table(ctx, geometry, nDet, detInc, sourceDetectorLength, &gam));
+  CTSIM_CHECK(ctx->helixErr.reserve(sizeof(int)));
   int zero = 0;
-  CTSIM_HIP(hipMemcpyAsync(err.ptr, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-  FeatherArgs a{nDet, nView, rotInc, fanBeamAngle, nView * rotInc / PI, gam.as<double>(), d_in, d_out,
-                err.as<int>()};
+  CTSIM_HIP(hipMemcpyAsync(ctx->helixErr.ptr, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  FeatherArgs a{nDet, nView, rotInc, fanBeamAngle, nView * rotInc / PI, gam, d_in, d_out,
+                ctx->helixErr.as<int>()};
   const size_t n = static_cast<size_t>(nView) * nDet;
   hipLaunchKernelGGL(feather_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
                      ctx->stream, a);
   CTSIM_HIP(hipGetLastError());
   int flag = 0;
-  CTSIM_HIP(hipMemcpyAsync(&flag, err.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  CTSIM_HIP(hipMemcpyAsync(&flag, ctx->helixErr.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   CTSIM_HIP(hipStreamSynchronize(ctx->stream));
   if (flag) return fail(CTSIM_ERR_INVALID, "HalfScanFeather: sample outside every weighting region");
   return CTSIM_OK;

@@ -58,3 +58,20 @@ def test_helical_rejects_short_scans(gpu_ctx):
         _run(gpu_ctx, 1, data, 2 * np.pi / 100, 0.01, np.pi / 3, 4.0, -1, False)
     with pytest.raises(ctsim_amd.CtsimError):
         _run(gpu_ctx, 0, data, 2 * np.pi / 100, 0.01, np.pi / 3, 4.0, -1, False)
+
+
+def test_helical_alternating_scans(gpu_ctx, oracle_lib):
+    """Two scans that differ only in detector spacing, alternated on one
+    context (its cached detector angles are rebuilt whenever the scan
+    changes): every call matches the oracle."""
+    rng = np.random.default_rng(77)
+    nDet, nView, turns, fan, sdl = 96, 600, 1.7, np.pi / 3, 4.0
+    data = rng.standard_normal((nView, nDet)).astype(np.float32)
+    rotInc = 2 * np.pi * turns / nView
+    base = 2 * sdl * np.tan(fan / 2) / nDet
+    for k, detInc in enumerate((base, 0.9 * base, base, 0.9 * base)):
+        print(f"case alternating scan {k} detInc={detInc:.6g}", flush=True)
+        ref, rang = oracle_lib.helical180li(2, data, rotInc, detInc, fan, sdl, -1, True)
+        out, ang = _run(gpu_ctx, 2, data, rotInc, detInc, fan, sdl, -1, True)
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+        assert np.array_equal(ang, rang)
