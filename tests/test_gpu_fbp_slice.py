@@ -18,6 +18,11 @@ CASES = [  # geometry, nDet, nView, nx, interp, backproj, rotation, filter, meth
     ("equiangular", 96, 64, 48, "linear", "table", 2 * np.pi, "abs_bandlimit", "convolution"),
     ("parallel", 200, 180, 96, "nearest", "idiff", np.pi, "abs_bandlimit", "fft"),
     ("parallel", 160, 120, 64, "linear", "diff", np.pi, "abs_hanning", "fft"),
+    # IntDiff's linear rule with its last cell included, through the FFT
+    # coefficient kernel; 255 cells in 256-point transforms (more than N/2:
+    # the first pass reads the whole row; 512 cells in 1024 points above skip
+    # the zero half)
+    ("parallel", 255, 200, 96, "linear", "idiff", np.pi, "abs_hamming", "fft"),
     # a fan-beam geometry runs its own backprojector whatever the name says
     ("equilinear", 160, 120, 64, "linear", "diff", 2 * np.pi, "abs_hanning", "fft"),
 ]
